@@ -1,0 +1,752 @@
+// dense_ops.hip -- per-timestep dense operations on the staggered fields (libfpc).
+//
+// Replaces gpu_dense_ops.hpp of the reference (JD63021/ibm-3d-fpc).  Layer 1
+// restates each reference kernel with the same argument meaning; layer 2 fuses
+// the pack -> CSR -> unpack sequences of main.cu:262-350 into structured MAC
+// stencils.  The structured D and G sum their terms in the CSR column order
+// (west, east, south, north, bottom, top for D; left then right for G), so the
+// results equal the CSR path's.
+//
+// Arithmetic note: device code is compiled with FP contraction, so a*b + c
+// becomes one fused multiply-add, as nvcc does for the reference kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "fpc_dense.h"
+#include "fpc_internal.hpp"
+
+namespace fpc {
+namespace {
+
+typedef unsigned char u8;
+
+__device__ inline long long iU(int i, int j, int k, int nx, int ny) { return ((long long)k * ny + j) * (nx + 1) + i; }
+__device__ inline long long iV(int i, int j, int k, int nx, int ny) { return ((long long)k * (ny + 1) + j) * nx + i; }
+__device__ inline long long iW(int i, int j, int k, int nx, int ny) { return ((long long)k * ny + j) * nx + i; }
+__device__ inline long long iP(int i, int j, int k, int nx, int ny) { return ((long long)k * ny + j) * nx + i; }
+
+// Out-of-box reads return the fallback (gpu_dense_ops.hpp:1-3).
+__device__ inline double gu(const double *u, int i, int j, int k, int nx, int ny, int nz, double fb) {
+  return (i < 0 || i > nx || j < 0 || j >= ny || k < 0 || k >= nz) ? fb : u[iU(i, j, k, nx, ny)];
+}
+__device__ inline double gv(const double *v, int i, int j, int k, int nx, int ny, int nz, double fb) {
+  return (i < 0 || i >= nx || j < 0 || j > ny || k < 0 || k >= nz) ? fb : v[iV(i, j, k, nx, ny)];
+}
+__device__ inline double gw(const double *w, int i, int j, int k, int nx, int ny, int nz, double fb) {
+  return (i < 0 || i >= nx || j < 0 || j >= ny || k < 0 || k > nz) ? fb : w[iW(i, j, k, nx, ny)];
+}
+
+__device__ inline double conv_u_at(const double *u, const double *v, const double *w, int i, int j, int k, int nx,
+                                   int ny, int nz, double dx, double dy, double dz) {
+  double uc = u[iU(i, j, k, nx, ny)];
+  double va = 0.25 * (gv(v, i - 1, j, k, nx, ny, nz, 0.0) + gv(v, i - 1, j + 1, k, nx, ny, nz, 0.0) +
+                      gv(v, i, j, k, nx, ny, nz, 0.0) + gv(v, i, j + 1, k, nx, ny, nz, 0.0));
+  double wa = 0.25 * (gw(w, i - 1, j, k, nx, ny, nz, 0.0) + gw(w, i - 1, j, k + 1, nx, ny, nz, 0.0) +
+                      gw(w, i, j, k, nx, ny, nz, 0.0) + gw(w, i, j, k + 1, nx, ny, nz, 0.0));
+  double ue = gu(u, i + 1, j, k, nx, ny, nz, uc), uw = gu(u, i - 1, j, k, nx, ny, nz, uc);
+  double un = gu(u, i, j + 1, k, nx, ny, nz, uc), us = gu(u, i, j - 1, k, nx, ny, nz, uc);
+  double ut = gu(u, i, j, k + 1, nx, ny, nz, uc), ub = gu(u, i, j, k - 1, nx, ny, nz, uc);
+  return uc * (ue - uw) / (2 * dx) + va * (un - us) / (2 * dy) + wa * (ut - ub) / (2 * dz);
+}
+
+__device__ inline double conv_v_at(const double *u, const double *v, const double *w, int i, int j, int k, int nx,
+                                   int ny, int nz, double dx, double dy, double dz) {
+  double vc = v[iV(i, j, k, nx, ny)];
+  double ua = 0.25 * (gu(u, i, j - 1, k, nx, ny, nz, 0.0) + gu(u, i + 1, j - 1, k, nx, ny, nz, 0.0) +
+                      gu(u, i, j, k, nx, ny, nz, 0.0) + gu(u, i + 1, j, k, nx, ny, nz, 0.0));
+  double wa = 0.25 * (gw(w, i, j - 1, k, nx, ny, nz, 0.0) + gw(w, i, j, k, nx, ny, nz, 0.0) +
+                      gw(w, i, j - 1, k + 1, nx, ny, nz, 0.0) + gw(w, i, j, k + 1, nx, ny, nz, 0.0));
+  double ve = gv(v, i + 1, j, k, nx, ny, nz, vc), vw = gv(v, i - 1, j, k, nx, ny, nz, vc);
+  double vn = gv(v, i, j + 1, k, nx, ny, nz, vc), vs = gv(v, i, j - 1, k, nx, ny, nz, vc);
+  double vt = gv(v, i, j, k + 1, nx, ny, nz, vc), vb = gv(v, i, j, k - 1, nx, ny, nz, vc);
+  return ua * (ve - vw) / (2 * dx) + vc * (vn - vs) / (2 * dy) + wa * (vt - vb) / (2 * dz);
+}
+
+__device__ inline double conv_w_at(const double *u, const double *v, const double *w, int i, int j, int k, int nx,
+                                   int ny, int nz, double dx, double dy, double dz) {
+  double wc = w[iW(i, j, k, nx, ny)];
+  double ua = 0.25 * (gu(u, i, j, k - 1, nx, ny, nz, 0.0) + gu(u, i + 1, j, k - 1, nx, ny, nz, 0.0) +
+                      gu(u, i, j, k, nx, ny, nz, 0.0) + gu(u, i + 1, j, k, nx, ny, nz, 0.0));
+  double va = 0.25 * (gv(v, i, j, k - 1, nx, ny, nz, 0.0) + gv(v, i, j + 1, k - 1, nx, ny, nz, 0.0) +
+                      gv(v, i, j, k, nx, ny, nz, 0.0) + gv(v, i, j + 1, k, nx, ny, nz, 0.0));
+  double we = gw(w, i + 1, j, k, nx, ny, nz, wc), ww = gw(w, i - 1, j, k, nx, ny, nz, wc);
+  double wn = gw(w, i, j + 1, k, nx, ny, nz, wc), ws = gw(w, i, j - 1, k, nx, ny, nz, wc);
+  double wt = gw(w, i, j, k + 1, nx, ny, nz, wc), wb = gw(w, i, j, k - 1, nx, ny, nz, wc);
+  return ua * (we - ww) / (2 * dx) + va * (wn - ws) / (2 * dy) + wc * (wt - wb) / (2 * dz);
+}
+
+// G p at a u face (the CSR row of immersed_operators.hpp:252-266: left then right term).
+__device__ inline double gradx_at(const double *p, const u8 *fluid, int i, int j, int k, int nx, int ny, double dx) {
+  double s = 0.0;
+  if (i == nx) {
+    long long L = iP(nx - 1, j, k, nx, ny);
+    if (fluid[L]) s += (-2.0 / dx) * p[L];
+    return s;
+  }
+  long long L = iP(i - 1, j, k, nx, ny), R = iP(i, j, k, nx, ny);
+  if (fluid[L]) s += (-1.0 / dx) * p[L];
+  if (fluid[R]) s += (1.0 / dx) * p[R];
+  return s;
+}
+__device__ inline double grady_at(const double *p, const u8 *fluid, int i, int j, int k, int nx, int ny, double dy) {
+  double s = 0.0;
+  long long S = iP(i, j - 1, k, nx, ny), N = iP(i, j, k, nx, ny);
+  if (fluid[S]) s += (-1.0 / dy) * p[S];
+  if (fluid[N]) s += (1.0 / dy) * p[N];
+  return s;
+}
+__device__ inline double gradz_at(const double *p, const u8 *fluid, int i, int j, int k, int nx, int ny, double dz) {
+  double s = 0.0;
+  long long B = iP(i, j, k - 1, nx, ny), T = iP(i, j, k, nx, ny);
+  if (fluid[B]) s += (-1.0 / dz) * p[B];
+  if (fluid[T]) s += (1.0 / dz) * p[T];
+  return s;
+}
+
+// D q at a fluid cell in CSR column order; qout supplies the outflow faces when non-null.
+__device__ inline double div_at(const fpc_flow_ops &o, const double *u, const double *v, const double *w,
+                                const double *qout, int i, int j, int k) {
+  int nx = o.nx, ny = o.ny, nz = o.nz;
+  const double *c = o.dcoef + 6 * ((long long)j * nx + i);
+  double s = 0.0;
+  if (i >= 1 && c[0] != 0.0) s += c[0] * u[iU(i, j, k, nx, ny)];
+  if (c[1] != 0.0) {
+    double qe = (qout && i == nx - 1) ? qout[(long long)j + (long long)ny * k] : u[iU(i + 1, j, k, nx, ny)];
+    s += c[1] * qe;
+  }
+  if (j >= 1 && c[2] != 0.0) s += c[2] * v[iV(i, j, k, nx, ny)];
+  if (j <= ny - 2 && c[3] != 0.0) s += c[3] * v[iV(i, j + 1, k, nx, ny)];
+  if (k >= 1 && c[4] != 0.0) s += c[4] * w[iW(i, j, k, nx, ny)];
+  if (k <= nz - 2 && c[5] != 0.0) s += c[5] * w[iW(i, j, k + 1, nx, ny)];
+  return s;
+}
+
+#define TID_1D long long tid = blockIdx.x * (long long)blockDim.x + threadIdx.x
+
+// ---------------------------------------------------------------- layer 1 kernels
+__global__ void k_bc(double *u, double *v, double *w, const u8 *au, const u8 *av, const u8 *aw, const double *uin,
+                     int nx, int ny, int nz) {
+  TID_1D;
+  long long Nu = (long long)(nx + 1) * ny * nz, Nv = (long long)nx * (ny + 1) * nz, Nw = (long long)nx * ny * (nz + 1);
+  if (tid < Nu) {
+    long long pl = (long long)(nx + 1) * ny;
+    int k = (int)(tid / pl);
+    long long rem = tid - k * pl;
+    int j = (int)(rem / (nx + 1));
+    int i = (int)(rem - (long long)j * (nx + 1));
+    if (i == 0) u[tid] = uin[j + (long long)ny * k];
+    else if (i == nx) {
+      long long s = iU(nx - 1, j, k, nx, ny);
+      u[tid] = (nx - 1 == 0) ? uin[j + (long long)ny * k] : (au[s] ? u[s] : 0.0);
+    } else if (!au[tid]) u[tid] = 0.0;
+  }
+  if (tid < Nv) {
+    long long pl = (long long)nx * (ny + 1);
+    int k = (int)(tid / pl);
+    long long rem = tid - k * pl;
+    int j = (int)(rem / nx);
+    if (j == 0 || j == ny || !av[tid]) v[tid] = 0.0;
+  }
+  if (tid < Nw) {
+    long long pl = (long long)nx * ny;
+    int k = (int)(tid / pl);
+    long long rem = tid - k * pl;
+    int j = (int)(rem / nx);
+    int i = (int)(rem - (long long)j * nx);
+    if (k == 0 || k == nz) w[tid] = 0.0;
+    else if (!aw[iW(i, j, k - 1, nx, ny)]) w[tid] = 0.0;  // the reference tests active_w[k-1]
+  }
+}
+
+__global__ void k_conv(int comp, double *conv, const double *u, const double *v, const double *w, const u8 *act, int nx,
+                       int ny, int nz, double dx, double dy, double dz) {
+  TID_1D;
+  int sx = nx + (comp == 0), sy = ny + (comp == 1), sz = nz + (comp == 2);
+  long long N = (long long)sx * sy * sz;
+  if (tid >= N) return;
+  long long pl = (long long)sx * sy;
+  int k = (int)(tid / pl);
+  long long rem = tid - k * pl;
+  int j = (int)(rem / sx);
+  int i = (int)(rem - (long long)j * sx);
+  conv[tid] = 0.0;
+  if (comp == 0) {
+    if (i < 1 || i > nx - 1 || !act[tid]) return;
+    conv[tid] = conv_u_at(u, v, w, i, j, k, nx, ny, nz, dx, dy, dz);
+  } else if (comp == 1) {
+    if (j < 1 || j > ny - 1 || !act[tid]) return;
+    conv[tid] = conv_v_at(u, v, w, i, j, k, nx, ny, nz, dx, dy, dz);
+  } else {
+    if (k < 1 || k > nz - 1 || !act[iW(i, j, k - 1, nx, ny)]) return;
+    conv[tid] = conv_w_at(u, v, w, i, j, k, nx, ny, nz, dx, dy, dz);
+  }
+}
+
+__global__ void k_build_full(int comp, double *rhsf, double *x0, const double *f, const double *conv,
+                             const double *gp, const u8 *act, const int *id, const double *bc, const double *scale,
+                             int nx, int ny, int nz, double rho, double dt, int zero) {
+  TID_1D;
+  int bx = nx - (comp == 0), by = ny - (comp == 1), bz = nz - (comp == 2);
+  long long N = (long long)bx * by * bz;
+  if (tid >= N) return;
+  long long pl = (long long)bx * by;
+  int k = (int)(tid / pl);
+  long long rem = tid - k * pl;
+  int j = (int)(rem / bx);
+  int i = (int)(rem - (long long)j * bx);
+  long long idx = comp == 0 ? iU(i + 1, j, k, nx, ny) : (comp == 1 ? iV(i, j + 1, k, nx, ny) : iW(i, j, k + 1, nx, ny));
+  int aid = id[idx];
+  double r = 0.0;
+  if (aid > 0 && act[idx]) r = ((rho / dt) * f[idx] - rho * conv[idx] - gp[idx] + bc[aid]) * scale[aid];
+  rhsf[tid] = r;
+  x0[tid] = zero ? 0.0 : f[idx];
+}
+
+__global__ void k_scatter(int comp, double *o, const double *x, const double *base, int nx, int ny, int nz) {
+  TID_1D;
+  int sx = nx + (comp == 0), sy = ny + (comp == 1), sz = nz + (comp == 2);
+  long long N = (long long)sx * sy * sz;
+  if (tid >= N) return;
+  if (base) o[tid] = base[tid];
+  long long pl = (long long)sx * sy;
+  int k = (int)(tid / pl);
+  long long rem = tid - k * pl;
+  int j = (int)(rem / sx);
+  int i = (int)(rem - (long long)j * sx);
+  if (comp == 0 && i >= 1 && i <= nx - 1) o[tid] = x[((long long)k * ny + j) * (nx - 1) + (i - 1)];
+  if (comp == 1 && j >= 1 && j <= ny - 1) o[tid] = x[((long long)k * (ny - 1) + (j - 1)) * nx + i];
+  if (comp == 2 && k >= 1 && k <= nz - 1) o[tid] = x[((long long)(k - 1) * ny + j) * nx + i];
+}
+
+__global__ void k_csr(int nrows, const int *rowptr, const int *colind, const double *vals, const double *x, double *y) {
+  TID_1D;
+  if (tid >= nrows) return;
+  double s = 0.0;
+  for (int jj = rowptr[tid]; jj < rowptr[tid + 1]; ++jj) s += vals[jj] * x[colind[jj]];
+  y[tid + 1] = s;
+}
+
+__global__ void k_pack_masked(double *vec, const double *p, const u8 *mask, const int *id, long long N) {
+  TID_1D;
+  if (tid >= N) return;
+  if (mask[tid]) {
+    int a = id[tid];
+    if (a > 0) vec[a] = p[tid];
+  }
+}
+
+__global__ void k_unpack_gp(const double *gp, double *out, const int *id, long long N, long long off) {
+  TID_1D;
+  if (tid >= N) return;
+  int a = id[tid];
+  out[tid] = a > 0 ? gp[off + a] : 0.0;
+}
+
+__global__ void k_pack_q(double *q, const double *f, const int *id, long long N, long long off) {
+  TID_1D;
+  if (tid >= N) return;
+  int a = id[tid];
+  if (a > 0) q[off + a] = f[tid];
+}
+
+__global__ void k_unpack_q(const double *q, double *f, const int *id, long long N, long long off) {
+  TID_1D;
+  if (tid >= N) return;
+  int a = id[tid];
+  if (a > 0) f[tid] = q[off + a];
+}
+
+__global__ void k_prhs(double *rhsf, double *x0, const double *Dqt, const double *dbc, const u8 *fluid, const int *id,
+                       long long N, double rho, double dt, double ramp) {
+  TID_1D;
+  if (tid >= N) return;
+  double r = 0.0;
+  if (fluid[tid]) {
+    int a = id[tid];
+    if (a > 0) r = -(rho / dt) * (Dqt[a] + ramp * dbc[a]);
+  }
+  rhsf[tid] = r;
+  x0[tid] = 0.0;
+}
+
+__global__ void k_qcorr(double *q, const double *g, double f, long long n1) {
+  TID_1D;
+  if (tid >= n1) return;
+  if (tid > 0) q[tid] -= f * g[tid];
+}
+
+__global__ void k_add_phi(double *p, const double *phi, const u8 *fluid, const int *id, long long N) {
+  TID_1D;
+  if (tid >= N) return;
+  if (fluid[tid]) {
+    int a = id[tid];
+    if (a > 0) p[tid] += phi[a];
+  }
+}
+
+__global__ void k_zero(double *p, long long n) {
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) p[t] = 0.0;
+}
+
+// ---------------------------------------------------------------- layer 2 kernels
+// Momentum rhs and x0 on a u/v/w box row, one thread per row.
+__global__ void k_explicit(int comp, fpc_flow_ops o, const double *u, const double *v, const double *w, const double *p,
+                           double *rhs, double *x0, int zero) {
+  int nx = o.nx, ny = o.ny, nz = o.nz;
+  int bx = nx - (comp == 0), by = ny - (comp == 1), bz = nz - (comp == 2);
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int k = blockIdx.z;
+  if (i >= bx || j >= by || k >= bz) return;
+  long long row = ((long long)k * by + j) * bx + i;
+  double rr = 0.0, xv;
+  double at = o.rho / o.dt;
+  if (comp == 0) {
+    int fi = i + 1;
+    long long t = iU(fi, j, k, nx, ny);
+    double f = u[t];
+    xv = f;
+    if (o.active_u[t]) {
+      double cv = conv_u_at(u, v, w, fi, j, k, nx, ny, nz, o.dx, o.dy, o.dz);
+      double g = o.active_qu[t] ? gradx_at(p, o.fluid_p, fi, j, k, nx, ny, o.dx) : 0.0;
+      rr = (at * f - o.rho * cv - g + o.rhs_u_bc[row]) * o.scale_u[row];
+    }
+  } else if (comp == 1) {
+    int fj = j + 1;
+    long long t = iV(i, fj, k, nx, ny);
+    double f = v[t];
+    xv = f;
+    if (o.active_v[t]) {
+      double cv = conv_v_at(u, v, w, i, fj, k, nx, ny, nz, o.dx, o.dy, o.dz);
+      double g = o.active_qv[t] ? grady_at(p, o.fluid_p, i, fj, k, nx, ny, o.dy) : 0.0;
+      rr = (at * f - o.rho * cv - g + 0.0) * o.scale_v[row];
+    }
+  } else {
+    int fk = k + 1;
+    long long t = iW(i, j, fk, nx, ny);
+    double f = w[t];
+    xv = f;
+    if (o.active_w[t]) {
+      double cv = o.active_w[iW(i, j, fk - 1, nx, ny)] ? conv_w_at(u, v, w, i, j, fk, nx, ny, nz, o.dx, o.dy, o.dz) : 0.0;
+      double g = o.active_qw[t] ? gradz_at(p, o.fluid_p, i, j, fk, nx, ny, o.dz) : 0.0;
+      rr = (at * f - o.rho * cv - g + 0.0) * o.scale_w[row];
+    }
+  }
+  rhs[row] = rr;
+  x0[row] = zero ? 0.0 : xv;
+}
+
+// Pressure rhs on every cell of the p box.
+__global__ void k_divergence(fpc_flow_ops o, const double *ut, const double *vt, const double *wt, double ramp,
+                             double *rhs, double *x0) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int k = blockIdx.z;
+  if (i >= o.nx || j >= o.ny) return;
+  long long c = iP(i, j, k, o.nx, o.ny);
+  double r = 0.0;
+  if (o.fluid_p[c]) {
+    double s = div_at(o, ut, vt, wt, nullptr, i, j, k);
+    double bc = (i == 0) ? o.div_bc[(long long)j + (long long)o.ny * k] : 0.0;
+    r = -(o.rho / o.dt) * (s + ramp * bc);
+  }
+  rhs[c] = r;
+  x0[c] = 0.0;
+}
+
+// Projection of the face velocities; thread t covers face t of each of u, v, w.
+__global__ void k_project(fpc_flow_ops o, const double *phi, const double *ut, const double *vt, const double *wt,
+                          double *u, double *v, double *w, double *qout) {
+  TID_1D;
+  int nx = o.nx, ny = o.ny, nz = o.nz;
+  double f = o.dt / o.rho;
+  long long Nu = (long long)(nx + 1) * ny * nz, Nv = (long long)nx * (ny + 1) * nz, Nw = (long long)nx * ny * (nz + 1);
+  if (tid < Nu) {
+    long long pl = (long long)(nx + 1) * ny;
+    int k = (int)(tid / pl);
+    long long rem = tid - k * pl;
+    int j = (int)(rem / (nx + 1));
+    int i = (int)(rem - (long long)j * (nx + 1));
+    double val = ut[tid];
+    if (o.active_qu[tid] && i >= 1) {
+      val = val - f * gradx_at(phi, o.fluid_p, i, j, k, nx, ny, o.dx);
+      if (i == nx) qout[(long long)j + (long long)ny * k] = val;
+    }
+    u[tid] = val;
+  }
+  if (tid < Nv) {
+    long long pl = (long long)nx * (ny + 1);
+    int k = (int)(tid / pl);
+    long long rem = tid - k * pl;
+    int j = (int)(rem / nx);
+    int i = (int)(rem - (long long)j * nx);
+    double val = vt[tid];
+    if (o.active_qv[tid] && j >= 1 && j <= ny - 1) val = val - f * grady_at(phi, o.fluid_p, i, j, k, nx, ny, o.dy);
+    v[tid] = val;
+  }
+  if (tid < Nw) {
+    long long pl = (long long)nx * ny;
+    int k = (int)(tid / pl);
+    long long rem = tid - k * pl;
+    int j = (int)(rem / nx);
+    int i = (int)(rem - (long long)j * nx);
+    double val = wt[tid];
+    if (o.active_qw[tid] && k >= 1 && k <= nz - 1) val = val - f * gradz_at(phi, o.fluid_p, i, j, k, nx, ny, o.dz);
+    w[tid] = val;
+  }
+}
+
+__global__ void k_add_phi_box(double *p, const double *phi, const u8 *fluid, long long N) {
+  TID_1D;
+  if (tid >= N) return;
+  if (fluid[tid]) p[tid] += phi[tid];
+}
+
+// Partial diagnostics per block: sum of squared divergence, max |u|,|v|,|w|, forces.
+__global__ void __launch_bounds__(256) k_diag_partials(fpc_flow_ops o, const double *u, const double *v, const double *w,
+                                                       const double *p, const double *qout, double ramp,
+                                                       double *part) {
+  __shared__ double lds[8];
+  int nx = o.nx, ny = o.ny, nz = o.nz;
+  long long Np = (long long)nx * ny * nz;
+  long long Nu = (long long)(nx + 1) * ny * nz, Nv = (long long)nx * (ny + 1) * nz, Nw = (long long)nx * ny * (nz + 1);
+  long long stride = (long long)gridDim.x * blockDim.x, t0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  double d2 = 0.0, mu = 0.0, mv = 0.0, mw = 0.0, fx = 0.0, fy = 0.0;
+  for (long long c = t0; c < Np; c += stride) {
+    if (!o.fluid_p[c]) continue;
+    int k = (int)(c / ((long long)nx * ny));
+    long long rem = c - (long long)k * nx * ny;
+    int j = (int)(rem / nx);
+    int i = (int)(rem - (long long)j * nx);
+    double s = div_at(o, u, v, w, qout, i, j, k);
+    double bc = (i == 0) ? o.div_bc[(long long)j + (long long)ny * k] : 0.0;
+    double d = s + ramp * bc;
+    d2 += d * d;
+  }
+  for (long long t = t0; t < Nu; t += stride) mu = fmax(mu, fabs(u[t]));
+  for (long long t = t0; t < Nv; t += stride) mv = fmax(mv, fabs(v[t]));
+  for (long long t = t0; t < Nw; t += stride) mw = fmax(mw, fabs(w[t]));
+  for (long long t = t0; t < o.n_fu; t += stride) fx += o.fu_w[t] * u[o.fu_idx[t]];
+  for (long long t = t0; t < o.n_px; t += stride) fx += o.px_w[t] * p[o.px_idx[t]];
+  for (long long t = t0; t < o.n_fv; t += stride) fy += o.fv_w[t] * v[o.fv_idx[t]];
+  for (long long t = t0; t < o.n_py; t += stride) fy += o.py_w[t] * p[o.py_idx[t]];
+  double vals[6] = {d2, mu, mv, mw, fx, fy};
+  for (int q = 0; q < 6; ++q) {
+    double x = vals[q];
+    bool isMax = (q >= 1 && q <= 3);
+    for (int off = 32; off > 0; off >>= 1) {
+      double y = __shfl_xor(x, off, 64);
+      x = isMax ? fmax(x, y) : x + y;
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double r = lds[0];
+      for (int wv = 1; wv < 4; ++wv) r = isMax ? fmax(r, lds[wv]) : r + lds[wv];
+      part[(long long)q * gridDim.x + blockIdx.x] = r;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_diag_final(const double *part, int nb, int Np, double *out) {
+  __shared__ double lds[16];
+  for (int q = 0; q < 6; ++q) {
+    bool isMax = (q >= 1 && q <= 3);
+    double x = isMax ? 0.0 : 0.0;
+    for (int t = threadIdx.x; t < nb; t += blockDim.x) x = isMax ? fmax(x, part[(long long)q * nb + t]) : x + part[(long long)q * nb + t];
+    for (int off = 32; off > 0; off >>= 1) {
+      double y = __shfl_xor(x, off, 64);
+      x = isMax ? fmax(x, y) : x + y;
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double r = lds[0];
+      for (int wv = 1; wv < (int)(blockDim.x >> 6); ++wv) r = isMax ? fmax(r, lds[wv]) : r + lds[wv];
+      if (q == 0) r = sqrt(r / (double)(Np > 1 ? Np : 1));
+      out[q] = r;
+    }
+  }
+}
+
+__global__ void k_nonfinite(const double *x, long long n, int *flag) {
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+    if (!isfinite(x[t])) *flag = 1;
+}
+
+unsigned nb1(long long n) { return (unsigned)std::max(1LL, (n + 255) / 256); }
+
+}  // namespace
+}  // namespace fpc
+
+using namespace fpc;
+
+#define LAUNCH1(n) nb1(n), 256, 0, stream()
+#define DONE()              \
+  do {                      \
+    FPC_LAUNCH_CHECK();     \
+    FPC_HIP(hipStreamSynchronize(stream())); \
+    return 0;               \
+  } while (0)
+
+extern "C" {
+
+int fpc_apply_bc_cleanup(double *u, double *v, double *w, const unsigned char *au, const unsigned char *av,
+                         const unsigned char *aw, const double *uin, int nx, int ny, int nz) {
+  long long N = std::max((long long)(nx + 1) * ny * nz, std::max((long long)nx * (ny + 1) * nz, (long long)nx * ny * (nz + 1)));
+  k_bc<<<LAUNCH1(N)>>>(u, v, w, au, av, aw, uin, nx, ny, nz);
+  DONE();
+}
+
+int fpc_conv_u(double *conv, const double *u, const double *v, const double *w, const unsigned char *a, int nx, int ny,
+               int nz, double dx, double dy, double dz) {
+  k_conv<<<LAUNCH1((long long)(nx + 1) * ny * nz)>>>(0, conv, u, v, w, a, nx, ny, nz, dx, dy, dz);
+  DONE();
+}
+int fpc_conv_v(double *conv, const double *u, const double *v, const double *w, const unsigned char *a, int nx, int ny,
+               int nz, double dx, double dy, double dz) {
+  k_conv<<<LAUNCH1((long long)nx * (ny + 1) * nz)>>>(1, conv, u, v, w, a, nx, ny, nz, dx, dy, dz);
+  DONE();
+}
+int fpc_conv_w(double *conv, const double *u, const double *v, const double *w, const unsigned char *a, int nx, int ny,
+               int nz, double dx, double dy, double dz) {
+  k_conv<<<LAUNCH1((long long)nx * ny * (nz + 1))>>>(2, conv, u, v, w, a, nx, ny, nz, dx, dy, dz);
+  DONE();
+}
+
+int fpc_build_u_full(double *rhsf, double *x0, const double *u, const double *conv, const double *gpx,
+                     const unsigned char *a, const int *id, const double *bc, const double *sc, int nx, int ny, int nz,
+                     double rho, double dt, int zero) {
+  k_build_full<<<LAUNCH1((long long)(nx - 1) * ny * nz)>>>(0, rhsf, x0, u, conv, gpx, a, id, bc, sc, nx, ny, nz, rho, dt, zero);
+  DONE();
+}
+int fpc_build_v_full(double *rhsf, double *x0, const double *v, const double *conv, const double *gpy,
+                     const unsigned char *a, const int *id, const double *bc, const double *sc, int nx, int ny, int nz,
+                     double rho, double dt, int zero) {
+  k_build_full<<<LAUNCH1((long long)nx * (ny - 1) * nz)>>>(1, rhsf, x0, v, conv, gpy, a, id, bc, sc, nx, ny, nz, rho, dt, zero);
+  DONE();
+}
+int fpc_build_w_full(double *rhsf, double *x0, const double *w, const double *conv, const double *gpz,
+                     const unsigned char *a, const int *id, const double *bc, const double *sc, int nx, int ny, int nz,
+                     double rho, double dt, int zero) {
+  k_build_full<<<LAUNCH1((long long)nx * ny * (nz - 1))>>>(2, rhsf, x0, w, conv, gpz, a, id, bc, sc, nx, ny, nz, rho, dt, zero);
+  DONE();
+}
+
+int fpc_scatter_u(double *o, const double *x, const double *base, int nx, int ny, int nz) {
+  k_scatter<<<LAUNCH1((long long)(nx + 1) * ny * nz)>>>(0, o, x, base, nx, ny, nz);
+  DONE();
+}
+int fpc_scatter_v(double *o, const double *x, const double *base, int nx, int ny, int nz) {
+  k_scatter<<<LAUNCH1((long long)nx * (ny + 1) * nz)>>>(1, o, x, base, nx, ny, nz);
+  DONE();
+}
+int fpc_scatter_w(double *o, const double *x, const double *base, int nx, int ny, int nz) {
+  k_scatter<<<LAUNCH1((long long)nx * ny * (nz + 1))>>>(2, o, x, base, nx, ny, nz);
+  DONE();
+}
+
+int fpc_spmv_csr_1based(int nrows, const int *rowptr, const int *colind, const double *vals, const double *x, double *y) {
+  if (nrows <= 0) return 0;
+  k_csr<<<LAUNCH1(nrows)>>>(nrows, rowptr, colind, vals, x, y);
+  DONE();
+}
+
+int fpc_pack_pressure_active(double *vec, const double *p, const unsigned char *fluid, const int *id, int N) {
+  k_pack_masked<<<LAUNCH1(N)>>>(vec, p, fluid, id, N);
+  DONE();
+}
+int fpc_unpack_gp_u(const double *gp, double *gpx, const int *id, int nx, int ny, int nz) {
+  k_unpack_gp<<<LAUNCH1((long long)(nx + 1) * ny * nz)>>>(gp, gpx, id, (long long)(nx + 1) * ny * nz, 0);
+  DONE();
+}
+int fpc_unpack_gp_v(const double *gp, double *gpy, const int *id, int nx, int ny, int nz, int nQu) {
+  k_unpack_gp<<<LAUNCH1((long long)nx * (ny + 1) * nz)>>>(gp, gpy, id, (long long)nx * (ny + 1) * nz, nQu);
+  DONE();
+}
+int fpc_unpack_gp_w(const double *gp, double *gpz, const int *id, int nx, int ny, int nz, int nQu, int nQv) {
+  k_unpack_gp<<<LAUNCH1((long long)nx * ny * (nz + 1))>>>(gp, gpz, id, (long long)nx * ny * (nz + 1), (long long)nQu + nQv);
+  DONE();
+}
+int fpc_pack_projection_q_u(double *q, const double *u, const int *id, int nx, int ny, int nz) {
+  k_pack_q<<<LAUNCH1((long long)(nx + 1) * ny * nz)>>>(q, u, id, (long long)(nx + 1) * ny * nz, 0);
+  DONE();
+}
+int fpc_pack_projection_q_v(double *q, const double *v, const int *id, int nx, int ny, int nz, int nQu) {
+  k_pack_q<<<LAUNCH1((long long)nx * (ny + 1) * nz)>>>(q, v, id, (long long)nx * (ny + 1) * nz, nQu);
+  DONE();
+}
+int fpc_pack_projection_q_w(double *q, const double *w, const int *id, int nx, int ny, int nz, int nQu, int nQv) {
+  k_pack_q<<<LAUNCH1((long long)nx * ny * (nz + 1))>>>(q, w, id, (long long)nx * ny * (nz + 1), (long long)nQu + nQv);
+  DONE();
+}
+int fpc_build_pressure_rhs_full(double *rhsf, double *x0, const double *Dqt, const double *dbc,
+                                const unsigned char *fluid, const int *id, int N, double rho, double dt, double ramp) {
+  k_prhs<<<LAUNCH1(N)>>>(rhsf, x0, Dqt, dbc, fluid, id, N, rho, dt, ramp);
+  DONE();
+}
+int fpc_pack_phi_active(double *phi, const double *xp, const unsigned char *fluid, const int *id, int N) {
+  k_pack_masked<<<LAUNCH1(N)>>>(phi, xp, fluid, id, N);
+  DONE();
+}
+int fpc_apply_q_correction(double *q, const double *g, double factor, int n1) {
+  k_qcorr<<<LAUNCH1(n1)>>>(q, g, factor, n1);
+  DONE();
+}
+int fpc_unpack_projection_q_u(const double *q, double *u, const int *id, int nx, int ny, int nz) {
+  k_unpack_q<<<LAUNCH1((long long)(nx + 1) * ny * nz)>>>(q, u, id, (long long)(nx + 1) * ny * nz, 0);
+  DONE();
+}
+int fpc_unpack_projection_q_v(const double *q, double *v, const int *id, int nx, int ny, int nz, int nQu) {
+  k_unpack_q<<<LAUNCH1((long long)nx * (ny + 1) * nz)>>>(q, v, id, (long long)nx * (ny + 1) * nz, nQu);
+  DONE();
+}
+int fpc_unpack_projection_q_w(const double *q, double *w, const int *id, int nx, int ny, int nz, int nQu, int nQv) {
+  k_unpack_q<<<LAUNCH1((long long)nx * ny * (nz + 1))>>>(q, w, id, (long long)nx * ny * (nz + 1), (long long)nQu + nQv);
+  DONE();
+}
+int fpc_add_pressure_phi(double *p, const double *phi, const unsigned char *fluid, const int *id, int N) {
+  k_add_phi<<<LAUNCH1(N)>>>(p, phi, fluid, id, N);
+  DONE();
+}
+int fpc_zero_doubles(double *p, long long n) {
+  if (n <= 0) return 0;
+  k_zero<<<(unsigned)std::min(65536LL, nb1(n) * 1LL), 256, 0, stream()>>>(p, n);
+  DONE();
+}
+
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- fused stages (asynchronous variants are internal)
+namespace fpc {
+int stage_explicit_async(const fpc_flow_ops *o, double *u, double *v, double *w, const double *p,
+                         const double *uin, double *ru, double *xu, double *rv, double *xv, double *rw, double *xw,
+                         int zero) {
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  long long N = std::max((long long)(nx + 1) * ny * nz, std::max((long long)nx * (ny + 1) * nz, (long long)nx * ny * (nz + 1)));
+  k_bc<<<LAUNCH1(N)>>>(u, v, w, o->active_u, o->active_v, o->active_w, uin, nx, ny, nz);
+  FPC_LAUNCH_CHECK();
+  double *outs[3][2] = {{ru, xu}, {rv, xv}, {rw, xw}};
+  for (int c = 0; c < 3; ++c) {
+    Dims d;
+    d.nx = nx - (c == 0); d.ny = ny - (c == 1); d.nz = nz - (c == 2);
+    dim3 g, b;
+    launch_shape(d, g, b);
+    k_explicit<<<g, b, 0, stream()>>>(c, *o, u, v, w, p, outs[c][0], outs[c][1], zero);
+    FPC_LAUNCH_CHECK();
+  }
+  return 0;
+}
+
+int stage_scatter_async(const fpc_flow_ops *o, const double *xu, const double *xv, const double *xw, const double *u,
+                        const double *v, const double *w, double *ut, double *vt, double *wt, const double *uin) {
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  k_scatter<<<LAUNCH1((long long)(nx + 1) * ny * nz)>>>(0, ut, xu, u, nx, ny, nz);
+  k_scatter<<<LAUNCH1((long long)nx * (ny + 1) * nz)>>>(1, vt, xv, v, nx, ny, nz);
+  k_scatter<<<LAUNCH1((long long)nx * ny * (nz + 1))>>>(2, wt, xw, w, nx, ny, nz);
+  FPC_LAUNCH_CHECK();
+  long long N = std::max((long long)(nx + 1) * ny * nz, std::max((long long)nx * (ny + 1) * nz, (long long)nx * ny * (nz + 1)));
+  k_bc<<<LAUNCH1(N)>>>(ut, vt, wt, o->active_u, o->active_v, o->active_w, uin, nx, ny, nz);
+  FPC_LAUNCH_CHECK();
+  return 0;
+}
+
+int stage_divergence_async(const fpc_flow_ops *o, const double *ut, const double *vt, const double *wt, double ramp,
+                           double *rhs, double *x0) {
+  Dims d;
+  d.nx = o->nx; d.ny = o->ny; d.nz = o->nz;
+  dim3 g, b;
+  launch_shape(d, g, b);
+  k_divergence<<<g, b, 0, stream()>>>(*o, ut, vt, wt, ramp, rhs, x0);
+  FPC_LAUNCH_CHECK();
+  return 0;
+}
+
+int stage_projection_async(const fpc_flow_ops *o, const double *phi, const double *ut, const double *vt,
+                           const double *wt, double *u, double *v, double *w, double *p, const double *uin,
+                           double *qout) {
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  long long N = std::max((long long)(nx + 1) * ny * nz, std::max((long long)nx * (ny + 1) * nz, (long long)nx * ny * (nz + 1)));
+  k_project<<<LAUNCH1(N)>>>(*o, phi, ut, vt, wt, u, v, w, qout);
+  FPC_LAUNCH_CHECK();
+  k_bc<<<LAUNCH1(N)>>>(u, v, w, o->active_u, o->active_v, o->active_w, uin, nx, ny, nz);
+  FPC_LAUNCH_CHECK();
+  long long Np = (long long)nx * ny * nz;
+  k_add_phi_box<<<LAUNCH1(Np)>>>(p, phi, o->fluid_p, Np);
+  FPC_LAUNCH_CHECK();
+  return 0;
+}
+
+int stage_diagnostics_async(const fpc_flow_ops *o, const double *u, const double *v, const double *w, const double *p,
+                            const double *qout, double ramp, int Np, double *out) {
+  const int NBD = 512;
+  double *part = scratch_doubles((size_t)NBD * 6, 6);
+  if (!part) return FPC_ERROR_MEMORY;
+  k_diag_partials<<<NBD, 256, 0, stream()>>>(*o, u, v, w, p, qout, ramp, part);
+  FPC_LAUNCH_CHECK();
+  k_diag_final<<<1, 1024, 0, stream()>>>(part, NBD, Np, out);
+  FPC_LAUNCH_CHECK();
+  return 0;
+}
+}  // namespace fpc
+
+extern "C" {
+
+int fpc_stage_explicit(const fpc_flow_ops *o, double *u, double *v, double *w, const double *p, const double *uin,
+                       double *ru, double *xu, double *rv, double *xv, double *rw, double *xw, int zero) {
+  int rc = stage_explicit_async(o, u, v, w, p, uin, ru, xu, rv, xv, rw, xw, zero);
+  if (rc) return rc;
+  FPC_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+int fpc_stage_scatter(const fpc_flow_ops *o, const double *xu, const double *xv, const double *xw, const double *u,
+                      const double *v, const double *w, double *ut, double *vt, double *wt, const double *uin) {
+  int rc = stage_scatter_async(o, xu, xv, xw, u, v, w, ut, vt, wt, uin);
+  if (rc) return rc;
+  FPC_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+int fpc_stage_divergence(const fpc_flow_ops *o, const double *ut, const double *vt, const double *wt, double ramp,
+                         double *rhs, double *x0) {
+  int rc = stage_divergence_async(o, ut, vt, wt, ramp, rhs, x0);
+  if (rc) return rc;
+  FPC_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+int fpc_stage_projection(const fpc_flow_ops *o, const double *phi, const double *ut, const double *vt,
+                         const double *wt, double *u, double *v, double *w, double *p, const double *uin,
+                         double *qout) {
+  int rc = stage_projection_async(o, phi, ut, vt, wt, u, v, w, p, uin, qout);
+  if (rc) return rc;
+  FPC_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+int fpc_stage_diagnostics(const fpc_flow_ops *o, const double *u, const double *v, const double *w, const double *p,
+                          const double *qout, double ramp, int Np, double *out) {
+  return stage_diagnostics_async(o, u, v, w, p, qout, ramp, Np, out);
+}
+
+int fpc_any_nonfinite(const double *x, long long n, int *flag) {
+  FPC_CHECK_ARG(flag, "null flag");
+  int *d = (int *)scratch_doubles(1, 7);
+  if (!d) return FPC_ERROR_MEMORY;
+  FPC_HIP(hipMemsetAsync(d, 0, sizeof(int), stream()));
+  if (n > 0) {
+    k_nonfinite<<<(unsigned)std::min(4096LL, nb1(n) * 1LL), 256, 0, stream()>>>(x, n, d);
+    FPC_LAUNCH_CHECK();
+  }
+  FPC_HIP(hipMemcpyAsync(flag, d, sizeof(int), hipMemcpyDeviceToHost, stream()));
+  FPC_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,172 @@
+// fpc_internal.hpp -- internal definitions shared by the libfpc HIP sources.
+//
+// Layout in HBM: a Struct box of n = (nx, ny, nz) points is stored x fastest,
+// then y, then z (common.hpp:230-238 of the reference).  A 7-point matrix is
+// stored as seven x-contiguous planes ("SoA"), canonical slot order
+// C, -x, +x, -y, +y, -z, +z (the reference stencil order, hypre_struct_utils.hpp:9),
+// each plane padded to a 256-byte multiple so every plane starts aligned.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "fpc_struct.h"
+
+namespace fpc {
+
+void set_error(int bits, const char *fmt, ...);
+int error_bits();
+hipStream_t stream();
+void ensure_init();
+
+#define FPC_HIP(call)                                                                                  \
+  do {                                                                                                 \
+    hipError_t e_ = (call);                                                                            \
+    if (e_ != hipSuccess) {                                                                            \
+      ::fpc::set_error(FPC_ERROR_GENERIC, "%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+      return FPC_ERROR_GENERIC;                                                                        \
+    }                                                                                                  \
+  } while (0)
+
+#define FPC_CHECK_ARG(cond, msg)                                          \
+  do {                                                                    \
+    if (!(cond)) {                                                        \
+      ::fpc::set_error(FPC_ERROR_ARG, "%s: %s", __func__, msg);          \
+      return FPC_ERROR_ARG;                                               \
+    }                                                                     \
+  } while (0)
+
+// Launch check: reports a failed launch as an error return.
+#define FPC_LAUNCH_CHECK() FPC_HIP(hipGetLastError())
+
+constexpr int kSlots = 7;
+constexpr int kMaxKDim = 128;       // largest GMRES restart length supported
+constexpr int kRedBlocks = 1024;    // blocks of the deterministic reductions
+
+inline long long pad256(long long n, int elsize) {
+  long long per = 256 / elsize;
+  return (n + per - 1) / per * per;
+}
+
+// Box geometry for launches: 2-D blocks over (x, y), grid z = box z.
+struct Dims {
+  int nx = 0, ny = 0, nz = 0;
+  long long N = 0;
+};
+
+inline void launch_shape(const Dims &d, dim3 &grid, dim3 &block) {
+  int tx = 64;
+  while (tx > 1 && tx / 2 >= d.nx) tx /= 2;
+  int ty = 256 / tx;
+  while (ty > 1 && ty / 2 >= d.ny) ty /= 2;
+  block = dim3(tx, ty, 1);
+  grid = dim3((d.nx + tx - 1) / tx, (d.ny + ty - 1) / ty, d.nz);
+}
+
+template <typename T>
+struct Stencil {
+  const T *c;        // 7 planes
+  long long pitch;   // elements between planes
+  int nx, ny, nz;
+};
+
+// Deterministic block reduction of one double per thread (wave64 shuffles + LDS).
+template <int BS>
+__device__ inline double block_sum(double v, double *lds) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) lds[wid] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    for (int w = 0; w < BS / 64; ++w) r += lds[w];
+  }
+  return r;  // valid in thread 0
+}
+
+}  // namespace fpc
+
+// ---------------------------------------------------------------- handles
+struct fpc_StructGrid_s {
+  int ndim = 3;
+  int il[3] = {0, 0, 0}, iu[3] = {-1, -1, -1};
+  int n[3] = {0, 0, 0};
+  long long N = 0;
+  bool extents = false, assembled = false;
+};
+
+struct fpc_StructStencil_s {
+  int ndim = 3, size = 0;
+  int off[fpc::kSlots][3] = {};
+  int set_mask = 0;
+};
+
+struct fpc_StructMatrix_s {
+  fpc_StructGrid_s grid;
+  int slot_of_entry[fpc::kSlots] = {0, 1, 2, 3, 4, 5, 6};
+  int nentries = 7;
+  long long pitch = 0;
+  double *coef = nullptr;    // device, 7 * pitch
+  double *dinv = nullptr;    // device, 1 / diagonal (Jacobi)
+  float *coef32 = nullptr;   // device, FP32 copy (mixed-precision paths)
+  float *dinv32 = nullptr;
+  bool initialized = false, assembled = false;
+  unsigned long long version = 0;  // bumped whenever values change
+};
+
+struct fpc_StructVector_s {
+  fpc_StructGrid_s grid;
+  double *data = nullptr;
+  bool initialized = false;
+};
+
+struct fpc_StructSolver_s {
+  int kind = 0;
+  void *impl = nullptr;
+};
+
+namespace fpc {
+
+enum SolverKind { SK_NONE = 0, SK_GMRES = 1, SK_FGMRES = 2, SK_JACOBI = 3, SK_PFMG = 4 };
+
+// Native preconditioners used inside GMRES (jacobi.hip, pfmg.hip).
+struct JacobiImpl {
+  int maxit = 1;
+  double tol = 0.0;
+  bool zero_guess = false;
+  fpc_StructMatrix_s *A = nullptr;
+};
+// x = M^-1 b with a zero initial guess (maxit sweeps); skip: device flag, work is skipped when *skip != 0.
+int jacobi_apply(JacobiImpl *J, const double *b, double *x, const int *skip);
+int pfmg_setup(void *pf, fpc_StructMatrix_s *A);
+int pfmg_apply(void *pf, const double *b, double *x, const int *skip);
+
+inline Dims dims_of(const fpc_StructGrid_s &g) {
+  Dims d;
+  d.nx = g.n[0]; d.ny = g.n[1]; d.nz = g.n[2]; d.N = g.N;
+  return d;
+}
+
+// Vector kernels used by several solvers (struct_mv.hip).
+int vec_copy(double *dst, const double *src, long long n);
+int vec_zero(double *dst, long long n);
+// x += y
+int axpy_inplace(double *x, const double *y, long long n);
+// r = b - A x (FP64), writes partial sums of r*r per block into part (kRedBlocks)
+int residual_norm_partials(const fpc_StructMatrix_s *A, const double *b, const double *x, double *r, double *part);
+// sum of kRedBlocks partials into *out (device)
+int reduce_partials(const double *part, int nparts, double *out);
+int dot_partials(const double *a, const double *b, long long n, double *part);
+int matvec(double alpha, const fpc_StructMatrix_s *A, const double *x, double beta, double *y);
+int ensure_dinv(fpc_StructMatrix_s *A);
+int ensure_fp32(fpc_StructMatrix_s *A);
+// Device scratch owned by the library (grows on demand, freed at Finalize).
+double *scratch_doubles(size_t n, int which);
+
+}  // namespace fpc

@@ -1,0 +1,762 @@
+// krylov.hip -- restarted (flexible) GMRES on the device and the Struct Jacobi
+// preconditioner (libfpc).
+//
+// Replaces HYPRE_StructGMRES* (reference main.cu:174-221, 284-297, 327-331)
+// and HYPRE_StructJacobi* (main.cu:179-182).  The control flow is HYPRE's
+// published right-preconditioned GMRES: the same stopping test
+// ||r|| <= max(atol, tol*||b||), the same Givens update, the same true-residual
+// check and restart rules, and the same Givens-based residual at a full restart.
+//
+// MI355X mapping:
+//  * all scalar state (Hessenberg matrix, rotations, residual estimate,
+//    convergence flag) lives in device memory and is updated by one-block
+//    kernels, so an Arnoldi step needs no host round trip;
+//  * the host enqueues a batch of Arnoldi steps (sized by the previous solve's
+//    count), then polls one small status record; kernels of steps past
+//    convergence see the flag and return at once;
+//  * classical Gram-Schmidt as one multi-dot pass plus one fused
+//    update-and-norm pass (instead of 2j passes for modified Gram-Schmidt);
+//  * basis vectors are stored unnormalised with a device scale factor, so no
+//    separate scaling pass is needed;
+//  * Jacobi (one sweep from zero: z = v / diag) is fused into the matvec;
+//  * every reduction is two-stage with a fixed block count: run-to-run
+//    bitwise deterministic.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "fpc_internal.hpp"
+
+namespace fpc {
+namespace {
+
+constexpr int KM = kMaxKDim;
+constexpr int NB = kRedBlocks;  // blocks of every partial-sum kernel
+
+// Status record polled by the host (first, small part of the device state).
+struct Status {
+  int i;        // Arnoldi steps completed in the current cycle
+  int done;     // 1: converged estimate, 2: cycle full or maxit reached
+  int iter;     // total steps
+  int maxit;
+  int kdim;
+  int pad;
+  double r_norm;  // current residual estimate |rs[i]|
+  double eps;
+};
+
+struct State {
+  Status st;
+  double sigma[KM + 1];          // scale of stored basis vector j (v_j = sigma_j p_j)
+  double rs[KM + 1];
+  double c[KM], s[KM];
+  double y[KM + 1];              // combination coefficients
+  double hh[(KM + 1) * KM];      // hh(row, col) = hh[col * (KM + 1) + row]
+};
+
+__device__ inline double &H(State *S, int row, int col) { return S->hh[col * (KM + 1) + row]; }
+
+// w = A z with z = sigma * p (* dinv when Jacobi is fused), one thread per point.
+// The current basis index is read from the device status.
+__global__ void k_arnoldi_matvec(const double *__restrict__ c, long long pitch, int nx, int ny, int nz,
+                                 const double *__restrict__ V, long long vp, const double *__restrict__ zin,
+                                 const double *__restrict__ dinv, State *S) {
+  if (S->st.done) return;
+  int cur = S->st.i;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int k = blockIdx.z;
+  if (i >= nx || j >= ny) return;
+  const double *z = zin ? zin : V + (long long)cur * vp;
+  double sg = S->sigma[cur];
+  double *w = (double *)V + (long long)(cur + 1) * vp;
+  long long r = ((long long)k * ny + j) * nx + i, pl = (long long)nx * ny;
+  double s;
+  if (dinv) {
+    s = c[r] * (z[r] * dinv[r]);
+    if (i > 0) s += c[pitch + r] * (z[r - 1] * dinv[r - 1]);
+    if (i < nx - 1) s += c[2 * pitch + r] * (z[r + 1] * dinv[r + 1]);
+    if (j > 0) s += c[3 * pitch + r] * (z[r - nx] * dinv[r - nx]);
+    if (j < ny - 1) s += c[4 * pitch + r] * (z[r + nx] * dinv[r + nx]);
+    if (k > 0) s += c[5 * pitch + r] * (z[r - pl] * dinv[r - pl]);
+    if (k < nz - 1) s += c[6 * pitch + r] * (z[r + pl] * dinv[r + pl]);
+  } else {
+    s = c[r] * z[r];
+    if (i > 0) s += c[pitch + r] * z[r - 1];
+    if (i < nx - 1) s += c[2 * pitch + r] * z[r + 1];
+    if (j > 0) s += c[3 * pitch + r] * z[r - nx];
+    if (j < ny - 1) s += c[4 * pitch + r] * z[r + nx];
+    if (k > 0) s += c[5 * pitch + r] * z[r - pl];
+    if (k < nz - 1) s += c[6 * pitch + r] * z[r + pl];
+  }
+  w[r] = sg * s;
+}
+
+// Partial dots <p_j, w> for j = 0..cur, eight vectors per pass over w.
+__global__ void __launch_bounds__(256) k_multidot(const double *__restrict__ V, long long vp, long long n, State *S,
+                                                  double *__restrict__ part) {
+  if (S->st.done) return;
+  __shared__ double lds[8 * 4];
+  int cur = S->st.i;
+  const double *w = V + (long long)(cur + 1) * vp;
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int j0 = 0; j0 <= cur; j0 += 8) {
+    int nj = min(8, cur + 1 - j0);
+    double acc[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc[q] = 0.0;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+      double wv = w[t];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (q < nj) acc[q] += V[(long long)(j0 + q) * vp + t] * wv;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      double v = acc[q];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      acc[q] = v;
+    }
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) lds[q * 4 + wid] = acc[q];
+    __syncthreads();
+    if (threadIdx.x < nj) {
+      int q = threadIdx.x;
+      part[(long long)(j0 + q) * NB + blockIdx.x] = ((lds[q * 4] + lds[q * 4 + 1]) + lds[q * 4 + 2]) + lds[q * 4 + 3];
+    }
+  }
+}
+
+// h_{j,cur} = sigma_j * sum_b part[j][b]; one block per j.
+__global__ void __launch_bounds__(256) k_reduce_h(State *S, const double *__restrict__ part) {
+  if (S->st.done) return;
+  __shared__ double lds[4];
+  int cur = S->st.i;
+  int jv = blockIdx.x;
+  if (jv > cur) return;
+  double acc = 0.0;
+  for (int t = threadIdx.x; t < NB; t += blockDim.x) acc += part[(long long)jv * NB + t];
+  double tot = block_sum<256>(acc, lds);
+  if (threadIdx.x == 0) H(S, jv, cur) = S->sigma[jv] * tot;
+}
+
+// w -= sum_j h_j sigma_j p_j with partial sums of w^2.
+__global__ void __launch_bounds__(256) k_orth_update(double *__restrict__ V, long long vp, long long n, State *S,
+                                                     double *__restrict__ part) {
+  if (S->st.done) return;
+  __shared__ double coef[KM + 1];
+  __shared__ double lds[4];
+  int cur = S->st.i;
+  for (int q = threadIdx.x; q <= cur; q += blockDim.x) coef[q] = H(S, q, cur) * S->sigma[q];
+  __syncthreads();
+  double *w = V + (long long)(cur + 1) * vp;
+  double acc = 0.0;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+    double v = w[t];
+    for (int q = 0; q <= cur; ++q) v -= coef[q] * V[(long long)q * vp + t];
+    w[t] = v;
+    acc += v * v;
+  }
+  double tot = block_sum<256>(acc, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+// Norm of the new vector, Givens update, convergence flag (HYPRE's Arnoldi tail).
+__global__ void __launch_bounds__(256) k_arnoldi_tail(State *S, const double *__restrict__ part) {
+  if (S->st.done) return;
+  __shared__ double lds[4];
+  double acc = 0.0;
+  for (int t = threadIdx.x; t < NB; t += blockDim.x) acc += part[t];
+  double tot = block_sum<256>(acc, lds);
+  if (threadIdx.x != 0) return;
+  int cur = S->st.i;  // HYPRE's i - 1
+  int i = cur + 1;
+  double t = sqrt(tot);
+  H(S, i, cur) = t;
+  S->sigma[i] = (t != 0.0) ? 1.0 / t : 1.0;
+  for (int j = 1; j < i; ++j) {
+    double tt = H(S, j - 1, cur);
+    H(S, j - 1, cur) = S->s[j - 1] * H(S, j, cur) + S->c[j - 1] * tt;
+    H(S, j, cur) = -S->s[j - 1] * tt + S->c[j - 1] * H(S, j, cur);
+  }
+  double g = sqrt(H(S, i, cur) * H(S, i, cur) + H(S, cur, cur) * H(S, cur, cur));
+  if (g == 0.0) g = 2.220446049250313e-16;
+  S->c[cur] = H(S, cur, cur) / g;
+  S->s[cur] = H(S, i, cur) / g;
+  S->rs[i] = -H(S, i, cur) * S->rs[cur] / g;
+  S->rs[cur] = S->c[cur] * S->rs[cur];
+  H(S, cur, cur) = S->s[cur] * H(S, i, cur) + S->c[cur] * H(S, cur, cur);
+  S->st.r_norm = fabs(S->rs[i]);
+  S->st.i = i;
+  S->st.iter += 1;
+  if (S->st.r_norm <= S->st.eps) S->st.done = 1;
+  else if (i >= S->st.kdim || S->st.iter >= S->st.maxit) S->st.done = 2;
+}
+
+// Upper-triangular solve; y_j = sigma_j * rs_j (coefficients of the stored p_j).
+__global__ void k_backsolve(State *S) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int i = S->st.i;
+  double rs[KM + 1];
+  for (int q = 0; q <= i; ++q) rs[q] = S->rs[q];
+  rs[i - 1] = rs[i - 1] / H(S, i - 1, i - 1);
+  for (int k = i - 2; k >= 0; --k) {
+    double t = 0.0;
+    for (int j = k + 1; j < i; ++j) t -= H(S, k, j) * rs[j];
+    t += rs[k];
+    rs[k] = t / H(S, k, k);
+  }
+  for (int q = 0; q < i; ++q) S->y[q] = rs[q] * S->sigma[q];
+}
+
+// out (+)= [dinv *] sum_j y_j P_j   (x update or the preconditioner input)
+__global__ void k_combine(const double *__restrict__ P, long long vp, long long n, const State *S,
+                          const double *__restrict__ dinv, double *__restrict__ out, int accumulate) {
+  __shared__ double y[KM + 1];
+  int i = S->st.i;
+  for (int q = threadIdx.x; q < i; q += blockDim.x) y[q] = S->y[q];
+  __syncthreads();
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+    double v = y[i - 1] * P[(long long)(i - 1) * vp + t];
+    for (int q = i - 2; q >= 0; --q) v += y[q] * P[(long long)q * vp + t];
+    if (dinv) v *= dinv[t];
+    out[t] = accumulate ? out[t] + v : v;
+  }
+}
+
+// HYPRE's residual at a full restart: rs_j rotated back, p_0 = sum_j rs_j v_j.
+__global__ void k_restart_coefs(State *S) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int i = S->st.i;
+  for (int j = i; j > 0; --j) {
+    S->rs[j - 1] = -S->s[j - 1] * S->rs[j];
+    S->rs[j] = S->c[j - 1] * S->rs[j];
+  }
+  for (int j = 0; j <= i; ++j) S->y[j] = S->rs[j] * S->sigma[j];
+}
+
+__global__ void k_restart_vec(double *__restrict__ V, long long vp, long long n, const State *S) {
+  __shared__ double y[KM + 1];
+  int i = S->st.i;
+  for (int q = threadIdx.x; q <= i; q += blockDim.x) y[q] = S->y[q];
+  __syncthreads();
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+    double v = y[0] * V[t];
+    for (int q = 1; q <= i; ++q) v += y[q] * V[(long long)q * vp + t];
+    V[t] = v;
+  }
+}
+
+// Start of a cycle: i = 0, rs_0 = r_norm, sigma_0 = 1/r_norm.
+__global__ void k_cycle_start(State *S, double r_norm, double eps, int iter, int maxit, int kdim) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  S->st.i = 0;
+  S->st.done = 0;
+  S->st.iter = iter;
+  S->st.maxit = maxit;
+  S->st.kdim = kdim;
+  S->st.r_norm = r_norm;
+  S->st.eps = eps;
+  S->rs[0] = r_norm;
+  S->sigma[0] = 1.0 / r_norm;
+}
+
+__global__ void k_scale_copy(double *__restrict__ dst, const double *__restrict__ src, const State *S, long long n) {
+  int cur = S->st.i;
+  double sg = S->sigma[cur];
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+    dst[t] = sg * src[t];
+}
+
+__global__ void k_set_done(State *S, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) S->st.done = v;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- Jacobi
+namespace {
+__global__ void k_jacobi_zero(const double *__restrict__ dinv, const double *__restrict__ b, double *__restrict__ x,
+                              long long n, const int *skip) {
+  if (skip && *skip) return;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+    x[t] = b[t] * dinv[t];
+}
+__global__ void k_jacobi_sweep(const double *__restrict__ c, long long pitch, int nx, int ny, int nz,
+                               const double *__restrict__ dinv, const double *__restrict__ b,
+                               const double *__restrict__ x, double *__restrict__ xn, const int *skip) {
+  if (skip && *skip) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int k = blockIdx.z;
+  if (i >= nx || j >= ny) return;
+  long long r = ((long long)k * ny + j) * nx + i, pl = (long long)nx * ny;
+  double s = c[r] * x[r];
+  if (i > 0) s += c[pitch + r] * x[r - 1];
+  if (i < nx - 1) s += c[2 * pitch + r] * x[r + 1];
+  if (j > 0) s += c[3 * pitch + r] * x[r - nx];
+  if (j < ny - 1) s += c[4 * pitch + r] * x[r + nx];
+  if (k > 0) s += c[5 * pitch + r] * x[r - pl];
+  if (k < nz - 1) s += c[6 * pitch + r] * x[r + pl];
+  xn[r] = x[r] + (b[r] - s) * dinv[r];
+}
+unsigned nblk(long long n) {
+  long long b = (n + 255) / 256;
+  return (unsigned)std::max(1LL, std::min(b, 65536LL));
+}
+}  // namespace
+
+int jacobi_apply(JacobiImpl *J, const double *b, double *x, const int *skip) {
+  fpc_StructMatrix_s *A = J->A;
+  int rc = ensure_dinv(A);
+  if (rc) return rc;
+  long long n = A->grid.N;
+  k_jacobi_zero<<<nblk(n), 256, 0, stream()>>>(A->dinv, b, x, n, skip);
+  FPC_LAUNCH_CHECK();
+  if (J->maxit > 1) {
+    double *t = scratch_doubles((size_t)n, 3);
+    if (!t) return FPC_ERROR_MEMORY;
+    Dims d = dims_of(A->grid);
+    dim3 g, bl;
+    launch_shape(d, g, bl);
+    for (int it = 1; it < J->maxit; ++it) {
+      k_jacobi_sweep<<<g, bl, 0, stream()>>>(A->coef, A->pitch, d.nx, d.ny, d.nz, A->dinv, b, x, t, skip);
+      FPC_LAUNCH_CHECK();
+      rc = vec_copy(x, t, n);
+      if (rc) return rc;
+    }
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------- GMRES host driver
+namespace {
+
+enum PcKind { PC_NONE = 0, PC_JACOBI1 = 1, PC_JACOBI = 2, PC_PFMG = 3, PC_CALLBACK = 4 };
+
+struct GmresImpl {
+  bool flexible = false;
+  double tol = 1.0e-6, atol = 0.0;  // HYPRE defaults
+  int maxit = 1000, kdim = 5, print = 0, logging = 0;
+  fpc_PtrToStructSolverFcn pc_solve = nullptr, pc_setup = nullptr;
+  fpc_StructSolver pc = nullptr;
+  int pc_kind = PC_NONE;
+  // workspace
+  long long N = 0, vp = 0;
+  int kd = 0;
+  double *V = nullptr;    // kd+1 basis vectors
+  double *Z = nullptr;    // flexible: kd preconditioned vectors; standard: one work vector
+  double *W = nullptr;    // work vector (true residual / precond input)
+  double *part = nullptr; // NB * (KM + 1)
+  double *dscal = nullptr;  // reduction outputs
+  State *S = nullptr;
+  Status *hst = nullptr;    // pinned
+  double *hscal = nullptr;  // pinned
+  int its = 0;
+  double relres = 0.0;
+  int pred = 4;  // expected Arnoldi steps per cycle (from the previous solve)
+  fpc_StructMatrix_s *A = nullptr;
+  fpc_StructVector_s tmp_in, tmp_out;  // callback preconditioner views
+};
+
+void free_ws(GmresImpl *G) {
+  if (G->V) (void)hipFree(G->V);
+  if (G->Z) (void)hipFree(G->Z);
+  if (G->W) (void)hipFree(G->W);
+  if (G->part) (void)hipFree(G->part);
+  if (G->dscal) (void)hipFree(G->dscal);
+  if (G->S) (void)hipFree(G->S);
+  if (G->hst) (void)hipHostFree(G->hst);
+  if (G->hscal) (void)hipHostFree(G->hscal);
+  G->V = G->Z = G->W = G->part = G->dscal = nullptr;
+  G->S = nullptr;
+  G->hst = nullptr;
+  G->hscal = nullptr;
+  G->N = 0;
+}
+
+GmresImpl *gimpl(fpc_StructSolver s) {
+  if (!s || (s->kind != SK_GMRES && s->kind != SK_FGMRES)) return nullptr;
+  return (GmresImpl *)s->impl;
+}
+
+int precond_apply(GmresImpl *G, const double *in, double *out, const int *skip) {
+  switch (G->pc_kind) {
+    case PC_JACOBI:
+      return jacobi_apply((JacobiImpl *)G->pc->impl, in, out, skip);
+    case PC_PFMG:
+      return pfmg_apply(G->pc->impl, in, out, skip);
+    case PC_CALLBACK: {
+      G->tmp_in.data = (double *)in;
+      G->tmp_out.data = out;
+      FPC_HIP(hipStreamSynchronize(stream()));
+      int rc = G->pc_solve(G->pc, G->A, &G->tmp_in, &G->tmp_out);
+      return rc & ~FPC_ERROR_CONV;
+    }
+    default:
+      return vec_copy(out, in, G->N);
+  }
+}
+
+}  // namespace
+}  // namespace fpc
+
+using namespace fpc;
+
+extern "C" {
+
+static int gmres_create(fpc_StructSolver *solver, bool flexible) {
+  FPC_CHECK_ARG(solver, "null handle pointer");
+  ensure_init();
+  fpc_StructSolver s = new fpc_StructSolver_s();
+  s->kind = flexible ? SK_FGMRES : SK_GMRES;
+  GmresImpl *G = new GmresImpl();
+  G->flexible = flexible;
+  s->impl = G;
+  *solver = s;
+  return 0;
+}
+
+int fpc_StructGMRESCreate(fpc_StructSolver *solver) { return gmres_create(solver, false); }
+int fpc_StructFlexGMRESCreate(fpc_StructSolver *solver) { return gmres_create(solver, true); }
+
+#define GIMPL(s)                                   \
+  GmresImpl *G = gimpl(s);                         \
+  FPC_CHECK_ARG(G, "not a GMRES solver handle")
+
+int fpc_StructGMRESSetTol(fpc_StructSolver s, double tol) { GIMPL(s); G->tol = tol; return 0; }
+int fpc_StructGMRESSetAbsoluteTol(fpc_StructSolver s, double atol) { GIMPL(s); G->atol = atol; return 0; }
+int fpc_StructGMRESSetMaxIter(fpc_StructSolver s, int maxit) { GIMPL(s); G->maxit = maxit; return 0; }
+int fpc_StructGMRESSetKDim(fpc_StructSolver s, int kdim) {
+  GIMPL(s);
+  FPC_CHECK_ARG(kdim >= 1 && kdim <= kMaxKDim, "kdim must be 1..128");
+  G->kdim = kdim;
+  return 0;
+}
+int fpc_StructGMRESSetPrintLevel(fpc_StructSolver s, int level) { GIMPL(s); G->print = level; return 0; }
+int fpc_StructGMRESSetLogging(fpc_StructSolver s, int logging) { GIMPL(s); G->logging = logging; return 0; }
+
+int fpc_StructGMRESSetPrecond(fpc_StructSolver s, fpc_PtrToStructSolverFcn solve, fpc_PtrToStructSolverFcn setup,
+                              fpc_StructSolver pc) {
+  GIMPL(s);
+  G->pc_solve = solve;
+  G->pc_setup = setup;
+  G->pc = pc;
+  if (!solve) G->pc_kind = PC_NONE;
+  else if (solve == fpc_StructJacobiSolve && pc && pc->kind == SK_JACOBI) G->pc_kind = PC_JACOBI;
+  else if (solve == fpc_StructPFMGSolve && pc && pc->kind == SK_PFMG) G->pc_kind = PC_PFMG;
+  else G->pc_kind = PC_CALLBACK;
+  return 0;
+}
+
+int fpc_StructGMRESSetup(fpc_StructSolver s, fpc_StructMatrix A, fpc_StructVector b, fpc_StructVector x) {
+  GIMPL(s);
+  FPC_CHECK_ARG(A && b && x, "null argument");
+  FPC_CHECK_ARG(A->initialized && b->initialized && x->initialized, "objects not initialized");
+  FPC_CHECK_ARG(b->grid.N == A->grid.N && x->grid.N == A->grid.N, "size mismatch");
+  G->A = A;
+  int rc = ensure_dinv(A);
+  if (rc) return rc;
+  long long N = A->grid.N;
+  if (G->N != N || G->kd != G->kdim) {
+    free_ws(G);
+    G->N = N;
+    G->kd = G->kdim;
+    G->vp = pad256(std::max(1LL, N), (int)sizeof(double));
+    FPC_HIP(hipMalloc(&G->V, (size_t)(G->kd + 1) * G->vp * sizeof(double)));
+    int nz = G->flexible ? G->kd : 1;
+    FPC_HIP(hipMalloc(&G->Z, (size_t)nz * G->vp * sizeof(double)));
+    FPC_HIP(hipMalloc(&G->W, (size_t)G->vp * sizeof(double)));
+    FPC_HIP(hipMalloc(&G->part, (size_t)NB * (KM + 1) * sizeof(double)));
+    FPC_HIP(hipMalloc(&G->dscal, 8 * sizeof(double)));
+    FPC_HIP(hipMalloc(&G->S, sizeof(State)));
+    FPC_HIP(hipHostMalloc(&G->hst, sizeof(Status)));
+    FPC_HIP(hipHostMalloc(&G->hscal, 8 * sizeof(double)));
+  }
+  G->tmp_in.grid = A->grid;
+  G->tmp_out.grid = A->grid;
+  G->tmp_in.initialized = G->tmp_out.initialized = true;
+  // Jacobi with one sweep from zero is fused into the Arnoldi matvec.
+  if (G->pc_kind == PC_JACOBI || G->pc_kind == PC_JACOBI1) {
+    JacobiImpl *J = (JacobiImpl *)G->pc->impl;
+    J->A = A;
+    G->pc_kind = (J->maxit <= 1 && !G->flexible) ? PC_JACOBI1 : PC_JACOBI;
+  }
+  if (G->pc_kind == PC_PFMG) {
+    rc = pfmg_setup(G->pc->impl, A);
+    if (rc) return rc;
+  }
+  if (G->pc_kind == PC_CALLBACK && G->pc_setup) {
+    rc = G->pc_setup(G->pc, A, b, x);
+    if (rc & ~FPC_ERROR_CONV) return rc;
+  }
+  FPC_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+// ||b - A x|| into V[0] (or W), and optionally ||b||; values land in hscal[0], hscal[1].
+static int true_residual(GmresImpl *G, const double *b, const double *x, double *rout, bool with_b) {
+  int rc = residual_norm_partials(G->A, b, x, rout, G->part);
+  if (rc) return rc;
+  rc = reduce_partials(G->part, NB, G->dscal);
+  if (rc) return rc;
+  if (with_b) {
+    rc = dot_partials(b, b, G->N, G->part + NB);
+    if (!rc) rc = reduce_partials(G->part + NB, NB, G->dscal + 1);
+    if (rc) return rc;
+  }
+  FPC_HIP(hipMemcpyAsync(G->hscal, G->dscal, 2 * sizeof(double), hipMemcpyDeviceToHost, stream()));
+  FPC_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+// One Arnoldi matvec with no preconditioner or with the fused one-sweep Jacobi.
+static int arnoldi_step(GmresImpl *G) {
+  Dims d = dims_of(G->A->grid);
+  dim3 g, b;
+  launch_shape(d, g, b);
+  const double *dinv = (G->pc_kind == PC_JACOBI1) ? G->A->dinv : nullptr;
+  k_arnoldi_matvec<<<g, b, 0, stream()>>>(G->A->coef, G->A->pitch, d.nx, d.ny, d.nz, G->V, G->vp, nullptr, dinv, G->S);
+  FPC_LAUNCH_CHECK();
+  return 0;
+}
+
+static int arnoldi_tail(GmresImpl *G) {
+  k_multidot<<<NB, 256, 0, stream()>>>(G->V, G->vp, G->N, G->S, G->part);
+  FPC_LAUNCH_CHECK();
+  k_reduce_h<<<KM + 1, 256, 0, stream()>>>(G->S, G->part);
+  FPC_LAUNCH_CHECK();
+  k_orth_update<<<NB, 256, 0, stream()>>>(G->V, G->vp, G->N, G->S, G->part);
+  FPC_LAUNCH_CHECK();
+  k_arnoldi_tail<<<1, 256, 0, stream()>>>(G->S, G->part);
+  FPC_LAUNCH_CHECK();
+  return 0;
+}
+
+// One Arnoldi step when the preconditioner runs separately; step = host-side index
+// of the current basis vector (used for the flexible Z slot).
+static int arnoldi_step_pc(GmresImpl *G, int step) {
+  Dims d = dims_of(G->A->grid);
+  dim3 g, b;
+  launch_shape(d, g, b);
+  int *skip = &G->S->st.done;
+  double *z = G->flexible ? G->Z + (long long)step * G->vp : G->Z;
+  // the preconditioner input is the stored (unnormalised) p_step
+  int rc = precond_apply(G, G->V + (long long)step * G->vp, z, skip);
+  if (rc) return rc;
+  k_arnoldi_matvec<<<g, b, 0, stream()>>>(G->A->coef, G->A->pitch, d.nx, d.ny, d.nz, G->V, G->vp, z, nullptr, G->S);
+  FPC_LAUNCH_CHECK();
+  return 0;
+}
+
+int fpc_StructGMRESSolve(fpc_StructSolver s, fpc_StructMatrix A, fpc_StructVector bv, fpc_StructVector xv) {
+  GIMPL(s);
+  FPC_CHECK_ARG(A && bv && xv && G->A == A && G->N == A->grid.N, "call Setup with this matrix first");
+  FPC_CHECK_ARG(bv->grid.N == G->N && xv->grid.N == G->N, "size mismatch");
+  const double *b = bv->data;
+  double *x = xv->data;
+  long long N = G->N;
+  int rc = ensure_dinv(A);
+  if (rc) return rc;
+  if (G->pc_kind == PC_PFMG) {
+    rc = pfmg_setup(G->pc->impl, A);  // no-op unless A changed
+    if (rc) return rc;
+  }
+  // initial residual into V[0]
+  rc = true_residual(G, b, x, G->V, true);
+  if (rc) return rc;
+  double r_norm = std::sqrt(G->hscal[0]);
+  double b_norm = std::sqrt(G->hscal[1]);
+  G->its = 0;
+  if (!std::isfinite(b_norm) || !std::isfinite(r_norm)) {
+    G->relres = NAN;
+    set_error(FPC_ERROR_GENERIC, "GMRES: non-finite right-hand side or initial residual");
+    return FPC_ERROR_GENERIC;
+  }
+  double den = (b_norm > 0.0) ? b_norm : r_norm;
+  double eps = std::max(G->atol, G->tol * den);
+  double real_old = b_norm;
+  int iter = 0;
+  bool converged = false;
+  unsigned nblk_n = (unsigned)std::max(1LL, std::min((N + 255) / 256, 65536LL));
+  bool first_cycle = true;
+  while (iter < G->maxit) {
+    if (r_norm == 0.0) { converged = true; break; }
+    if (r_norm <= eps) {
+      rc = true_residual(G, b, x, G->W, false);
+      if (rc) return rc;
+      r_norm = std::sqrt(G->hscal[0]);
+      if (r_norm <= eps) { converged = true; break; }
+      rc = vec_copy(G->V, G->W, N);
+      if (rc) return rc;
+    }
+    k_cycle_start<<<1, 1, 0, stream()>>>(G->S, r_norm, eps, iter, G->maxit, G->kd);
+    FPC_LAUNCH_CHECK();
+    // Arnoldi steps in batches; the status record says when to stop.
+    int issued = 0;
+    int batch = std::max(1, std::min(G->pred, G->kd));
+    if (G->pc_kind == PC_CALLBACK) batch = 1;
+    for (;;) {
+      int nb = std::min(batch, G->kd - issued);
+      nb = std::min(nb, G->maxit - iter - issued);
+      if (nb <= 0) nb = 1;
+      for (int q = 0; q < nb; ++q) {
+        if (G->pc_kind == PC_NONE || G->pc_kind == PC_JACOBI1) {
+          rc = arnoldi_step(G);
+        } else {
+          rc = arnoldi_step_pc(G, issued + q);
+        }
+        if (rc) return rc;
+        rc = arnoldi_tail(G);
+        if (rc) return rc;
+      }
+      issued += nb;
+      FPC_HIP(hipMemcpyAsync(G->hst, &G->S->st, sizeof(Status), hipMemcpyDeviceToHost, stream()));
+      FPC_HIP(hipStreamSynchronize(stream()));
+      if (G->hst->done) break;
+      if (issued >= G->kd) break;  // cannot happen without done, kept as a guard
+      batch = (G->pc_kind == PC_PFMG || G->pc_kind == PC_CALLBACK) ? 1 : 2;
+    }
+    int i = G->hst->i;
+    iter = G->hst->iter;
+    r_norm = G->hst->r_norm;
+    if (first_cycle) G->pred = std::max(1, i);
+    first_cycle = false;
+    // x update
+    k_backsolve<<<1, 1, 0, stream()>>>(G->S);
+    FPC_LAUNCH_CHECK();
+    if (G->flexible) {
+      k_combine<<<nblk_n, 256, 0, stream()>>>(G->Z, G->vp, N, G->S, nullptr, x, 1);
+      FPC_LAUNCH_CHECK();
+    } else if (G->pc_kind == PC_JACOBI1 || G->pc_kind == PC_NONE) {
+      k_combine<<<nblk_n, 256, 0, stream()>>>(G->V, G->vp, N, G->S, G->pc_kind == PC_JACOBI1 ? A->dinv : nullptr, x, 1);
+      FPC_LAUNCH_CHECK();
+    } else {
+      k_combine<<<nblk_n, 256, 0, stream()>>>(G->V, G->vp, N, G->S, nullptr, G->W, 0);
+      FPC_LAUNCH_CHECK();
+      rc = precond_apply(G, G->W, G->Z, nullptr);
+      if (rc) return rc;
+      rc = axpy_inplace(x, G->Z, N);
+      if (rc) return rc;
+    }
+    if (r_norm <= eps) {
+      rc = true_residual(G, b, x, G->V, false);
+      if (rc) return rc;
+      double real_new = std::sqrt(G->hscal[0]);
+      r_norm = real_new;
+      if (r_norm <= eps) { converged = true; break; }
+      if (real_new >= real_old) { converged = true; break; }
+      real_old = real_new;
+      continue;  // V[0] holds the true residual
+    }
+    // full cycle without convergence: Givens-based residual as the new p_0
+    k_restart_coefs<<<1, 1, 0, stream()>>>(G->S);
+    FPC_LAUNCH_CHECK();
+    k_restart_vec<<<nblk_n, 256, 0, stream()>>>(G->V, G->vp, N, G->S);
+    FPC_LAUNCH_CHECK();
+  }
+  FPC_HIP(hipStreamSynchronize(stream()));
+  G->its = iter;
+  G->relres = (b_norm > 0.0) ? r_norm / b_norm : r_norm;
+  if (G->print > 0) std::printf("GMRES: %d iterations, relative residual %.6e\n", iter, G->relres);
+  (void)converged;
+  if (iter >= G->maxit && r_norm > eps && eps > 0) {
+    set_error(FPC_ERROR_CONV, "GMRES: maximum iterations reached");
+    return FPC_ERROR_CONV;
+  }
+  return 0;
+}
+
+int fpc_StructGMRESGetNumIterations(fpc_StructSolver s, int *n) {
+  GIMPL(s);
+  FPC_CHECK_ARG(n, "null pointer");
+  *n = G->its;
+  return 0;
+}
+
+int fpc_StructGMRESGetFinalRelativeResidualNorm(fpc_StructSolver s, double *r) {
+  GIMPL(s);
+  FPC_CHECK_ARG(r, "null pointer");
+  *r = G->relres;
+  return 0;
+}
+
+int fpc_StructGMRESDestroy(fpc_StructSolver s) {
+  if (!s) return 0;
+  GmresImpl *G = gimpl(s);
+  if (G) {
+    free_ws(G);
+    delete G;
+  }
+  delete s;
+  return 0;
+}
+
+// ---------------------------------------------------------------- Jacobi handles
+int fpc_StructJacobiCreate(fpc_StructSolver *solver) {
+  FPC_CHECK_ARG(solver, "null handle pointer");
+  fpc_StructSolver s = new fpc_StructSolver_s();
+  s->kind = SK_JACOBI;
+  s->impl = new JacobiImpl();
+  *solver = s;
+  return 0;
+}
+
+#define JIMPL(s)                                                              \
+  FPC_CHECK_ARG(s && s->kind == SK_JACOBI, "not a Jacobi solver handle");     \
+  JacobiImpl *J = (JacobiImpl *)s->impl
+
+int fpc_StructJacobiSetMaxIter(fpc_StructSolver s, int maxit) { JIMPL(s); J->maxit = std::max(1, maxit); return 0; }
+int fpc_StructJacobiSetTol(fpc_StructSolver s, double tol) { JIMPL(s); J->tol = tol; return 0; }
+int fpc_StructJacobiSetZeroGuess(fpc_StructSolver s) { JIMPL(s); J->zero_guess = true; return 0; }
+int fpc_StructJacobiSetNonZeroGuess(fpc_StructSolver s) { JIMPL(s); J->zero_guess = false; return 0; }
+
+int fpc_StructJacobiSetup(fpc_StructSolver s, fpc_StructMatrix A, fpc_StructVector b, fpc_StructVector x) {
+  JIMPL(s);
+  FPC_CHECK_ARG(A && A->initialized, "matrix not initialized");
+  (void)b;
+  (void)x;
+  J->A = A;
+  return ensure_dinv(A);
+}
+
+// Stand-alone Jacobi: maxit sweeps, from zero when ZeroGuess is set, else from x.
+int fpc_StructJacobiSolve(fpc_StructSolver s, fpc_StructMatrix A, fpc_StructVector b, fpc_StructVector x) {
+  JIMPL(s);
+  FPC_CHECK_ARG(A && b && x && A->grid.N == b->grid.N && A->grid.N == x->grid.N, "bad arguments");
+  J->A = A;
+  int rc = ensure_dinv(A);
+  if (rc) return rc;
+  if (J->zero_guess) {
+    rc = jacobi_apply(J, b->data, x->data, nullptr);
+  } else {
+    long long n = A->grid.N;
+    double *t = scratch_doubles((size_t)n, 4);
+    if (!t) return FPC_ERROR_MEMORY;
+    Dims d = dims_of(A->grid);
+    dim3 g, bl;
+    launch_shape(d, g, bl);
+    for (int it = 0; it < J->maxit; ++it) {
+      k_jacobi_sweep<<<g, bl, 0, stream()>>>(A->coef, A->pitch, d.nx, d.ny, d.nz, A->dinv, b->data, x->data, t, nullptr);
+      FPC_LAUNCH_CHECK();
+      rc = vec_copy(x->data, t, n);
+      if (rc) return rc;
+    }
+  }
+  if (rc) return rc;
+  FPC_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+int fpc_StructJacobiDestroy(fpc_StructSolver s) {
+  if (!s) return 0;
+  if (s->kind == SK_JACOBI) delete (JacobiImpl *)s->impl;
+  delete s;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,796 @@
+// pfmg.hip -- PFMG-style semicoarsening multigrid (libfpc).
+//
+// Replaces HYPRE_StructPFMG* as the pressure preconditioner of the reference
+// (main.cu:213-220: MaxIter 1, Tol 0, zero guess, RelaxType, RAPType).  HYPRE's
+// PFMG internals are not part of the reference and its version is unpinned, so
+// this is a PFMG-style method with the build's own, fully stated definition:
+//
+//  * each level halves one direction d: the one with the largest total in-box
+//    coupling c_d = sum over rows (k, j, i order) of |a_-d| + |a_+d|; a later
+//    direction must exceed the best so far by a relative 1e-9 to win.  Coarse
+//    point c sits on fine index 2c+1, nc = floor(n/2);
+//  * interpolation at an F point (even index along d): w-/+ = -a_-/+d / centre,
+//    centre = diagonal + the couplings across d (collapsed); C points inject;
+//    restriction is the transpose;
+//  * coarse operator (non-Galerkin, 7-point on every level, RAP type 0 and 1):
+//    couplings across d are the Galerkin couplings with that cross offset summed
+//    over the offsets along d; couplings along d are the 1-D Galerkin product of
+//    the collapsed stencil; the centre keeps the Galerkin row sum.  For the
+//    7-point Laplacian this is twice the operator rediscretised on the
+//    semicoarsened grid, which keeps the V-cycle rate mesh-independent;
+//  * relaxation: Jacobi (type 0), weighted Jacobi with w = 2/(2 + c_d/sum c)
+//    (type 1), red/black Gauss-Seidel symmetric (2) or forward (3);
+//  * coarsening stops at <= 64 points; that level is solved exactly with a
+//    dense inverse held in FP64.
+//
+// The hierarchy is built once per matrix on the host in FP64 and uploaded as
+// x-contiguous planes; in FP32 mode every level's coefficients, smoothing,
+// restriction and prolongation run in FP32 (half the bytes of this
+// bandwidth-bound cycle) while the coarsest solve stays FP64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "fpc_internal.hpp"
+
+namespace fpc {
+namespace {
+
+constexpr int kCoarseMax = 64;
+
+inline int em(int d) { return 1 + 2 * d; }
+inline int ep(int d) { return 2 + 2 * d; }
+
+// ---------------------------------------------------------------- host hierarchy
+struct HLevel {
+  int n[3] = {0, 0, 0};
+  long long N = 0;
+  std::vector<double> a;  // 7 * N, slot-major planes
+  int cdir = -1;
+  double omega = 1.0;
+  std::vector<double> wm, wp;
+  double A(int s, long long r) const { return a[(size_t)s * (size_t)N + (size_t)r]; }
+};
+
+inline long long lin(const HLevel &L, int i, int j, int k) { return ((long long)k * L.n[1] + j) * L.n[0] + i; }
+
+// Coefficient of slot s at point p, zero when the neighbour leaves the box.
+inline double coef_inbox(const HLevel &L, const int *p, int s) {
+  if (s != 0) {
+    int d = (s - 1) / 2, sg = (s & 1) ? -1 : 1;
+    int q = p[d] + sg;
+    if (q < 0 || q >= L.n[d]) return 0.0;
+  }
+  return L.A(s, lin(L, p[0], p[1], p[2]));
+}
+
+void choose_direction(HLevel &L) {
+  double c[3] = {0.0, 0.0, 0.0};
+  int p[3];
+  for (p[2] = 0; p[2] < L.n[2]; ++p[2])
+    for (p[1] = 0; p[1] < L.n[1]; ++p[1])
+      for (p[0] = 0; p[0] < L.n[0]; ++p[0])
+        for (int d = 0; d < 3; ++d) c[d] += std::abs(coef_inbox(L, p, em(d))) + std::abs(coef_inbox(L, p, ep(d)));
+  L.cdir = -1;
+  for (int d = 0; d < 3; ++d) {
+    if (L.n[d] < 2) continue;
+    if (L.cdir < 0 || c[d] > c[L.cdir] * (1.0 + 1.0e-9)) L.cdir = d;
+  }
+  double tot = c[0] + c[1] + c[2];
+  double s = (L.cdir >= 0 && tot > 0.0) ? c[L.cdir] / tot : 1.0;
+  L.omega = 2.0 / (2.0 + s);
+}
+
+void interp_weights(HLevel &L) {
+  int d = L.cdir;
+  L.wm.assign((size_t)L.N, 0.0);
+  L.wp.assign((size_t)L.N, 0.0);
+  int p[3];
+  for (p[2] = 0; p[2] < L.n[2]; ++p[2])
+    for (p[1] = 0; p[1] < L.n[1]; ++p[1])
+      for (p[0] = 0; p[0] < L.n[0]; ++p[0]) {
+        if (p[d] & 1) continue;
+        long long r = lin(L, p[0], p[1], p[2]);
+        double ctr = L.A(0, r);
+        for (int s = 1; s < kSlots; ++s)
+          if (s != em(d) && s != ep(d)) ctr += coef_inbox(L, p, s);
+        if (ctr != 0.0) {
+          L.wm[(size_t)r] = -coef_inbox(L, p, em(d)) / ctr;
+          L.wp[(size_t)r] = -coef_inbox(L, p, ep(d)) / ctr;
+        }
+      }
+}
+
+HLevel coarsen(const HLevel &F) {
+  int d = F.cdir;
+  HLevel C;
+  for (int q = 0; q < 3; ++q) C.n[q] = F.n[q];
+  C.n[d] = F.n[d] / 2;
+  C.N = (long long)C.n[0] * C.n[1] * C.n[2];
+  C.a.assign((size_t)kSlots * (size_t)C.N, 0.0);
+  int cp[3];
+  for (cp[2] = 0; cp[2] < C.n[2]; ++cp[2])
+    for (cp[1] = 0; cp[1] < C.n[1]; ++cp[1])
+      for (cp[0] = 0; cp[0] < C.n[0]; ++cp[0]) {
+        int f0[3] = {cp[0], cp[1], cp[2]};
+        f0[d] = 2 * cp[d] + 1;
+        int fm[3] = {f0[0], f0[1], f0[2]}, fp[3] = {f0[0], f0[1], f0[2]};
+        fm[d] -= 1;
+        fp[d] += 1;
+        bool hasp = fp[d] < F.n[d];
+        long long rfm = lin(F, fm[0], fm[1], fm[2]);
+        long long rfp = hasp ? lin(F, fp[0], fp[1], fp[2]) : -1;
+        double rm = F.wp[(size_t)rfm];
+        double rp = hasp ? F.wm[(size_t)rfp] : 0.0;
+        double a0m = coef_inbox(F, fm, 0), a0p = hasp ? coef_inbox(F, fp, 0) : 0.0;
+        double ac = coef_inbox(F, f0, 0) + rm * (coef_inbox(F, f0, em(d)) + coef_inbox(F, fm, ep(d)) + rm * a0m);
+        if (hasp) ac += rp * (coef_inbox(F, f0, ep(d)) + coef_inbox(F, fp, em(d)) + rp * a0p);
+        // along d: Galerkin form (for the row sum) and the 1-D Galerkin product of
+        // the collapsed stencil (what is stored)
+        double wmm = F.wm[(size_t)rfm];
+        double amG = coef_inbox(F, f0, em(d)) * wmm + rm * a0m * wmm + rm * coef_inbox(F, fm, em(d));
+        double cmm = a0m, cpp = a0p;
+        for (int s = 1; s < kSlots; ++s) {
+          if (s == em(d) || s == ep(d)) continue;
+          cmm += coef_inbox(F, fm, s);
+          if (hasp) cpp += coef_inbox(F, fp, s);
+        }
+        double am = coef_inbox(F, f0, em(d)) * wmm + rm * cmm * wmm + rm * coef_inbox(F, fm, em(d));
+        double apG = 0.0, apv = 0.0;
+        if (hasp) {
+          double wpp = F.wp[(size_t)rfp];
+          apG = coef_inbox(F, f0, ep(d)) * wpp + rp * a0p * wpp + rp * coef_inbox(F, fp, ep(d));
+          apv = coef_inbox(F, f0, ep(d)) * wpp + rp * cpp * wpp + rp * coef_inbox(F, fp, ep(d));
+        }
+        long long rc = lin(C, cp[0], cp[1], cp[2]);
+        // across d: Galerkin couplings with this cross offset, summed over the offsets along d
+        for (int s = 1; s < kSlots; ++s) {
+          if (s == em(d) || s == ep(d)) continue;
+          int e = (s - 1) / 2, sg = (s & 1) ? -1 : 1;
+          double v = coef_inbox(F, f0, s);
+          double cm = coef_inbox(F, fm, s);
+          if (cm != 0.0) {
+            int q[3] = {fm[0], fm[1], fm[2]};
+            q[e] += sg;
+            long long rq = lin(F, q[0], q[1], q[2]);
+            v += rm * cm * (F.wm[(size_t)rq] + F.wp[(size_t)rq]);
+          }
+          if (hasp) {
+            double cq = coef_inbox(F, fp, s);
+            if (cq != 0.0) {
+              int q[3] = {fp[0], fp[1], fp[2]};
+              q[e] += sg;
+              long long rq = lin(F, q[0], q[1], q[2]);
+              v += rp * cq * (F.wm[(size_t)rq] + F.wp[(size_t)rq]);
+            }
+          }
+          C.a[(size_t)s * (size_t)C.N + (size_t)rc] = v;
+        }
+        // centre: keeps the Galerkin row sum
+        C.a[(size_t)rc] = ac + (amG - am) + (apG - apv);
+        C.a[(size_t)em(d) * (size_t)C.N + (size_t)rc] = am;
+        C.a[(size_t)ep(d) * (size_t)C.N + (size_t)rc] = apv;
+      }
+  return C;
+}
+
+// Dense inverse of the coarsest operator (Gauss-Jordan with partial pivoting).
+std::vector<double> dense_inverse(const HLevel &L) {
+  int n = (int)L.N;
+  std::vector<double> M((size_t)n * n, 0.0), Inv((size_t)n * n, 0.0);
+  int p[3];
+  for (p[2] = 0; p[2] < L.n[2]; ++p[2])
+    for (p[1] = 0; p[1] < L.n[1]; ++p[1])
+      for (p[0] = 0; p[0] < L.n[0]; ++p[0]) {
+        long long r = lin(L, p[0], p[1], p[2]);
+        M[(size_t)r * n + r] += L.A(0, r);
+        for (int s = 1; s < kSlots; ++s) {
+          int d = (s - 1) / 2, sg = (s & 1) ? -1 : 1;
+          int q[3] = {p[0], p[1], p[2]};
+          q[d] += sg;
+          if (q[d] < 0 || q[d] >= L.n[d]) continue;
+          M[(size_t)r * n + lin(L, q[0], q[1], q[2])] += L.A(s, r);
+        }
+      }
+  for (int i = 0; i < n; ++i) Inv[(size_t)i * n + i] = 1.0;
+  for (int c = 0; c < n; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < n; ++r)
+      if (std::abs(M[(size_t)r * n + c]) > std::abs(M[(size_t)piv * n + c])) piv = r;
+    if (piv != c)
+      for (int t = 0; t < n; ++t) {
+        std::swap(M[(size_t)c * n + t], M[(size_t)piv * n + t]);
+        std::swap(Inv[(size_t)c * n + t], Inv[(size_t)piv * n + t]);
+      }
+    double dg = M[(size_t)c * n + c];
+    if (dg == 0.0) continue;
+    for (int t = 0; t < n; ++t) {
+      M[(size_t)c * n + t] /= dg;
+      Inv[(size_t)c * n + t] /= dg;
+    }
+    for (int r = 0; r < n; ++r) {
+      if (r == c) continue;
+      double f = M[(size_t)r * n + c];
+      if (f == 0.0) continue;
+      for (int t = 0; t < n; ++t) {
+        M[(size_t)r * n + t] -= f * M[(size_t)c * n + t];
+        Inv[(size_t)r * n + t] -= f * Inv[(size_t)c * n + t];
+      }
+    }
+  }
+  return Inv;
+}
+
+// ---------------------------------------------------------------- device kernels
+template <typename T>
+__global__ void k_relax_zero(const T *__restrict__ dinv, const T *__restrict__ b, T *__restrict__ x, T w, long long n,
+                             const int *skip) {
+  if (skip && *skip) return;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+    x[t] = w * b[t] * dinv[t];
+}
+
+template <typename T>
+__device__ inline T apply7(const T *__restrict__ c, long long pitch, long long r, int i, int j, int k, int nx, int ny,
+                           int nz, const T *__restrict__ x) {
+  long long pl = (long long)nx * ny;
+  T s = c[r] * x[r];
+  if (i > 0) s += c[pitch + r] * x[r - 1];
+  if (i < nx - 1) s += c[2 * pitch + r] * x[r + 1];
+  if (j > 0) s += c[3 * pitch + r] * x[r - nx];
+  if (j < ny - 1) s += c[4 * pitch + r] * x[r + nx];
+  if (k > 0) s += c[5 * pitch + r] * x[r - pl];
+  if (k < nz - 1) s += c[6 * pitch + r] * x[r + pl];
+  return s;
+}
+
+template <typename T>
+__global__ void k_residual_mg(const T *__restrict__ c, long long pitch, int nx, int ny, int nz, const T *__restrict__ b,
+                              const T *__restrict__ x, T *__restrict__ r, const int *skip) {
+  if (skip && *skip) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int k = blockIdx.z;
+  if (i >= nx || j >= ny) return;
+  long long t = ((long long)k * ny + j) * nx + i;
+  r[t] = b[t] - apply7(c, pitch, t, i, j, k, nx, ny, nz, x);
+}
+
+template <typename T>
+__global__ void k_relax_jacobi(const T *__restrict__ c, long long pitch, int nx, int ny, int nz,
+                               const T *__restrict__ dinv, const T *__restrict__ b, const T *__restrict__ x,
+                               T *__restrict__ y, T w, const int *skip) {
+  if (skip && *skip) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int k = blockIdx.z;
+  if (i >= nx || j >= ny) return;
+  long long t = ((long long)k * ny + j) * nx + i;
+  y[t] = x[t] + w * (b[t] - apply7(c, pitch, t, i, j, k, nx, ny, nz, x)) * dinv[t];
+}
+
+// Red/black sweep of one colour, in place.  first: the other colour is zero and
+// this colour starts from zero (zero-guess first half-sweep).
+template <typename T>
+__global__ void k_relax_rb(const T *__restrict__ c, long long pitch, int nx, int ny, int nz, const T *__restrict__ dinv,
+                           const T *__restrict__ b, T *__restrict__ x, int color, int first, const int *skip) {
+  if (skip && *skip) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int k = blockIdx.z;
+  if (i >= nx || j >= ny) return;
+  long long t = ((long long)k * ny + j) * nx + i, pl = (long long)nx * ny;
+  if (((i + j + k) & 1) != color) {
+    if (first) x[t] = (T)0;
+    return;
+  }
+  if (first) {
+    x[t] = b[t] * dinv[t];
+    return;
+  }
+  T s = b[t];
+  if (i > 0) s -= c[pitch + t] * x[t - 1];
+  if (i < nx - 1) s -= c[2 * pitch + t] * x[t + 1];
+  if (j > 0) s -= c[3 * pitch + t] * x[t - nx];
+  if (j < ny - 1) s -= c[4 * pitch + t] * x[t + nx];
+  if (k > 0) s -= c[5 * pitch + t] * x[t - pl];
+  if (k < nz - 1) s -= c[6 * pitch + t] * x[t + pl];
+  x[t] = s * dinv[t];
+}
+
+// b_c = r(F0) + w+(Fm) r(Fm) + w-(Fp) r(Fp), one thread per coarse point.
+template <typename T>
+__global__ void k_restrict(int fnx, int fny, int fnz, int d, const T *__restrict__ r, const T *__restrict__ wm,
+                           const T *__restrict__ wp, int cnx, int cny, int cnz, T *__restrict__ bc, const int *skip) {
+  if (skip && *skip) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int k = blockIdx.z;
+  if (i >= cnx || j >= cny) return;
+  int p[3] = {i, j, k};
+  int fn[3] = {fnx, fny, fnz};
+  p[d] = 2 * p[d] + 1;
+  long long stride = (d == 0) ? 1 : (d == 1 ? (long long)fnx : (long long)fnx * fny);
+  long long f0 = ((long long)p[2] * fny + p[1]) * fnx + p[0];
+  T v = r[f0] + wp[f0 - stride] * r[f0 - stride];
+  if (p[d] + 1 < fn[d]) v += wm[f0 + stride] * r[f0 + stride];
+  bc[((long long)k * cny + j) * cnx + i] = v;
+}
+
+// x_f += P x_c, one thread per fine point.
+template <typename T>
+__global__ void k_prolong_add(int fnx, int fny, int fnz, int d, const T *__restrict__ wm, const T *__restrict__ wp,
+                              const T *__restrict__ xc, int cnx, int cny, int cnz, T *__restrict__ xf,
+                              const int *skip) {
+  if (skip && *skip) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int k = blockIdx.z;
+  if (i >= fnx || j >= fny) return;
+  int p[3] = {i, j, k};
+  int nc[3] = {cnx, cny, cnz};
+  long long t = ((long long)k * fny + j) * fnx + i;
+  long long cstride = (d == 0) ? 1 : (d == 1 ? (long long)cnx : (long long)cnx * cny);
+  int q[3] = {i, j, k};
+  if (p[d] & 1) {
+    q[d] = (p[d] - 1) >> 1;
+    xf[t] += xc[((long long)q[2] * cny + q[1]) * cnx + q[0]];
+  } else {
+    q[d] = p[d] >> 1;  // right coarse neighbour
+    long long cr = ((long long)q[2] * cny + q[1]) * cnx + q[0];
+    T s = (T)0;
+    if (q[d] - 1 >= 0) s += wm[t] * xc[cr - cstride];
+    if (q[d] < nc[d]) s += wp[t] * xc[cr];
+    xf[t] += s;
+  }
+}
+
+// x = Ainv b on the coarsest level (FP64 inverse), one block.
+template <typename T>
+__global__ void k_coarse_solve(const double *__restrict__ inv, int n, const T *__restrict__ b, T *__restrict__ x,
+                               const int *skip) {
+  if (skip && *skip) return;
+  __shared__ double bs[kCoarseMax];
+  for (int t = threadIdx.x; t < n; t += blockDim.x) bs[t] = (double)b[t];
+  __syncthreads();
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    double s = 0.0;
+    for (int q = 0; q < n; ++q) s += inv[(size_t)r * n + q] * bs[q];
+    x[r] = (T)s;
+  }
+}
+
+template <typename S, typename D>
+__global__ void k_convert(const S *__restrict__ s, D *__restrict__ d, long long n, const int *skip) {
+  if (skip && *skip) return;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+    d[t] = (D)s[t];
+}
+
+template <typename T>
+__global__ void k_copy_t(const T *__restrict__ s, T *__restrict__ d, long long n, const int *skip) {
+  if (skip && *skip) return;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+    d[t] = s[t];
+}
+
+unsigned nb(long long n) { return (unsigned)std::max(1LL, std::min((n + 255) / 256, 65536LL)); }
+
+// ---------------------------------------------------------------- device hierarchy
+struct DLevel {
+  Dims d;
+  int cdir = -1;
+  double omega = 1.0;
+  long long pitch = 0;
+  void *coef = nullptr;  // owned unless level 0 borrows the matrix planes
+  bool own_coef = true;
+  void *dinv = nullptr;
+  bool own_dinv = true;
+  void *wm = nullptr, *wp = nullptr;
+  void *x = nullptr, *y = nullptr, *b = nullptr, *r = nullptr, *tmp = nullptr;
+};
+
+struct PfmgImpl {
+  int maxit = 1;
+  double tol = 0.0;
+  int relax = 1, rap = 1, npre = 1, npost = 1, maxlev = 0, print = 0, bits = 64;
+  bool zero_guess = false;
+  int its = 0;
+  fpc_StructMatrix_s *A = nullptr;
+  unsigned long long version = ~0ULL;
+  int built_bits = 0;
+  std::vector<DLevel> L;
+  double *inv = nullptr;
+  int ncoarse = 0;
+  double *w64 = nullptr;  // FP64 work (stand-alone solves)
+  double *e64 = nullptr;
+};
+
+void free_levels(PfmgImpl *P) {
+  for (auto &l : P->L) {
+    if (l.own_coef && l.coef) (void)hipFree(l.coef);
+    if (l.own_dinv && l.dinv) (void)hipFree(l.dinv);
+    for (void *q : {l.wm, l.wp, l.x, l.y, l.b, l.r, l.tmp})
+      if (q) (void)hipFree(q);
+  }
+  P->L.clear();
+  if (P->inv) (void)hipFree(P->inv);
+  P->inv = nullptr;
+  if (P->w64) (void)hipFree(P->w64);
+  if (P->e64) (void)hipFree(P->e64);
+  P->w64 = P->e64 = nullptr;
+}
+
+template <typename T>
+int upload(void **dst, const std::vector<double> &src, size_t count, size_t n_alloc) {
+  std::vector<T> tmp(n_alloc, (T)0);
+  for (size_t q = 0; q < count; ++q) tmp[q] = (T)src[q];
+  FPC_HIP(hipMalloc(dst, n_alloc * sizeof(T)));
+  FPC_HIP(hipMemcpy(*dst, tmp.data(), n_alloc * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+template <typename T>
+int alloc_t(void **dst, long long n) {
+  FPC_HIP(hipMalloc(dst, (size_t)std::max(1LL, n) * sizeof(T)));
+  FPC_HIP(hipMemset(*dst, 0, (size_t)std::max(1LL, n) * sizeof(T)));
+  return 0;
+}
+
+template <typename T>
+int build_device(PfmgImpl *P, std::vector<HLevel> &H) {
+  int rc;
+  for (size_t l = 0; l < H.size(); ++l) {
+    HLevel &h = H[l];
+    DLevel dl;
+    dl.d.nx = h.n[0]; dl.d.ny = h.n[1]; dl.d.nz = h.n[2]; dl.d.N = h.N;
+    dl.cdir = h.cdir;
+    dl.omega = h.omega;
+    dl.pitch = pad256(std::max(1LL, h.N), (int)sizeof(T));
+    if (l == 0) {
+      // Level 0 borrows the matrix planes (FP64 or the FP32 copy).
+      if (sizeof(T) == 8) {
+        dl.coef = P->A->coef;
+        dl.dinv = P->A->dinv;
+        dl.pitch = P->A->pitch;
+      } else {
+        rc = ensure_fp32(P->A);
+        if (rc) return rc;
+        dl.coef = P->A->coef32;
+        dl.dinv = P->A->dinv32;
+        dl.pitch = P->A->pitch;
+      }
+      dl.own_coef = dl.own_dinv = false;
+    } else {
+      std::vector<double> planes((size_t)kSlots * (size_t)dl.pitch, 0.0), dv((size_t)dl.pitch, 0.0);
+      for (int s = 0; s < kSlots; ++s)
+        for (long long r = 0; r < h.N; ++r) planes[(size_t)s * (size_t)dl.pitch + (size_t)r] = h.A(s, r);
+      for (long long r = 0; r < h.N; ++r) {
+        double a0 = h.A(0, r);
+        dv[(size_t)r] = (a0 != 0.0) ? 1.0 / a0 : 0.0;
+      }
+      if ((rc = upload<T>(&dl.coef, planes, planes.size(), planes.size()))) return rc;
+      if ((rc = upload<T>(&dl.dinv, dv, dv.size(), dv.size()))) return rc;
+    }
+    if (h.cdir >= 0) {
+      if ((rc = upload<T>(&dl.wm, h.wm, (size_t)h.N, (size_t)h.N))) return rc;
+      if ((rc = upload<T>(&dl.wp, h.wp, (size_t)h.N, (size_t)h.N))) return rc;
+    }
+    if ((rc = alloc_t<T>(&dl.x, h.N))) return rc;
+    if ((rc = alloc_t<T>(&dl.y, h.N))) return rc;
+    if ((rc = alloc_t<T>(&dl.r, h.N))) return rc;
+    if ((rc = alloc_t<T>(&dl.tmp, h.N))) return rc;
+    if (l > 0 || sizeof(T) != 8)
+      if ((rc = alloc_t<T>(&dl.b, h.N))) return rc;
+    P->L.push_back(dl);
+  }
+  return 0;
+}
+
+int build(PfmgImpl *P, fpc_StructMatrix_s *A) {
+  free_levels(P);
+  P->A = A;
+  int rc = ensure_dinv(A);
+  if (rc) return rc;
+  std::vector<HLevel> H(1);
+  HLevel &h0 = H[0];
+  h0.n[0] = A->grid.n[0]; h0.n[1] = A->grid.n[1]; h0.n[2] = A->grid.n[2];
+  h0.N = A->grid.N;
+  h0.a.assign((size_t)kSlots * (size_t)h0.N, 0.0);
+  for (int s = 0; s < kSlots; ++s)
+    FPC_HIP(hipMemcpyAsync(h0.a.data() + (size_t)s * (size_t)h0.N, A->coef + (long long)s * A->pitch,
+                           (size_t)h0.N * sizeof(double), hipMemcpyDeviceToHost, stream()));
+  FPC_HIP(hipStreamSynchronize(stream()));
+  for (;;) {
+    HLevel &h = H.back();
+    choose_direction(h);
+    // MaxLevels only stops coarsening once the level is small enough for the
+    // dense coarsest solve.
+    bool lev_cap = P->maxlev > 0 && (int)H.size() >= P->maxlev && h.N <= kCoarseMax;
+    if (h.N <= kCoarseMax || h.cdir < 0 || lev_cap) {
+      h.cdir = -1;
+      break;
+    }
+    interp_weights(h);
+    HLevel c = coarsen(h);
+    H.push_back(std::move(c));
+  }
+  std::vector<double> inv = dense_inverse(H.back());
+  P->ncoarse = (int)H.back().N;
+  if ((rc = upload<double>((void **)&P->inv, inv, inv.size(), std::max<size_t>(1, inv.size())))) return rc;
+  rc = (P->bits == 32) ? build_device<float>(P, H) : build_device<double>(P, H);
+  if (rc) return rc;
+  P->version = A->version;
+  P->built_bits = P->bits;
+  return 0;
+}
+
+// One V-cycle with zero initial guess: out = M^-1 in.
+template <typename T>
+int vcycle(PfmgImpl *P, const double *in, double *out, const int *skip) {
+  hipStream_t st = stream();
+  int nl = (int)P->L.size();
+  bool fp64 = sizeof(T) == 8;
+  DLevel &L0 = P->L[0];
+  const T *b0;
+  if (fp64) {
+    b0 = (const T *)in;
+  } else {
+    k_convert<double, T><<<nb(L0.d.N), 256, 0, st>>>(in, (T *)L0.b, L0.d.N, skip);
+    FPC_LAUNCH_CHECK();
+    b0 = (const T *)L0.b;
+  }
+  bool rb = P->relax >= 2;
+  // level-0 solution buffer: the output itself in FP64
+  T *y0 = fp64 ? (T *)out : (T *)L0.y;
+  auto bof = [&](int l) -> const T * { return l == 0 ? b0 : (const T *)P->L[(size_t)l].b; };
+  auto yof = [&](int l) -> T * { return l == 0 ? y0 : (T *)P->L[(size_t)l].y; };
+  // red/black relaxes in place (x == y); Jacobi smooths x into y
+  auto xof = [&](int l) -> T * { return rb ? yof(l) : (T *)P->L[(size_t)l].x; };
+  auto w_of = [&](int l) -> T { return (T)(P->relax == 0 ? 1.0 : P->L[(size_t)l].omega); };
+  for (int l = 0; l < nl - 1; ++l) {
+    DLevel &F = P->L[(size_t)l];
+    DLevel &C = P->L[(size_t)l + 1];
+    dim3 g, bl;
+    launch_shape(F.d, g, bl);
+    const T *c = (const T *)F.coef;
+    const T *dv = (const T *)F.dinv;
+    T *x = xof(l);
+    if (rb) {
+      for (int sw = 0; sw < P->npre; ++sw) {
+        int c0 = 0, c1 = 1;
+        k_relax_rb<T><<<g, bl, 0, st>>>(c, F.pitch, F.d.nx, F.d.ny, F.d.nz, dv, bof(l), x, c0, sw == 0, skip);
+        k_relax_rb<T><<<g, bl, 0, st>>>(c, F.pitch, F.d.nx, F.d.ny, F.d.nz, dv, bof(l), x, c1, 0, skip);
+      }
+    } else {
+      k_relax_zero<T><<<nb(F.d.N), 256, 0, st>>>(dv, bof(l), x, w_of(l), F.d.N, skip);
+      for (int sw = 1; sw < P->npre; ++sw) {
+        k_relax_jacobi<T><<<g, bl, 0, st>>>(c, F.pitch, F.d.nx, F.d.ny, F.d.nz, dv, bof(l), x, (T *)F.tmp, w_of(l), skip);
+        k_copy_t<T><<<nb(F.d.N), 256, 0, st>>>((const T *)F.tmp, x, F.d.N, skip);
+      }
+    }
+    FPC_LAUNCH_CHECK();
+    k_residual_mg<T><<<g, bl, 0, st>>>(c, F.pitch, F.d.nx, F.d.ny, F.d.nz, bof(l), x, (T *)F.r, skip);
+    dim3 gc, bc;
+    launch_shape(C.d, gc, bc);
+    k_restrict<T><<<gc, bc, 0, st>>>(F.d.nx, F.d.ny, F.d.nz, F.cdir, (const T *)F.r, (const T *)F.wm, (const T *)F.wp,
+                                    C.d.nx, C.d.ny, C.d.nz, (T *)C.b, skip);
+    FPC_LAUNCH_CHECK();
+  }
+  {
+    int lc = nl - 1;
+    k_coarse_solve<T><<<1, 64, 0, st>>>(P->inv, P->ncoarse, bof(lc), yof(lc), skip);
+    FPC_LAUNCH_CHECK();
+  }
+  for (int l = nl - 2; l >= 0; --l) {
+    DLevel &F = P->L[(size_t)l];
+    DLevel &C = P->L[(size_t)l + 1];
+    dim3 g, bl;
+    launch_shape(F.d, g, bl);
+    const T *c = (const T *)F.coef;
+    const T *dv = (const T *)F.dinv;
+    T *x = xof(l);
+    k_prolong_add<T><<<g, bl, 0, st>>>(F.d.nx, F.d.ny, F.d.nz, F.cdir, (const T *)F.wm, (const T *)F.wp,
+                                       (const T *)yof(l + 1), C.d.nx, C.d.ny, C.d.nz, x, skip);
+    if (rb) {
+      for (int sw = 0; sw < P->npost; ++sw) {
+        int c0 = (P->relax == 2) ? 1 : 0, c1 = 1 - c0;
+        k_relax_rb<T><<<g, bl, 0, st>>>(c, F.pitch, F.d.nx, F.d.ny, F.d.nz, dv, bof(l), x, c0, 0, skip);
+        k_relax_rb<T><<<g, bl, 0, st>>>(c, F.pitch, F.d.nx, F.d.ny, F.d.nz, dv, bof(l), x, c1, 0, skip);
+      }
+      // x == y for red/black
+    } else {
+      T *y = yof(l);
+      if (P->npost <= 0) {
+        k_copy_t<T><<<nb(F.d.N), 256, 0, st>>>(x, y, F.d.N, skip);
+      }
+      for (int sw = 0; sw < P->npost; ++sw) {
+        bool last = (sw == P->npost - 1);
+        T *dst = last ? y : (T *)F.tmp;
+        k_relax_jacobi<T><<<g, bl, 0, st>>>(c, F.pitch, F.d.nx, F.d.ny, F.d.nz, dv, bof(l), x, dst, w_of(l), skip);
+        if (!last) k_copy_t<T><<<nb(F.d.N), 256, 0, st>>>((const T *)F.tmp, x, F.d.N, skip);
+      }
+    }
+    FPC_LAUNCH_CHECK();
+  }
+  if (!fp64) {
+    k_convert<T, double><<<nb(L0.d.N), 256, 0, st>>>((const T *)L0.y, out, L0.d.N, skip);
+    FPC_LAUNCH_CHECK();
+  }
+  return 0;
+}
+
+PfmgImpl *pimpl(fpc_StructSolver s) {
+  if (!s || s->kind != SK_PFMG) return nullptr;
+  return (PfmgImpl *)s->impl;
+}
+
+}  // namespace
+
+int pfmg_setup(void *pf, fpc_StructMatrix_s *A) {
+  PfmgImpl *P = (PfmgImpl *)pf;
+  if (P->A == A && P->version == A->version && P->built_bits == P->bits && !P->L.empty()) return 0;
+  return build(P, A);
+}
+
+int pfmg_apply(void *pf, const double *b, double *x, const int *skip) {
+  PfmgImpl *P = (PfmgImpl *)pf;
+  if (P->L.empty()) {
+    set_error(FPC_ERROR_GENERIC, "PFMG used before setup");
+    return FPC_ERROR_GENERIC;
+  }
+  int rc = (P->built_bits == 32) ? vcycle<float>(P, b, x, skip) : vcycle<double>(P, b, x, skip);
+  if (rc) return rc;
+  // further cycles as a stationary iteration (MaxIter > 1 as a preconditioner)
+  for (int it = 1; it < P->maxit; ++it) {
+    long long n = P->A->grid.N;
+    if (!P->w64) FPC_HIP(hipMalloc(&P->w64, (size_t)std::max(1LL, n) * sizeof(double)));
+    if (!P->e64) FPC_HIP(hipMalloc(&P->e64, (size_t)std::max(1LL, n) * sizeof(double)));
+    if ((rc = residual_norm_partials(P->A, b, x, P->w64, scratch_doubles(kRedBlocks, 5)))) return rc;
+    rc = (P->built_bits == 32) ? vcycle<float>(P, P->w64, P->e64, skip) : vcycle<double>(P, P->w64, P->e64, skip);
+    if (rc) return rc;
+    if ((rc = axpy_inplace(x, P->e64, n))) return rc;
+  }
+  return 0;
+}
+
+}  // namespace fpc
+
+using namespace fpc;
+
+extern "C" {
+
+int fpc_StructPFMGCreate(fpc_StructSolver *solver) {
+  FPC_CHECK_ARG(solver, "null handle pointer");
+  ensure_init();
+  fpc_StructSolver s = new fpc_StructSolver_s();
+  s->kind = SK_PFMG;
+  s->impl = new PfmgImpl();
+  *solver = s;
+  return 0;
+}
+
+#define PIMPL(s)                                  \
+  PfmgImpl *P = pimpl(s);                         \
+  FPC_CHECK_ARG(P, "not a PFMG solver handle")
+
+int fpc_StructPFMGSetMaxIter(fpc_StructSolver s, int maxit) { PIMPL(s); P->maxit = std::max(1, maxit); return 0; }
+int fpc_StructPFMGSetTol(fpc_StructSolver s, double tol) { PIMPL(s); P->tol = tol; return 0; }
+int fpc_StructPFMGSetRelaxType(fpc_StructSolver s, int t) {
+  PIMPL(s);
+  FPC_CHECK_ARG(t >= 0 && t <= 3, "relax type must be 0..3");
+  P->relax = t;
+  return 0;
+}
+int fpc_StructPFMGSetRAPType(fpc_StructSolver s, int t) {
+  PIMPL(s);
+  FPC_CHECK_ARG(t == 0 || t == 1, "RAP type must be 0 or 1");
+  P->rap = t;
+  return 0;
+}
+int fpc_StructPFMGSetNumPreRelax(fpc_StructSolver s, int n) { PIMPL(s); P->npre = std::max(1, n); return 0; }
+int fpc_StructPFMGSetNumPostRelax(fpc_StructSolver s, int n) { PIMPL(s); P->npost = std::max(0, n); return 0; }
+int fpc_StructPFMGSetMaxLevels(fpc_StructSolver s, int n) { PIMPL(s); P->maxlev = std::max(0, n); return 0; }
+int fpc_StructPFMGSetPrintLevel(fpc_StructSolver s, int level) { PIMPL(s); P->print = level; return 0; }
+int fpc_StructPFMGSetZeroGuess(fpc_StructSolver s) { PIMPL(s); P->zero_guess = true; return 0; }
+int fpc_StructPFMGSetNonZeroGuess(fpc_StructSolver s) { PIMPL(s); P->zero_guess = false; return 0; }
+int fpc_StructPFMGSetPrecision(fpc_StructSolver s, int bits) {
+  PIMPL(s);
+  FPC_CHECK_ARG(bits == 32 || bits == 64, "precision must be 32 or 64");
+  P->bits = bits;
+  return 0;
+}
+
+int fpc_StructPFMGSetup(fpc_StructSolver s, fpc_StructMatrix A, fpc_StructVector b, fpc_StructVector x) {
+  PIMPL(s);
+  FPC_CHECK_ARG(A && A->initialized, "matrix not initialized");
+  (void)b;
+  (void)x;
+  int rc = pfmg_setup(P, A);
+  if (rc) return rc;
+  FPC_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+// Stand-alone solve: MaxIter V-cycles (x <- x + M^-1 (b - A x)), stopping when
+// ||r||/||b|| <= Tol if Tol > 0.
+int fpc_StructPFMGSolve(fpc_StructSolver s, fpc_StructMatrix A, fpc_StructVector b, fpc_StructVector x) {
+  PIMPL(s);
+  FPC_CHECK_ARG(A && b && x && A->grid.N == b->grid.N && A->grid.N == x->grid.N, "bad arguments");
+  int rc = pfmg_setup(P, A);
+  if (rc) return rc;
+  long long n = A->grid.N;
+  if (!P->w64) FPC_HIP(hipMalloc(&P->w64, (size_t)std::max(1LL, n) * sizeof(double)));
+  if (!P->e64) FPC_HIP(hipMalloc(&P->e64, (size_t)std::max(1LL, n) * sizeof(double)));
+  double *part = scratch_doubles(kRedBlocks + 8, 5);
+  if (!part) return FPC_ERROR_MEMORY;
+  double bnorm2 = 0.0;
+  if (P->tol > 0.0) {
+    if ((rc = dot_partials(b->data, b->data, n, part)) || (rc = reduce_partials(part, kRedBlocks, part + kRedBlocks)))
+      return rc;
+    FPC_HIP(hipMemcpy(&bnorm2, part + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost));
+  }
+  P->its = 0;
+  for (int it = 0; it < P->maxit; ++it) {
+    if (it == 0 && P->zero_guess) {
+      rc = (P->built_bits == 32) ? vcycle<float>(P, b->data, x->data, nullptr) : vcycle<double>(P, b->data, x->data, nullptr);
+      if (rc) return rc;
+    } else {
+      if ((rc = residual_norm_partials(A, b->data, x->data, P->w64, part))) return rc;
+      rc = (P->built_bits == 32) ? vcycle<float>(P, P->w64, P->e64, nullptr) : vcycle<double>(P, P->w64, P->e64, nullptr);
+      if (rc) return rc;
+      if ((rc = axpy_inplace(x->data, P->e64, n))) return rc;
+    }
+    P->its = it + 1;
+    if (P->tol > 0.0) {
+      if ((rc = residual_norm_partials(A, b->data, x->data, nullptr, part)) ||
+          (rc = reduce_partials(part, kRedBlocks, part + kRedBlocks)))
+        return rc;
+      double r2 = 0.0;
+      FPC_HIP(hipMemcpy(&r2, part + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost));
+      if (std::sqrt(r2) <= P->tol * std::sqrt(bnorm2)) break;
+    }
+  }
+  FPC_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+int fpc_StructPFMGGetNumIterations(fpc_StructSolver s, int *n) {
+  PIMPL(s);
+  FPC_CHECK_ARG(n, "null pointer");
+  *n = P->its;
+  return 0;
+}
+
+int fpc_StructPFMGGetNumLevels(fpc_StructSolver s, int *n) {
+  PIMPL(s);
+  FPC_CHECK_ARG(n, "null pointer");
+  *n = (int)P->L.size();
+  return 0;
+}
+
+int fpc_StructPFMGGetLevelInfo(fpc_StructSolver s, int level, int *dims, int *cdir, double *weight) {
+  PIMPL(s);
+  FPC_CHECK_ARG(level >= 0 && level < (int)P->L.size(), "level out of range");
+  const DLevel &l = P->L[(size_t)level];
+  if (dims) { dims[0] = l.d.nx; dims[1] = l.d.ny; dims[2] = l.d.nz; }
+  if (cdir) *cdir = l.cdir;
+  if (weight) *weight = l.omega;
+  return 0;
+}
+
+int fpc_StructPFMGDestroy(fpc_StructSolver s) {
+  if (!s) return 0;
+  PfmgImpl *P = pimpl(s);
+  if (P) {
+    free_levels(P);
+    delete P;
+  }
+  delete s;
+  return 0;
+}
+
+}  // extern "C"

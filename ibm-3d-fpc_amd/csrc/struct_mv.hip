@@ -1,0 +1,713 @@
+// struct_mv.hip -- Struct grid/stencil/matrix/vector objects, the 7-point
+// matvec and the deterministic reductions (libfpc).
+//
+// Replaces the reference's HYPRE Struct object calls behind the helper
+// signatures of hypre_struct_utils.hpp:1-43 (see include/fpc_struct.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "fpc_internal.hpp"
+
+namespace fpc {
+
+namespace {
+int g_error = 0;
+char g_msg[512] = "";
+hipStream_t g_stream = nullptr;
+bool g_own_stream = false;
+bool g_init = false;
+std::vector<double *> g_scratch(8, nullptr);
+std::vector<size_t> g_scratch_n(8, 0);
+}  // namespace
+
+void set_error(int bits, const char *fmt, ...) {
+  g_error |= bits;
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_msg, sizeof(g_msg), fmt, ap);
+  va_end(ap);
+}
+
+int error_bits() { return g_error; }
+
+void ensure_init() {
+  if (g_init) return;
+  if (!g_stream) {
+    if (hipStreamCreate(&g_stream) != hipSuccess) {
+      set_error(FPC_ERROR_GENERIC, "hipStreamCreate failed");
+      g_stream = nullptr;
+    } else {
+      g_own_stream = true;
+    }
+  }
+  g_init = true;
+}
+
+hipStream_t stream() {
+  ensure_init();
+  return g_stream;
+}
+
+double *scratch_doubles(size_t n, int which) {
+  if (which < 0 || which >= (int)g_scratch.size()) return nullptr;
+  if (g_scratch_n[(size_t)which] < n) {
+    if (g_scratch[(size_t)which]) (void)hipFree(g_scratch[(size_t)which]);
+    g_scratch[(size_t)which] = nullptr;
+    g_scratch_n[(size_t)which] = 0;
+    void *p = nullptr;
+    if (hipMalloc(&p, n * sizeof(double)) != hipSuccess) {
+      set_error(FPC_ERROR_MEMORY, "scratch allocation of %zu doubles failed", n);
+      return nullptr;
+    }
+    g_scratch[(size_t)which] = (double *)p;
+    g_scratch_n[(size_t)which] = n;
+  }
+  return g_scratch[(size_t)which];
+}
+
+// ---------------------------------------------------------------- kernels
+namespace {
+
+// Scatter AoS values of a sub-box into the SoA planes.
+__global__ void k_set_matrix_box(double *coef, long long pitch, int gnx, int gny, int ox, int oy, int oz, int ex,
+                                 int ey, int ez, int nent, const int *slots, const double *vals, int getmode,
+                                 double *outvals) {
+  long long npts = (long long)ex * ey * ez;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < npts; t += (long long)gridDim.x * blockDim.x) {
+    int i = (int)(t % ex);
+    long long q = t / ex;
+    int j = (int)(q % ey);
+    int k = (int)(q / ey);
+    long long r = ((long long)(k + oz) * gny + (j + oy)) * gnx + (i + ox);
+    for (int e = 0; e < nent; ++e) {
+      int s = slots[e];
+      if (getmode) outvals[t * nent + e] = coef[s * pitch + r];
+      else coef[s * pitch + r] = vals[t * nent + e];
+    }
+  }
+}
+
+__global__ void k_vector_box(double *data, int gnx, int gny, int ox, int oy, int oz, int ex, int ey, int ez,
+                             const double *in, double *out) {
+  long long npts = (long long)ex * ey * ez;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < npts; t += (long long)gridDim.x * blockDim.x) {
+    int i = (int)(t % ex);
+    long long q = t / ex;
+    int j = (int)(q % ey);
+    int k = (int)(q / ey);
+    long long r = ((long long)(k + oz) * gny + (j + oy)) * gnx + (i + ox);
+    if (in) data[r] = in[t];
+    else out[t] = data[r];
+  }
+}
+
+__global__ void k_fill(double *x, long long n, double v) {
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) x[t] = v;
+}
+
+__global__ void k_axpy1(double *x, const double *y, long long n) {
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) x[t] += y[t];
+}
+
+__global__ void k_copy(double *d, const double *s, long long n) {
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) d[t] = s[t];
+}
+
+template <typename T>
+__global__ void k_dinv(const T *coef, T *dinv, long long n) {
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+    T d = coef[t];
+    dinv[t] = (d != (T)0) ? (T)1 / d : (T)0;
+  }
+}
+
+__global__ void k_to_fp32(const double *s, float *d, long long n) {
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) d[t] = (float)s[t];
+}
+
+// y = alpha * A x + beta * y, one thread per box point.
+__global__ void k_matvec(const double *__restrict__ c, long long pitch, int nx, int ny, int nz, double alpha,
+                         const double *__restrict__ x, double beta, double *__restrict__ y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int k = blockIdx.z;
+  if (i >= nx || j >= ny) return;
+  long long r = ((long long)k * ny + j) * nx + i;
+  long long pl = (long long)nx * ny;
+  double s = c[r] * x[r];
+  if (i > 0) s += c[pitch + r] * x[r - 1];
+  if (i < nx - 1) s += c[2 * pitch + r] * x[r + 1];
+  if (j > 0) s += c[3 * pitch + r] * x[r - nx];
+  if (j < ny - 1) s += c[4 * pitch + r] * x[r + nx];
+  if (k > 0) s += c[5 * pitch + r] * x[r - pl];
+  if (k < nz - 1) s += c[6 * pitch + r] * x[r + pl];
+  double out = alpha * s;
+  if (beta != 0.0) out += beta * y[r];
+  y[r] = out;
+}
+
+// r = b - A x with per-block partial sums of r^2.
+__global__ void __launch_bounds__(256) k_residual(const double *__restrict__ c, long long pitch, int nx, int ny, int nz,
+                                                  const double *__restrict__ b, const double *__restrict__ x,
+                                                  double *__restrict__ r, double *__restrict__ part) {
+  __shared__ double lds[8];
+  long long N = (long long)nx * ny * nz, pl = (long long)nx * ny;
+  double acc = 0.0;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < N; t += (long long)gridDim.x * blockDim.x) {
+    int i = (int)(t % nx);
+    long long q = t / nx;
+    int j = (int)(q % ny);
+    int k = (int)(q / ny);
+    double s = c[t] * x[t];
+    if (i > 0) s += c[pitch + t] * x[t - 1];
+    if (i < nx - 1) s += c[2 * pitch + t] * x[t + 1];
+    if (j > 0) s += c[3 * pitch + t] * x[t - nx];
+    if (j < ny - 1) s += c[4 * pitch + t] * x[t + nx];
+    if (k > 0) s += c[5 * pitch + t] * x[t - pl];
+    if (k < nz - 1) s += c[6 * pitch + t] * x[t + pl];
+    double rv = b[t] - s;
+    if (r) r[t] = rv;
+    acc += rv * rv;
+  }
+  double tot = block_sum<256>(acc, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(256) k_dot(const double *__restrict__ a, const double *__restrict__ b, long long n,
+                                             double *__restrict__ part) {
+  __shared__ double lds[8];
+  double acc = 0.0;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+    acc += a[t] * b[t];
+  double tot = block_sum<256>(acc, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(1024) k_reduce(const double *__restrict__ part, int n, double *__restrict__ out) {
+  __shared__ double lds[16];
+  double acc = 0.0;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) acc += part[t];
+  double tot = block_sum<1024>(acc, lds);
+  if (threadIdx.x == 0) *out = tot;
+}
+
+unsigned blocks_for(long long n) {
+  long long b = (n + 255) / 256;
+  if (b < 1) b = 1;
+  if (b > 65536) b = 65536;
+  return (unsigned)b;
+}
+
+}  // namespace
+
+int vec_copy(double *dst, const double *src, long long n) {
+  if (n <= 0) return 0;
+  k_copy<<<blocks_for(n), 256, 0, stream()>>>(dst, src, n);
+  FPC_LAUNCH_CHECK();
+  return 0;
+}
+
+int axpy_inplace(double *x, const double *y, long long n) {
+  if (n <= 0) return 0;
+  k_axpy1<<<blocks_for(n), 256, 0, stream()>>>(x, y, n);
+  FPC_LAUNCH_CHECK();
+  return 0;
+}
+
+int vec_zero(double *dst, long long n) {
+  if (n <= 0) return 0;
+  k_fill<<<blocks_for(n), 256, 0, stream()>>>(dst, n, 0.0);
+  FPC_LAUNCH_CHECK();
+  return 0;
+}
+
+int matvec(double alpha, const fpc_StructMatrix_s *A, const double *x, double beta, double *y) {
+  Dims d = dims_of(A->grid);
+  if (d.N == 0) return 0;
+  dim3 g, b;
+  launch_shape(d, g, b);
+  k_matvec<<<g, b, 0, stream()>>>(A->coef, A->pitch, d.nx, d.ny, d.nz, alpha, x, beta, y);
+  FPC_LAUNCH_CHECK();
+  return 0;
+}
+
+int residual_norm_partials(const fpc_StructMatrix_s *A, const double *b, const double *x, double *r, double *part) {
+  Dims d = dims_of(A->grid);
+  k_residual<<<kRedBlocks, 256, 0, stream()>>>(A->coef, A->pitch, d.nx, d.ny, d.nz, b, x, r, part);
+  FPC_LAUNCH_CHECK();
+  return 0;
+}
+
+int dot_partials(const double *a, const double *b, long long n, double *part) {
+  k_dot<<<kRedBlocks, 256, 0, stream()>>>(a, b, n, part);
+  FPC_LAUNCH_CHECK();
+  return 0;
+}
+
+int reduce_partials(const double *part, int nparts, double *out) {
+  k_reduce<<<1, 1024, 0, stream()>>>(part, nparts, out);
+  FPC_LAUNCH_CHECK();
+  return 0;
+}
+
+int ensure_dinv(fpc_StructMatrix_s *A) {
+  if (A->dinv) return 0;
+  FPC_HIP(hipMalloc(&A->dinv, (size_t)A->pitch * sizeof(double)));
+  k_dinv<double><<<blocks_for(A->grid.N), 256, 0, stream()>>>(A->coef, A->dinv, A->grid.N);
+  FPC_LAUNCH_CHECK();
+  return 0;
+}
+
+int ensure_fp32(fpc_StructMatrix_s *A) {
+  if (A->coef32) return 0;
+  FPC_HIP(hipMalloc(&A->coef32, (size_t)kSlots * A->pitch * sizeof(float)));
+  FPC_HIP(hipMalloc(&A->dinv32, (size_t)A->pitch * sizeof(float)));
+  long long n = (long long)kSlots * A->pitch;
+  k_to_fp32<<<blocks_for(n), 256, 0, stream()>>>(A->coef, A->coef32, n);
+  FPC_LAUNCH_CHECK();
+  k_dinv<float><<<blocks_for(A->grid.N), 256, 0, stream()>>>(A->coef32, A->dinv32, A->grid.N);
+  FPC_LAUNCH_CHECK();
+  return 0;
+}
+
+static void drop_derived(fpc_StructMatrix_s *A) {
+  if (A->dinv) (void)hipFree(A->dinv);
+  if (A->coef32) (void)hipFree(A->coef32);
+  if (A->dinv32) (void)hipFree(A->dinv32);
+  A->dinv = nullptr;
+  A->coef32 = nullptr;
+  A->dinv32 = nullptr;
+}
+
+}  // namespace fpc
+
+using namespace fpc;
+
+// Maps an offset to its canonical slot (C,-x,+x,-y,+y,-z,+z) or -1.
+static int slot_of_offset(const int *o) {
+  static const int off[kSlots][3] = {{0, 0, 0}, {-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+  for (int s = 0; s < kSlots; ++s)
+    if (o[0] == off[s][0] && o[1] == off[s][1] && o[2] == off[s][2]) return s;
+  return -1;
+}
+
+// Sub-box [il, iu] of a grid -> offsets and extents; false when outside.
+static bool sub_box(const fpc_StructGrid_s &g, const int *il, const int *iu, int *o, int *e) {
+  for (int d = 0; d < 3; ++d) {
+    o[d] = il[d] - g.il[d];
+    e[d] = iu[d] - il[d] + 1;
+    if (o[d] < 0 || e[d] < 0 || o[d] + e[d] > g.n[d]) return false;
+  }
+  return true;
+}
+
+extern "C" {
+
+int fpc_Initialize(void) {
+  ensure_init();
+  return error_bits() & FPC_ERROR_GENERIC;
+}
+
+int fpc_Finalize(void) {
+  for (size_t w = 0; w < g_scratch.size(); ++w) {
+    if (g_scratch[w]) (void)hipFree(g_scratch[w]);
+    g_scratch[w] = nullptr;
+    g_scratch_n[w] = 0;
+  }
+  if (g_stream && g_own_stream) (void)hipStreamDestroy(g_stream);
+  g_stream = nullptr;
+  g_own_stream = false;
+  g_init = false;
+  return 0;
+}
+
+int fpc_SetMemoryLocationDevice(void) { return 0; }
+int fpc_SetExecutionPolicyDevice(void) { return 0; }
+int fpc_GetError(void) { return g_error; }
+int fpc_ClearError(int bits) {
+  g_error &= ~bits;
+  return 0;
+}
+const char *fpc_GetErrorMessage(void) { return g_msg; }
+
+int fpc_SetStream(void *s) {
+  if (g_stream && g_own_stream) (void)hipStreamDestroy(g_stream);
+  g_stream = (hipStream_t)s;
+  g_own_stream = false;
+  if (!s) {
+    g_init = false;
+    ensure_init();
+  }
+  g_init = true;
+  return 0;
+}
+void *fpc_GetStream(void) { return (void *)stream(); }
+
+// ---------------------------------------------------------------- grid
+int fpc_StructGridCreate(int ndim, fpc_StructGrid *grid) {
+  FPC_CHECK_ARG(grid, "null handle pointer");
+  FPC_CHECK_ARG(ndim >= 1 && ndim <= 3, "ndim must be 1..3");
+  ensure_init();
+  fpc_StructGrid g = new fpc_StructGrid_s();
+  g->ndim = ndim;
+  *grid = g;
+  return 0;
+}
+
+int fpc_StructGridSetExtents(fpc_StructGrid g, const int *il, const int *iu) {
+  FPC_CHECK_ARG(g && il && iu, "null argument");
+  FPC_CHECK_ARG(!g->extents, "only one box per grid is supported");
+  for (int d = 0; d < 3; ++d) {
+    g->il[d] = d < g->ndim ? il[d] : 0;
+    g->iu[d] = d < g->ndim ? iu[d] : 0;
+    FPC_CHECK_ARG(g->iu[d] >= g->il[d], "empty box");
+    g->n[d] = g->iu[d] - g->il[d] + 1;
+  }
+  g->N = (long long)g->n[0] * g->n[1] * g->n[2];
+  g->extents = true;
+  return 0;
+}
+
+int fpc_StructGridAssemble(fpc_StructGrid g) {
+  FPC_CHECK_ARG(g && g->extents, "grid has no extents");
+  g->assembled = true;
+  return 0;
+}
+
+int fpc_StructGridDestroy(fpc_StructGrid g) {
+  delete g;
+  return 0;
+}
+
+// ---------------------------------------------------------------- stencil
+int fpc_StructStencilCreate(int ndim, int size, fpc_StructStencil *st) {
+  FPC_CHECK_ARG(st, "null handle pointer");
+  FPC_CHECK_ARG(size >= 1 && size <= kSlots, "stencil size must be 1..7 (7-point stencils)");
+  fpc_StructStencil s = new fpc_StructStencil_s();
+  s->ndim = ndim;
+  s->size = size;
+  *st = s;
+  return 0;
+}
+
+int fpc_StructStencilSetElement(fpc_StructStencil st, int entry, const int *offset) {
+  FPC_CHECK_ARG(st && offset, "null argument");
+  FPC_CHECK_ARG(entry >= 0 && entry < st->size, "entry out of range");
+  int o[3] = {0, 0, 0};
+  for (int d = 0; d < st->ndim && d < 3; ++d) o[d] = offset[d];
+  FPC_CHECK_ARG(slot_of_offset(o) >= 0, "only the 7-point offsets are supported");
+  for (int d = 0; d < 3; ++d) st->off[entry][d] = o[d];
+  st->set_mask |= 1 << entry;
+  return 0;
+}
+
+int fpc_StructStencilDestroy(fpc_StructStencil st) {
+  delete st;
+  return 0;
+}
+
+// ---------------------------------------------------------------- matrix
+int fpc_StructMatrixCreate(fpc_StructGrid grid, fpc_StructStencil st, fpc_StructMatrix *A) {
+  FPC_CHECK_ARG(grid && st && A, "null argument");
+  FPC_CHECK_ARG(grid->assembled, "grid not assembled");
+  FPC_CHECK_ARG(st->set_mask == (1 << st->size) - 1, "stencil elements not all set");
+  fpc_StructMatrix M = new fpc_StructMatrix_s();
+  M->grid = *grid;
+  M->nentries = st->size;
+  for (int e = 0; e < st->size; ++e) M->slot_of_entry[e] = slot_of_offset(st->off[e]);
+  *A = M;
+  return 0;
+}
+
+int fpc_StructMatrixInitialize(fpc_StructMatrix A) {
+  FPC_CHECK_ARG(A, "null matrix");
+  if (!A->coef) {
+    A->pitch = pad256(A->grid.N, (int)sizeof(double));
+    FPC_HIP(hipMalloc(&A->coef, (size_t)kSlots * A->pitch * sizeof(double)));
+    FPC_HIP(hipMemsetAsync(A->coef, 0, (size_t)kSlots * A->pitch * sizeof(double), stream()));
+  }
+  A->initialized = true;
+  return 0;
+}
+
+static int matrix_box_io(fpc_StructMatrix A, const int *il, const int *iu, int nent, const int *entries,
+                         const double *vin, double *vout) {
+  FPC_CHECK_ARG(A && il && iu && entries, "null argument");
+  FPC_CHECK_ARG(A->initialized, "matrix not initialized");
+  FPC_CHECK_ARG(nent >= 1 && nent <= A->nentries, "bad number of entries");
+  int o[3], e[3];
+  FPC_CHECK_ARG(sub_box(A->grid, il, iu, o, e), "box outside the grid");
+  int slots[kSlots];
+  for (int t = 0; t < nent; ++t) {
+    FPC_CHECK_ARG(entries[t] >= 0 && entries[t] < A->nentries, "entry index out of range");
+    slots[t] = A->slot_of_entry[entries[t]];
+  }
+  long long npts = (long long)e[0] * e[1] * e[2];
+  if (npts == 0) return 0;
+  size_t nval = (size_t)npts * nent;
+  double *stage = scratch_doubles(nval, 0);
+  double *dslots = scratch_doubles(kSlots, 1);
+  if (!stage || !dslots) return FPC_ERROR_MEMORY;
+  int *islots = (int *)dslots;
+  FPC_HIP(hipMemcpyAsync(islots, slots, sizeof(int) * nent, hipMemcpyHostToDevice, stream()));
+  if (vin) FPC_HIP(hipMemcpyAsync(stage, vin, nval * sizeof(double), hipMemcpyDefault, stream()));
+  k_set_matrix_box<<<blocks_for(npts), 256, 0, stream()>>>(A->coef, A->pitch, A->grid.n[0], A->grid.n[1], o[0], o[1],
+                                                             o[2], e[0], e[1], e[2], nent, islots, stage, vin ? 0 : 1,
+                                                             stage);
+  FPC_LAUNCH_CHECK();
+  if (vout) FPC_HIP(hipMemcpyAsync(vout, stage, nval * sizeof(double), hipMemcpyDefault, stream()));
+  FPC_HIP(hipStreamSynchronize(stream()));
+  if (vin) {
+    drop_derived(A);
+    A->version++;
+  }
+  return 0;
+}
+
+int fpc_StructMatrixSetBoxValues(fpc_StructMatrix A, const int *il, const int *iu, int nent, const int *entries,
+                                 const double *values) {
+  FPC_CHECK_ARG(values, "null values");
+  return matrix_box_io(A, il, iu, nent, entries, values, nullptr);
+}
+
+int fpc_StructMatrixGetBoxValues(fpc_StructMatrix A, const int *il, const int *iu, int nent, const int *entries,
+                                 double *values) {
+  FPC_CHECK_ARG(values, "null values");
+  return matrix_box_io(A, il, iu, nent, entries, nullptr, values);
+}
+
+int fpc_StructMatrixAssemble(fpc_StructMatrix A) {
+  FPC_CHECK_ARG(A && A->initialized, "matrix not initialized");
+  int rc = ensure_dinv(A);
+  if (rc) return rc;
+  FPC_HIP(hipStreamSynchronize(stream()));
+  A->assembled = true;
+  return 0;
+}
+
+int fpc_StructMatrixMatvec(double alpha, fpc_StructMatrix A, fpc_StructVector x, double beta, fpc_StructVector y) {
+  FPC_CHECK_ARG(A && x && y, "null argument");
+  FPC_CHECK_ARG(A->initialized && x->data && y->data, "objects not initialized");
+  FPC_CHECK_ARG(x->grid.N == A->grid.N && y->grid.N == A->grid.N, "size mismatch");
+  FPC_CHECK_ARG(x->data != y->data, "x and y must differ");
+  int rc = matvec(alpha, A, x->data, beta, y->data);
+  if (rc) return rc;
+  FPC_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+int fpc_StructMatrixDestroy(fpc_StructMatrix A) {
+  if (!A) return 0;
+  drop_derived(A);
+  if (A->coef) (void)hipFree(A->coef);
+  delete A;
+  return 0;
+}
+
+// ---------------------------------------------------------------- vector
+int fpc_StructVectorCreate(fpc_StructGrid grid, fpc_StructVector *x) {
+  FPC_CHECK_ARG(grid && x, "null argument");
+  FPC_CHECK_ARG(grid->assembled, "grid not assembled");
+  fpc_StructVector v = new fpc_StructVector_s();
+  v->grid = *grid;
+  *x = v;
+  return 0;
+}
+
+int fpc_StructVectorInitialize(fpc_StructVector x) {
+  FPC_CHECK_ARG(x, "null vector");
+  if (!x->data) {
+    size_t n = (size_t)pad256(std::max(1LL, x->grid.N), (int)sizeof(double));
+    FPC_HIP(hipMalloc(&x->data, n * sizeof(double)));
+    FPC_HIP(hipMemsetAsync(x->data, 0, n * sizeof(double), stream()));
+    FPC_HIP(hipStreamSynchronize(stream()));
+  }
+  x->initialized = true;
+  return 0;
+}
+
+static int vector_box_io(fpc_StructVector x, const int *il, const int *iu, const double *vin, double *vout) {
+  FPC_CHECK_ARG(x && il && iu, "null argument");
+  FPC_CHECK_ARG(x->initialized, "vector not initialized");
+  int o[3], e[3];
+  FPC_CHECK_ARG(sub_box(x->grid, il, iu, o, e), "box outside the grid");
+  long long npts = (long long)e[0] * e[1] * e[2];
+  if (npts == 0) return 0;
+  bool whole = (npts == x->grid.N);
+  if (whole) {
+    if (vin) FPC_HIP(hipMemcpyAsync(x->data, vin, (size_t)npts * sizeof(double), hipMemcpyDefault, stream()));
+    else FPC_HIP(hipMemcpyAsync(vout, x->data, (size_t)npts * sizeof(double), hipMemcpyDefault, stream()));
+  } else {
+    double *stage = scratch_doubles((size_t)npts, 2);
+    if (!stage) return FPC_ERROR_MEMORY;
+    if (vin) FPC_HIP(hipMemcpyAsync(stage, vin, (size_t)npts * sizeof(double), hipMemcpyDefault, stream()));
+    k_vector_box<<<blocks_for(npts), 256, 0, stream()>>>(x->data, x->grid.n[0], x->grid.n[1], o[0], o[1], o[2], e[0],
+                                                        e[1], e[2], vin ? stage : nullptr, vin ? nullptr : stage);
+    FPC_LAUNCH_CHECK();
+    if (vout) FPC_HIP(hipMemcpyAsync(vout, stage, (size_t)npts * sizeof(double), hipMemcpyDefault, stream()));
+  }
+  FPC_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+int fpc_StructVectorSetBoxValues(fpc_StructVector x, const int *il, const int *iu, const double *values) {
+  FPC_CHECK_ARG(values, "null values");
+  return vector_box_io(x, il, iu, values, nullptr);
+}
+
+int fpc_StructVectorGetBoxValues(fpc_StructVector x, const int *il, const int *iu, double *values) {
+  FPC_CHECK_ARG(values, "null values");
+  return vector_box_io(x, il, iu, nullptr, values);
+}
+
+int fpc_StructVectorSetConstantValues(fpc_StructVector x, double value) {
+  FPC_CHECK_ARG(x && x->initialized, "vector not initialized");
+  if (x->grid.N > 0) {
+    k_fill<<<blocks_for(x->grid.N), 256, 0, stream()>>>(x->data, x->grid.N, value);
+    FPC_LAUNCH_CHECK();
+  }
+  FPC_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+int fpc_StructVectorAssemble(fpc_StructVector x) {
+  FPC_CHECK_ARG(x && x->initialized, "vector not initialized");
+  return 0;
+}
+
+int fpc_StructVectorGetDataPointer(fpc_StructVector x, double **p) {
+  FPC_CHECK_ARG(x && p && x->initialized, "vector not initialized");
+  *p = x->data;
+  return 0;
+}
+
+int fpc_StructVectorDestroy(fpc_StructVector x) {
+  if (!x) return 0;
+  if (x->data) (void)hipFree(x->data);
+  delete x;
+  return 0;
+}
+
+// ---------------------------------------------------------------- helpers (hypre_struct_utils.hpp:1-26)
+fpc_StructGrid fpc_CreateGrid3D(int nx, int ny, int nz) {
+  fpc_StructGrid g = nullptr;
+  int il[3] = {1, 1, 1}, iu[3] = {nx, ny, nz};
+  if (fpc_StructGridCreate(3, &g)) return nullptr;
+  if (fpc_StructGridSetExtents(g, il, iu) || fpc_StructGridAssemble(g)) {
+    fpc_StructGridDestroy(g);
+    return nullptr;
+  }
+  return g;
+}
+
+fpc_StructStencil fpc_CreateStencil3D(void) {
+  fpc_StructStencil st = nullptr;
+  int offs[7][3] = {{0, 0, 0}, {-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+  if (fpc_StructStencilCreate(3, 7, &st)) return nullptr;
+  for (int s = 0; s < 7; ++s)
+    if (fpc_StructStencilSetElement(st, s, offs[s])) {
+      fpc_StructStencilDestroy(st);
+      return nullptr;
+    }
+  return st;
+}
+
+int fpc_SetMatrix3D(fpc_StructMatrix A, int nx, int ny, int nz, const double *vals) {
+  int il[3] = {1, 1, 1}, iu[3] = {nx, ny, nz}, entries[7] = {0, 1, 2, 3, 4, 5, 6};
+  int rc = fpc_StructMatrixInitialize(A);
+  if (!rc) rc = fpc_StructMatrixSetBoxValues(A, il, iu, 7, entries, vals);
+  if (!rc) rc = fpc_StructMatrixAssemble(A);
+  return rc;
+}
+
+fpc_StructMatrix fpc_CreateMatrix3D(fpc_StructGrid grid, fpc_StructStencil st, int nx, int ny, int nz,
+                                    const double *vals) {
+  fpc_StructMatrix A = nullptr;
+  if (fpc_StructMatrixCreate(grid, st, &A)) return nullptr;
+  if (fpc_SetMatrix3D(A, nx, ny, nz, vals)) {
+    fpc_StructMatrixDestroy(A);
+    return nullptr;
+  }
+  return A;
+}
+
+int fpc_SetVector3D(fpc_StructVector x, int nx, int ny, int nz, const double *vals) {
+  int il[3] = {1, 1, 1}, iu[3] = {nx, ny, nz};
+  int rc = fpc_StructVectorSetBoxValues(x, il, iu, vals);
+  if (!rc) rc = fpc_StructVectorAssemble(x);
+  return rc;
+}
+
+int fpc_GetVector3D(fpc_StructVector x, int nx, int ny, int nz, double *vals) {
+  int il[3] = {1, 1, 1}, iu[3] = {nx, ny, nz};
+  return fpc_StructVectorGetBoxValues(x, il, iu, vals);
+}
+
+fpc_StructVector fpc_CreateVector3D(fpc_StructGrid grid) {
+  fpc_StructVector x = nullptr;
+  if (fpc_StructVectorCreate(grid, &x)) return nullptr;
+  if (fpc_StructVectorInitialize(x)) {
+    fpc_StructVectorDestroy(x);
+    return nullptr;
+  }
+  return x;
+}
+
+// ---------------------------------------------------------------- memory helpers (hypre_struct_utils.hpp:27-43)
+static int dmalloc(void **p, size_t bytes) {
+  FPC_CHECK_ARG(p, "null pointer");
+  ensure_init();
+  *p = nullptr;
+  FPC_HIP(hipMalloc(p, bytes ? bytes : 1));
+  return 0;
+}
+
+int fpc_MallocManagedC(double **p, size_t n) { return dmalloc((void **)p, n * sizeof(double)); }
+int fpc_MallocManagedD(double **p, size_t n) { return dmalloc((void **)p, n * sizeof(double)); }
+int fpc_MallocManagedI(int **p, size_t n) { return dmalloc((void **)p, n * sizeof(int)); }
+int fpc_MallocManagedB(unsigned char **p, size_t n) { return dmalloc((void **)p, n); }
+
+static int dcopyvec(void **p, const void *src, size_t bytes) {
+  int rc = dmalloc(p, bytes);
+  if (rc) return rc;
+  if (bytes) {
+    FPC_HIP(hipMemcpyAsync(*p, src, bytes, hipMemcpyDefault, stream()));
+    FPC_HIP(hipStreamSynchronize(stream()));
+  }
+  return 0;
+}
+
+int fpc_CopyVecToManagedC(double **p, const double *s, size_t n) { return dcopyvec((void **)p, s, n * sizeof(double)); }
+int fpc_CopyVecToManagedD(double **p, const double *s, size_t n) { return dcopyvec((void **)p, s, n * sizeof(double)); }
+int fpc_CopyVecToManagedI(int **p, const int *s, size_t n) { return dcopyvec((void **)p, s, n * sizeof(int)); }
+int fpc_CopyVecToManagedB(unsigned char **p, const unsigned char *s, size_t n) { return dcopyvec((void **)p, s, n); }
+
+int fpc_Free(void *p) {
+  if (p) FPC_HIP(hipFree(p));
+  return 0;
+}
+
+int fpc_MemcpyToHost(void *dst, const void *src, size_t bytes) {
+  if (!bytes) return 0;
+  FPC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, stream()));
+  FPC_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+int fpc_MemcpyToDevice(void *dst, const void *src, size_t bytes) {
+  if (!bytes) return 0;
+  FPC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, stream()));
+  FPC_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+int fpc_DeviceSynchronize(void) {
+  FPC_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,87 @@
+/*
+ * fpc_sim.h -- C ABI of the one-time host setup and of the time-step driver (libfpc.so).
+ *
+ * The setup restates the reference's CPU geometry / assembly / embedding
+ * (immersed_geometry.hpp:1-160, immersed_operators.hpp:1-287, 397-447, 737-748,
+ * solver_helpers.hpp:51-84, 227-329, main.cu:20-121) directly on the structured
+ * grid, without sparse maps, and reproduces the embedded stencils, masks and
+ * index sets bit for bit -- including the low bits that the reference's
+ * std::unordered_map iteration order gives Ap's diagonal and the l1 row scales.
+ *
+ * The driver mirrors main.cu:8-424: the same Struct objects and solver settings
+ * (through fpc_struct.h) and the same per-step order (main.cu:249-376), with
+ * the dense ops run as the fused stages of fpc_dense.h.
+ */
+#ifndef FPC_SIM_H
+#define FPC_SIM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Params (common.hpp:45-85) plus extensions the production configurations need. */
+typedef struct {
+  int nx, ny, nz;
+  double Lx, Ly, Lz, rho, nu, Um, dt;
+  int nsteps, print_every, inlet_ramp_steps, use_zero_initial_guess, nsub_beta;
+  double beta_tol, alpha_tol, xcirc, ycirc, Rcirc;
+  int vel_maxit, vel_kdim;
+  double vel_tol;
+  int vel_jacobi_iters;
+  int p_maxit, p_kdim;
+  double p_tol;
+  int p_pc_maxit, p_pc_relax_type, p_pc_rap_type;
+  /* extensions (no reference counterpart) */
+  int init_profile;          /* 1: u = inlet profile on active u faces at t = 0 */
+  double noise_amp;          /* seeded uniform noise in (-a, a) on active faces */
+  unsigned long long seed;
+  int mg_precision;          /* 64: FP64 GMRES+PFMG; 32: FP32 PFMG inside FP64 flexible GMRES */
+  int vel_print, p_print;
+} fpc_params;
+
+void fpc_default_params(fpc_params *par);
+/* Reference flag names (common.hpp:87-127) plus -init-profile, -noise-amp, -seed,
+ * -mg-precision.  Unknown flags are ignored, as in the reference. */
+int fpc_parse_args(int argc, char **argv, fpc_params *par);
+
+/* ---- one-time setup ---- */
+typedef struct fpc_setup_s *fpc_setup;
+int fpc_setup_create(const fpc_params *par, fpc_setup *setup);
+/* Named arrays (same names as the reference variables): beta2d alpha_u2d alpha_v2d
+ * fluid2d fluid_p active_{u,v,w,qu,qv,qw} id_{u,v,w,p,qu,qv,qw} rhs_u_bc rhs_v_bc rhs_w_bc
+ * div_bc scale_{u,v,w} (1-based active vectors) Au_vals Av_vals Aw_vals Ap_vals (AoS boxes)
+ * D_rowptr D_colind D_vals G_rowptr G_colind G_vals uin; plus the structured forms
+ * scale_{u,v,w}_box rhs_u_bc_box dcoef div_bc_plane. */
+long long fpc_setup_len(fpc_setup s, const char *name);
+int fpc_setup_elsize(fpc_setup s, const char *name);
+int fpc_setup_copy(fpc_setup s, const char *name, void *dst, long long nbytes);
+/* Nu Nv Nw Np Nqu Nqv Nqw n_pruned */
+long long fpc_setup_int(fpc_setup s, const char *name);
+int fpc_setup_destroy(fpc_setup s);
+
+/* ---- time-step driver ---- */
+typedef struct fpc_sim_s *fpc_sim;
+int fpc_sim_create(const fpc_params *par, fpc_sim *sim);
+/* Runs nsteps full steps (main.cu:249-376). */
+int fpc_sim_step(fpc_sim sim, int nsteps);
+/* One phase of a step: 0 ramp, 1 explicit stage, 2 momentum solves, 3 scatter,
+ * 4 divergence, 5 pressure solve, 6 projection, 7 diagnostics. */
+int fpc_sim_phase(fpc_sim sim, int phase);
+/* Fields: u v w p u_tilde v_tilde w_tilde rhs_u x_u rhs_v x_v rhs_w x_w rhs_p x_p
+ * uin_step qout, and every setup array. */
+long long fpc_sim_len(fpc_sim sim, const char *name);
+int fpc_sim_copy(fpc_sim sim, const char *name, void *dst, long long nbytes);
+int fpc_sim_set(fpc_sim sim, const char *name, const void *src, long long nbytes);
+/* Device pointer of a field (no copy). */
+int fpc_sim_device_ptr(fpc_sim sim, const char *name, void **ptr);
+/* step ramp div_l2 umax vmax wmax its_u its_v its_w its_p relres_u relres_v relres_w
+ * relres_p fx fy cd cl err_u err_v err_w err_p; NaN if unknown. */
+double fpc_sim_scalar(fpc_sim sim, const char *name);
+/* Per-stage wall time (s) accumulated over steps: t_build t_mom t_p t_proj t_diag */
+double fpc_sim_timer(fpc_sim sim, const char *name);
+int fpc_sim_destroy(fpc_sim sim);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FPC_SIM_H */
