@@ -8,6 +8,20 @@ profile + seeded noise of amplitude 1e-3 Um.  FP64 throughout ("value"); the
 FP32-multigrid form (FP32 V-cycle inside FP64 flexible GMRES) is timed on the
 same steps and reported beside it.
 
+roofline: the kernel class with the largest device time (found on the last
+warmup step with every class timed) is timed over the whole timed region with
+HIP events on the library's stream; achieved = its algorithmic bytes per
+launch / its mean launch duration, against the 8 TB/s HBM3E peak.  Launches
+enqueued ahead of a converged GMRES cycle return at once and are not counted.
+traffic = HBM bytes per launch read from profiles/hbm_traffic.txt (FETCH_SIZE +
+WRITE_SIZE per dispatch, calibrated on a copy kernel) when that file names the
+class, else null.
+
+cpu_baseline: the oracle (single-threaded C++ restatement of the same path)
+on a 16-deep z-slab of the same channel (512x96x16, Lz scaled so the grid
+spacing is unchanged): two steps after one untimed step, expressed as
+time-steps/s of the 512x96x96 grid by the ratio of cells.
+
 A step is the whole hot path of main.cu:249-376: BC cleanup, convection,
 grad p, momentum rhs, three GMRES+Jacobi solves, scatter, divergence, the
 GMRES+PFMG pressure solve, projection and the device-side diagnostics.  Inputs
@@ -27,9 +41,26 @@ WORKLOAD = dict(nx=512, ny=96, nz=96, Lx=2.5, Ly=0.41, Lz=0.41, xcirc=0.5, ycirc
                 p_maxit=250, init_profile=1, noise_amp=1e-3 * 2.25, seed=0)
 
 
+HBM_PEAK_GBS = 8000.0
+
+
+def _classes(rep):
+    return {k: {"launches": n, "ms": ms, "gbs": (b / (ms * 1e-3) / 1e9) if ms > 0 else None}
+            for k, (n, ms, b) in rep.items()}
+
+
 def run_sim(F, torch, kw, warmup, steps):
+    """W untimed steps (the last one with every kernel class timed), then K timed steps
+    with the dominant class timed by HIP events."""
     sim = F.Sim(F.default_params(**kw))
-    sim.step(warmup)
+    if warmup > 1:
+        sim.step(warmup - 1)
+    F.profile_enable("*")
+    sim.step(1 if warmup >= 1 else 0)
+    torch.cuda.synchronize()
+    classes = _classes(F.profile_report()) if warmup >= 1 else {}
+    dom = max(classes, key=lambda k: classes[k]["ms"]) if classes else "*"
+    F.profile_enable(dom)
     torch.cuda.synchronize()
     its = []
     t0 = time.perf_counter()
@@ -38,7 +69,52 @@ def run_sim(F, torch, kw, warmup, steps):
         its.append([int(sim.scalar(k)) for k in ("its_u", "its_v", "its_w", "its_p")])
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    return sim, dt, its
+    rep = F.profile_report()
+    F.profile_enable("")
+    if dom == "*":
+        classes = _classes(rep)
+        dom = max(classes, key=lambda k: classes[k]["ms"])
+    return sim, dt, its, dom, rep[dom], classes
+
+
+def read_traffic(cls):
+    """(fetch_bytes, write_bytes) per launch of a class from profiles/hbm_traffic.txt, or None."""
+    path = os.path.join(REPO, "profiles", "hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) >= 3 and not line.startswith("#") and f[0] == cls:
+            return float(f[1]), float(f[2])
+    return None
+
+
+def roofline(dom, rec):
+    n, ms, b = rec
+    per_launch = b / n
+    mean_s = ms * 1e-3 / n
+    achieved = per_launch / mean_s / 1e9
+    tr = read_traffic(dom)
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": (tr[0] + tr[1]) if tr else None,
+            "kernel": dom, "launches": n, "mean_us": mean_s * 1e6, "algorithmic_bytes_per_launch": per_launch}
+
+
+def cpu_baseline():
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O
+    nz = 16
+    kw = dict(WORKLOAD, nz=nz, Lz=WORKLOAD["Lz"] * nz / WORKLOAD["nz"])
+    sim = O.Sim(O.default_params(**kw))
+    sim.step(1)
+    t0 = time.perf_counter()
+    nsteps = 2
+    sim.step(nsteps)
+    dt = time.perf_counter() - t0
+    full = WORKLOAD["nx"] * WORKLOAD["ny"] * WORKLOAD["nz"]
+    return {"value": nsteps / dt * (nz / WORKLOAD["nz"]), "unit": "time-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle, 512x96x{nz} slab of the same channel (same spacing), steps 2-3 timed "
+                      f"({dt:.1f} s), scaled by cells {512 * 96 * nz}/{full}"}
 
 
 def main():
@@ -48,6 +124,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 2 GiB device-to-device copy 5 times (scale for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -55,12 +133,20 @@ def main():
     import numpy as np
     import torch
 
+    torch.cuda.set_device(0)
+    if args.copy_calibration:
+        a = torch.ones(1 << 28, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": a.numel() * 8, "copies": 5}))
+        return
     import fpc_amd as F
     F.lib()
-    torch.cuda.set_device(0)
-    sim64, t64, its64 = run_sim(F, torch, WORKLOAD, args.warmup, args.steps)
+    sim64, t64, its64, dom, rec, classes = run_sim(F, torch, WORKLOAD, args.warmup, args.steps)
     res = {
-        "metric": "time-steps/s @512x96x96 FP64 (GMRES+Jacobi u/v/w, GMRES+PFMG p, rtol 1e-8)",
+        "metric": "time-steps/s @512x96x96: FP64 vs FP32-MG at 1e-8 resid; stencil HBM GB/s",
         "value": args.steps / t64,
         "unit": "time-steps/s",
         "n_gpus": 1,
@@ -77,11 +163,13 @@ def main():
         "iterations_mean": [float(x) for x in np.mean(np.array(its64), axis=0)],
         "timers_s": {k: sim64.timer(k) for k in ("t_build", "t_mom", "t_p", "t_proj", "t_diag")},
         "cd": sim64.scalar("cd"), "cl": sim64.scalar("cl"),
-        "roofline": None,
+        "roofline": roofline(dom, rec),
         "cpu_baseline": None,
+        "kernel_classes": classes,
     }
     if not args.no_fp32:
-        sim32, t32, its32 = run_sim(F, torch, dict(WORKLOAD, mg_precision=32), args.warmup, args.steps)
+        sim32, t32, its32, dom32, rec32, cls32 = run_sim(F, torch, dict(WORKLOAD, mg_precision=32), args.warmup,
+                                                         args.steps)
         errs = {}
         for f in ("u", "v", "w", "p"):
             a, b = sim64.get(f), sim32.get(f)
@@ -89,8 +177,11 @@ def main():
         res["fp32mg"] = {"value": args.steps / t32, "ms_per_step": 1e3 * t32 / args.steps,
                          "speedup_vs_fp64": t64 / t32, "field_rel_l2_vs_fp64": errs,
                          "cd_rel_err": abs(sim32.scalar("cd") - sim64.scalar("cd")) / abs(sim64.scalar("cd")),
-                         "iterations_mean": [float(x) for x in np.mean(np.array(its32), axis=0)]}
+                         "iterations_mean": [float(x) for x in np.mean(np.array(its32), axis=0)],
+                         "roofline": roofline(dom32, rec32), "kernel_classes": cls32}
         sim32.close()
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline()
     print(json.dumps(res))
 
 

@@ -637,7 +637,13 @@ int stage_explicit_async(const fpc_flow_ops *o, double *u, double *v, double *w,
     d.nx = nx - (c == 0); d.ny = ny - (c == 1); d.nz = nz - (c == 2);
     dim3 g, b;
     launch_shape(d, g, b);
-    k_explicit<<<g, b, 0, stream()>>>(c, *o, u, v, w, p, outs[c][0], outs[c][1], zero);
+    {
+      // bytes per box row: the three velocity fields and p once (shared reads cached),
+      // masks (4 bytes), scale and bc (16), rhs and x0 (16)
+      FPC_PROF_BEGIN("explicit");
+      k_explicit<<<g, b, 0, stream()>>>(c, *o, u, v, w, p, outs[c][0], outs[c][1], zero);
+      FPC_PROF_END("explicit", (double)d.nx * d.ny * d.nz * 68.0);
+    }
     FPC_LAUNCH_CHECK();
   }
   return 0;
@@ -662,7 +668,12 @@ int stage_divergence_async(const fpc_flow_ops *o, const double *ut, const double
   d.nx = o->nx; d.ny = o->ny; d.nz = o->nz;
   dim3 g, b;
   launch_shape(d, g, b);
-  k_divergence<<<g, b, 0, stream()>>>(*o, ut, vt, wt, ramp, rhs, x0);
+  {
+    // bytes per cell: three face velocities, fluid mask, rhs and x0
+    FPC_PROF_BEGIN("divergence");
+    k_divergence<<<g, b, 0, stream()>>>(*o, ut, vt, wt, ramp, rhs, x0);
+    FPC_PROF_END("divergence", (double)d.nx * d.ny * d.nz * 41.0);
+  }
   FPC_LAUNCH_CHECK();
   return 0;
 }
@@ -672,7 +683,12 @@ int stage_projection_async(const fpc_flow_ops *o, const double *phi, const doubl
                            double *qout) {
   int nx = o->nx, ny = o->ny, nz = o->nz;
   long long N = std::max((long long)(nx + 1) * ny * nz, std::max((long long)nx * (ny + 1) * nz, (long long)nx * ny * (nz + 1)));
-  k_project<<<LAUNCH1(N)>>>(*o, phi, ut, vt, wt, u, v, w, qout);
+  {
+    // bytes per face (three components): u~ read, u written, phi read once per cell, masks
+    FPC_PROF_BEGIN("projection");
+    k_project<<<LAUNCH1(N)>>>(*o, phi, ut, vt, wt, u, v, w, qout);
+    FPC_PROF_END("projection", (double)N * 3.0 * 18.0 + (double)nx * ny * nz * 9.0);
+  }
   FPC_LAUNCH_CHECK();
   k_bc<<<LAUNCH1(N)>>>(u, v, w, o->active_u, o->active_v, o->active_w, uin, nx, ny, nz);
   FPC_LAUNCH_CHECK();

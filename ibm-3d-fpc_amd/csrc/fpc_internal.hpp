@@ -41,6 +41,22 @@ void ensure_init();
     }                                                                     \
   } while (0)
 
+// Kernel timing with HIP events on the library stream (fpc_ProfileEnable).
+bool prof_on(const char *cls);
+void prof_begin(const char *cls);
+void prof_end(const char *cls, double bytes);
+// Launches enqueued ahead of a device-side stop flag may return at once; they are
+// tagged with the Arnoldi step they belong to and dropped once the host knows how
+// many steps ran (records from index `from` with tag >= ran).
+size_t prof_mark();
+void prof_tag(int tag);
+void prof_drop_skipped(size_t from, int ran);
+#define FPC_PROF_BEGIN(cls)                    \
+  const bool prof_on_ = ::fpc::prof_on(cls);   \
+  if (prof_on_) ::fpc::prof_begin(cls)
+#define FPC_PROF_END(cls, bytes) \
+  if (prof_on_) ::fpc::prof_end(cls, (double)(bytes))
+
 // Launch check: reports a failed launch as an error return.
 #define FPC_LAUNCH_CHECK() FPC_HIP(hipGetLastError())
 

@@ -520,17 +520,32 @@ static int arnoldi_step(GmresImpl *G) {
   dim3 g, b;
   launch_shape(d, g, b);
   const double *dinv = (G->pc_kind == PC_JACOBI1) ? G->A->dinv : nullptr;
+  // bytes per row: 7 coefficients, the input, 1/diag when Jacobi is fused, the output
+  FPC_PROF_BEGIN(dinv ? "spmv_jacobi" : "spmv");
   k_arnoldi_matvec<<<g, b, 0, stream()>>>(G->A->coef, G->A->pitch, d.nx, d.ny, d.nz, G->V, G->vp, nullptr, dinv, G->S);
+  FPC_PROF_END(dinv ? "spmv_jacobi" : "spmv", (double)d.N * (dinv ? 80.0 : 72.0));
   FPC_LAUNCH_CHECK();
   return 0;
 }
 
-static int arnoldi_tail(GmresImpl *G) {
-  k_multidot<<<NB, 256, 0, stream()>>>(G->V, G->vp, G->N, G->S, G->part);
+// step: host-side index of the current basis vector (for the byte counts only).
+static int arnoldi_tail(GmresImpl *G, int step) {
+  double nv = (double)(step + 1);
+  {
+    // bytes per row: the step+1 basis vectors once, w once per group of eight
+    FPC_PROF_BEGIN("multidot");
+    k_multidot<<<NB, 256, 0, stream()>>>(G->V, G->vp, G->N, G->S, G->part);
+    FPC_PROF_END("multidot", (double)G->N * 8.0 * (nv + std::ceil(nv / 8.0)));
+  }
   FPC_LAUNCH_CHECK();
   k_reduce_h<<<KM + 1, 256, 0, stream()>>>(G->S, G->part);
   FPC_LAUNCH_CHECK();
-  k_orth_update<<<NB, 256, 0, stream()>>>(G->V, G->vp, G->N, G->S, G->part);
+  {
+    // bytes per row: the step+1 basis vectors, w read and written
+    FPC_PROF_BEGIN("orth_update");
+    k_orth_update<<<NB, 256, 0, stream()>>>(G->V, G->vp, G->N, G->S, G->part);
+    FPC_PROF_END("orth_update", (double)G->N * 8.0 * (nv + 2.0));
+  }
   FPC_LAUNCH_CHECK();
   k_arnoldi_tail<<<1, 256, 0, stream()>>>(G->S, G->part);
   FPC_LAUNCH_CHECK();
@@ -548,7 +563,9 @@ static int arnoldi_step_pc(GmresImpl *G, int step) {
   // the preconditioner input is the stored (unnormalised) p_step
   int rc = precond_apply(G, G->V + (long long)step * G->vp, z, skip);
   if (rc) return rc;
+  FPC_PROF_BEGIN("spmv");
   k_arnoldi_matvec<<<g, b, 0, stream()>>>(G->A->coef, G->A->pitch, d.nx, d.ny, d.nz, G->V, G->vp, z, nullptr, G->S);
+  FPC_PROF_END("spmv", (double)d.N * 72.0);
   FPC_LAUNCH_CHECK();
   return 0;
 }
@@ -598,6 +615,7 @@ int fpc_StructGMRESSolve(fpc_StructSolver s, fpc_StructMatrix A, fpc_StructVecto
     FPC_LAUNCH_CHECK();
     // Arnoldi steps in batches; the status record says when to stop.
     int issued = 0;
+    const size_t prof_from = prof_mark();
     int batch = std::max(1, std::min(G->pred, G->kd));
     if (G->pc_kind == PC_CALLBACK) batch = 1;
     for (;;) {
@@ -605,13 +623,14 @@ int fpc_StructGMRESSolve(fpc_StructSolver s, fpc_StructMatrix A, fpc_StructVecto
       nb = std::min(nb, G->maxit - iter - issued);
       if (nb <= 0) nb = 1;
       for (int q = 0; q < nb; ++q) {
+        prof_tag(issued + q);
         if (G->pc_kind == PC_NONE || G->pc_kind == PC_JACOBI1) {
           rc = arnoldi_step(G);
         } else {
           rc = arnoldi_step_pc(G, issued + q);
         }
         if (rc) return rc;
-        rc = arnoldi_tail(G);
+        rc = arnoldi_tail(G, issued + q);
         if (rc) return rc;
       }
       issued += nb;
@@ -622,6 +641,8 @@ int fpc_StructGMRESSolve(fpc_StructSolver s, fpc_StructMatrix A, fpc_StructVecto
       batch = (G->pc_kind == PC_PFMG || G->pc_kind == PC_CALLBACK) ? 1 : 2;
     }
     int i = G->hst->i;
+    prof_tag(-1);
+    prof_drop_skipped(prof_from, i);
     iter = G->hst->iter;
     r_norm = G->hst->r_norm;
     if (first_cycle) G->pred = std::max(1, i);

@@ -348,6 +348,77 @@ __global__ void k_prolong_add(int fnx, int fny, int fnz, int d, const T *__restr
   }
 }
 
+// Down-sweep of one level with a one-sweep zero-guess Jacobi smoother, fused:
+// x = w b dinv is recomputed on the fly at every stencil point, so only the
+// residual r = b - A x is written (x is recomputed by the up-sweep).
+template <typename T>
+__global__ void k_down_jacobi(const T *__restrict__ c, long long pitch, int nx, int ny, int nz,
+                              const T *__restrict__ dinv, const T *__restrict__ b, T *__restrict__ r, T w,
+                              const int *skip) {
+  if (skip && *skip) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int k = blockIdx.z;
+  if (i >= nx || j >= ny) return;
+  long long t = ((long long)k * ny + j) * nx + i, pl = (long long)nx * ny;
+  T s = c[t] * (w * b[t] * dinv[t]);
+  if (i > 0) s += c[pitch + t] * (w * b[t - 1] * dinv[t - 1]);
+  if (i < nx - 1) s += c[2 * pitch + t] * (w * b[t + 1] * dinv[t + 1]);
+  if (j > 0) s += c[3 * pitch + t] * (w * b[t - nx] * dinv[t - nx]);
+  if (j < ny - 1) s += c[4 * pitch + t] * (w * b[t + nx] * dinv[t + nx]);
+  if (k > 0) s += c[5 * pitch + t] * (w * b[t - pl] * dinv[t - pl]);
+  if (k < nz - 1) s += c[6 * pitch + t] * (w * b[t + pl] * dinv[t + pl]);
+  r[t] = b[t] - s;
+}
+
+// Pre-smoothed value plus the prolonged coarse correction at fine point p.
+template <typename T>
+__device__ inline T corrected(const T *b, const T *dinv, const T *wm, const T *wp, const T *yc, T w, const int *p,
+                              int d, int fnx, int fny, int cnx, int cny, const int *nc) {
+  long long t = ((long long)p[2] * fny + p[1]) * fnx + p[0];
+  T x = w * b[t] * dinv[t];
+  int q[3] = {p[0], p[1], p[2]};
+  long long cstride = (d == 0) ? 1 : (d == 1 ? (long long)cnx : (long long)cnx * cny);
+  if (p[d] & 1) {
+    q[d] = (p[d] - 1) >> 1;
+    x += yc[((long long)q[2] * cny + q[1]) * cnx + q[0]];
+  } else {
+    q[d] = p[d] >> 1;
+    long long cr = ((long long)q[2] * cny + q[1]) * cnx + q[0];
+    T s = (T)0;
+    if (q[d] - 1 >= 0) s += wm[t] * yc[cr - cstride];
+    if (q[d] < nc[d]) s += wp[t] * yc[cr];
+    x += s;
+  }
+  return x;
+}
+
+// Up-sweep of one level, fused: x = w b dinv + P y_c at the point and its
+// neighbours, then one Jacobi sweep y = x + w (b - A x) dinv.
+template <typename T>
+__global__ void k_up_jacobi(const T *__restrict__ c, long long pitch, int nx, int ny, int nz,
+                            const T *__restrict__ dinv, const T *__restrict__ b, const T *__restrict__ wm,
+                            const T *__restrict__ wp, const T *__restrict__ yc, int d, int cnx, int cny, int cnz,
+                            T *__restrict__ y, T w, const int *skip) {
+  if (skip && *skip) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int k = blockIdx.z;
+  if (i >= nx || j >= ny) return;
+  int nc[3] = {cnx, cny, cnz};
+  long long t = ((long long)k * ny + j) * nx + i;
+  int p[3] = {i, j, k};
+  T xc = corrected(b, dinv, wm, wp, yc, w, p, d, nx, ny, cnx, cny, nc);
+  T s = c[t] * xc;
+  if (i > 0) { int q[3] = {i - 1, j, k}; s += c[pitch + t] * corrected(b, dinv, wm, wp, yc, w, q, d, nx, ny, cnx, cny, nc); }
+  if (i < nx - 1) { int q[3] = {i + 1, j, k}; s += c[2 * pitch + t] * corrected(b, dinv, wm, wp, yc, w, q, d, nx, ny, cnx, cny, nc); }
+  if (j > 0) { int q[3] = {i, j - 1, k}; s += c[3 * pitch + t] * corrected(b, dinv, wm, wp, yc, w, q, d, nx, ny, cnx, cny, nc); }
+  if (j < ny - 1) { int q[3] = {i, j + 1, k}; s += c[4 * pitch + t] * corrected(b, dinv, wm, wp, yc, w, q, d, nx, ny, cnx, cny, nc); }
+  if (k > 0) { int q[3] = {i, j, k - 1}; s += c[5 * pitch + t] * corrected(b, dinv, wm, wp, yc, w, q, d, nx, ny, cnx, cny, nc); }
+  if (k < nz - 1) { int q[3] = {i, j, k + 1}; s += c[6 * pitch + t] * corrected(b, dinv, wm, wp, yc, w, q, d, nx, ny, cnx, cny, nc); }
+  y[t] = xc + w * (b[t] - s) * dinv[t];
+}
+
 // x = Ainv b on the coarsest level (FP64 inverse), one block.
 template <typename T>
 __global__ void k_coarse_solve(const double *__restrict__ inv, int n, const T *__restrict__ b, T *__restrict__ x,
@@ -544,6 +615,8 @@ int vcycle(PfmgImpl *P, const double *in, double *out, const int *skip) {
     b0 = (const T *)L0.b;
   }
   bool rb = P->relax >= 2;
+  // one Jacobi sweep before and after: the fused down/up kernels
+  bool fused = !rb && P->npre == 1 && P->npost == 1;
   // level-0 solution buffer: the output itself in FP64
   T *y0 = fp64 ? (T *)out : (T *)L0.y;
   auto bof = [&](int l) -> const T * { return l == 0 ? b0 : (const T *)P->L[(size_t)l].b; };
@@ -559,6 +632,27 @@ int vcycle(PfmgImpl *P, const double *in, double *out, const int *skip) {
     const T *c = (const T *)F.coef;
     const T *dv = (const T *)F.dinv;
     T *x = xof(l);
+    const double sz = (double)sizeof(T);
+    if (fused) {
+      {
+        // bytes per point: 7 coefficients, b, 1/diag, r
+        FPC_PROF_BEGIN("mg_down");
+        k_down_jacobi<T><<<g, bl, 0, st>>>(c, F.pitch, F.d.nx, F.d.ny, F.d.nz, dv, bof(l), (T *)F.r, w_of(l), skip);
+        FPC_PROF_END("mg_down", (double)F.d.N * 10.0 * sz);
+      }
+      FPC_LAUNCH_CHECK();
+      dim3 gc, bc;
+      launch_shape(C.d, gc, bc);
+      {
+        // bytes: r and one interpolation weight per fine point, b of each coarse point
+        FPC_PROF_BEGIN("mg_restrict");
+        k_restrict<T><<<gc, bc, 0, st>>>(F.d.nx, F.d.ny, F.d.nz, F.cdir, (const T *)F.r, (const T *)F.wm,
+                                        (const T *)F.wp, C.d.nx, C.d.ny, C.d.nz, (T *)C.b, skip);
+        FPC_PROF_END("mg_restrict", ((double)F.d.N * 2.0 + (double)C.d.N) * sz);
+      }
+      FPC_LAUNCH_CHECK();
+      continue;
+    }
     if (rb) {
       for (int sw = 0; sw < P->npre; ++sw) {
         int c0 = 0, c1 = 1;
@@ -593,6 +687,18 @@ int vcycle(PfmgImpl *P, const double *in, double *out, const int *skip) {
     const T *c = (const T *)F.coef;
     const T *dv = (const T *)F.dinv;
     T *x = xof(l);
+    if (fused) {
+      {
+        // bytes per fine point: 7 coefficients, b, 1/diag, two weights, y; plus the coarse y
+        FPC_PROF_BEGIN("mg_up");
+        k_up_jacobi<T><<<g, bl, 0, st>>>(c, F.pitch, F.d.nx, F.d.ny, F.d.nz, dv, bof(l), (const T *)F.wm,
+                                         (const T *)F.wp, (const T *)yof(l + 1), F.cdir, C.d.nx, C.d.ny, C.d.nz,
+                                         yof(l), w_of(l), skip);
+        FPC_PROF_END("mg_up", ((double)F.d.N * 12.0 + (double)C.d.N) * (double)sizeof(T));
+      }
+      FPC_LAUNCH_CHECK();
+      continue;
+    }
     k_prolong_add<T><<<g, bl, 0, st>>>(F.d.nx, F.d.ny, F.d.nz, F.cdir, (const T *)F.wm, (const T *)F.wp,
                                        (const T *)yof(l + 1), C.d.nx, C.d.ny, C.d.nz, x, skip);
     if (rb) {
@@ -779,6 +885,29 @@ int fpc_StructPFMGGetLevelInfo(fpc_StructSolver s, int level, int *dims, int *cd
   if (dims) { dims[0] = l.d.nx; dims[1] = l.d.ny; dims[2] = l.d.nz; }
   if (cdir) *cdir = l.cdir;
   if (weight) *weight = l.omega;
+  return 0;
+}
+
+int fpc_StructPFMGGetLevelMatrix(fpc_StructSolver s, int level, double *values) {
+  PIMPL(s);
+  FPC_CHECK_ARG(values, "null values");
+  FPC_CHECK_ARG(level >= 0 && level < (int)P->L.size(), "level out of range");
+  const DLevel &l = P->L[(size_t)level];
+  long long N = l.d.N;
+  std::vector<double> planes((size_t)kSlots * (size_t)N);
+  for (int q = 0; q < kSlots; ++q) {
+    if (P->built_bits == 32) {
+      std::vector<float> t((size_t)N);
+      FPC_HIP(hipMemcpy(t.data(), (const float *)l.coef + (long long)q * l.pitch, (size_t)N * sizeof(float),
+                        hipMemcpyDeviceToHost));
+      for (long long r = 0; r < N; ++r) planes[(size_t)q * (size_t)N + (size_t)r] = t[(size_t)r];
+    } else {
+      FPC_HIP(hipMemcpy(planes.data() + (size_t)q * (size_t)N, (const double *)l.coef + (long long)q * l.pitch,
+                        (size_t)N * sizeof(double), hipMemcpyDeviceToHost));
+    }
+  }
+  for (long long r = 0; r < N; ++r)
+    for (int q = 0; q < kSlots; ++q) values[7 * r + q] = planes[(size_t)q * (size_t)N + (size_t)r];
   return 0;
 }
 

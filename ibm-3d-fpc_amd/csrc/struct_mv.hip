@@ -69,6 +69,59 @@ double *scratch_doubles(size_t n, int which) {
   return g_scratch[(size_t)which];
 }
 
+// ---------------------------------------------------------------- kernel timing
+namespace {
+struct ProfRec {
+  std::string cls;
+  double bytes;
+  hipEvent_t e0, e1;
+  int tag;
+  bool dropped;
+};
+std::string g_prof_filter;
+bool g_prof = false;
+std::vector<ProfRec> g_prof_recs;
+std::vector<std::pair<hipEvent_t, hipEvent_t>> g_prof_pool;
+size_t g_prof_used = 0;
+hipEvent_t g_prof_open = nullptr;
+int g_prof_tag = -1;
+}  // namespace
+
+size_t prof_mark() { return g_prof_recs.size(); }
+
+void prof_tag(int tag) { g_prof_tag = tag; }
+
+void prof_drop_skipped(size_t from, int ran) {
+  for (size_t q = from; q < g_prof_recs.size(); ++q)
+    if (g_prof_recs[q].tag >= ran) g_prof_recs[q].dropped = true;
+}
+
+bool prof_on(const char *cls) {
+  if (!g_prof) return false;
+  if (g_prof_filter == "*") return true;
+  std::string f = "," + g_prof_filter + ",", c = "," + std::string(cls) + ",";
+  return f.find(c) != std::string::npos;
+}
+
+void prof_begin(const char *cls) {
+  (void)cls;
+  if (g_prof_used == g_prof_pool.size()) {
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+    g_prof_pool.emplace_back(a, b);
+  }
+  g_prof_open = g_prof_pool[g_prof_used].first;
+  (void)hipEventRecord(g_prof_open, stream());
+}
+
+void prof_end(const char *cls, double bytes) {
+  if (!g_prof_open || g_prof_used >= g_prof_pool.size()) return;
+  auto &pr = g_prof_pool[g_prof_used++];
+  (void)hipEventRecord(pr.second, stream());
+  g_prof_recs.push_back({cls, bytes, pr.first, pr.second, g_prof_tag, false});
+  g_prof_open = nullptr;
+}
+
 // ---------------------------------------------------------------- kernels
 namespace {
 
@@ -703,6 +756,54 @@ int fpc_MemcpyToDevice(void *dst, const void *src, size_t bytes) {
   FPC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, stream()));
   FPC_HIP(hipStreamSynchronize(stream()));
   return 0;
+}
+
+int fpc_ProfileEnable(const char *names) {
+  g_prof_filter = names ? names : "";
+  g_prof = !g_prof_filter.empty();
+  return 0;
+}
+
+int fpc_ProfileReset(void) {
+  g_prof_recs.clear();
+  g_prof_used = 0;
+  g_prof_open = nullptr;
+  return 0;
+}
+
+int fpc_ProfileReport(char *buf, int buflen) {
+  FPC_HIP(hipStreamSynchronize(stream()));
+  std::vector<std::string> names;
+  std::vector<long long> cnt;
+  std::vector<double> ms, bytes;
+  for (auto &r : g_prof_recs) {
+    if (r.dropped) continue;
+    float t = 0.0f;
+    if (hipEventElapsedTime(&t, r.e0, r.e1) != hipSuccess) t = 0.0f;
+    size_t q = 0;
+    while (q < names.size() && names[q] != r.cls) ++q;
+    if (q == names.size()) {
+      names.push_back(r.cls);
+      cnt.push_back(0);
+      ms.push_back(0.0);
+      bytes.push_back(0.0);
+    }
+    cnt[q] += 1;
+    ms[q] += t;
+    bytes[q] += r.bytes;
+  }
+  std::string out;
+  char line[256];
+  for (size_t q = 0; q < names.size(); ++q) {
+    snprintf(line, sizeof(line), "%s %lld %.6f %.6e\n", names[q].c_str(), cnt[q], ms[q], bytes[q]);
+    out += line;
+  }
+  if (buf && buflen > 0) {
+    size_t n = std::min(out.size(), (size_t)buflen - 1);
+    std::memcpy(buf, out.data(), n);
+    buf[n] = 0;
+  }
+  return (int)out.size() + 1;
 }
 
 int fpc_DeviceSynchronize(void) {

@@ -110,6 +110,7 @@ SIGNATURES = {
     "fpc_StructPFMGSetup": (_i, [_vp, _vp, _vp, _vp]), "fpc_StructPFMGSolve": (_i, [_vp, _vp, _vp, _vp]),
     "fpc_StructPFMGGetNumIterations": (_i, [_vp, _pi]), "fpc_StructPFMGGetNumLevels": (_i, [_vp, _pi]),
     "fpc_StructPFMGGetLevelInfo": (_i, [_vp, _i, _pi, _pi, _pd]), "fpc_StructPFMGDestroy": (_i, [_vp]),
+    "fpc_StructPFMGGetLevelMatrix": (_i, [_vp, _i, _vp]),
     "fpc_CreateGrid3D": (_vp, [_i, _i, _i]), "fpc_CreateStencil3D": (_vp, []),
     "fpc_SetMatrix3D": (_i, [_vp, _i, _i, _i, _vp]), "fpc_CreateMatrix3D": (_vp, [_vp, _vp, _i, _i, _i, _vp]),
     "fpc_SetVector3D": (_i, [_vp, _i, _i, _i, _vp]), "fpc_GetVector3D": (_i, [_vp, _i, _i, _i, _vp]),
@@ -120,6 +121,7 @@ SIGNATURES = {
     "fpc_CopyVecToManagedI": (_i, [_pvp, _vp, _sz]), "fpc_CopyVecToManagedB": (_i, [_pvp, _vp, _sz]),
     "fpc_Free": (_i, [_vp]), "fpc_MemcpyToHost": (_i, [_vp, _vp, _sz]), "fpc_MemcpyToDevice": (_i, [_vp, _vp, _sz]),
     "fpc_DeviceSynchronize": (_i, []),
+    "fpc_ProfileEnable": (_i, [_cp]), "fpc_ProfileReset": (_i, []), "fpc_ProfileReport": (_i, [_cp, _i]),
     # fpc_dense.h
     "fpc_apply_bc_cleanup": (_i, [_vp] * 7 + [_i] * 3),
     "fpc_conv_u": (_i, [_vp] * 5 + [_i] * 3 + [_d] * 3), "fpc_conv_v": (_i, [_vp] * 5 + [_i] * 3 + [_d] * 3),
@@ -292,6 +294,23 @@ class Sim:
         return float(lib().fpc_sim_timer(self.h, name.encode()))
 
 
+def profile_enable(names="*"):
+    check(lib().fpc_ProfileEnable(names.encode() if names else None), "ProfileEnable")
+    lib().fpc_ProfileReset()
+
+
+def profile_report():
+    """{kernel class: (launches, total_ms, algorithmic_bytes)} of the timed launches."""
+    n = lib().fpc_ProfileReport(None, 0)
+    buf = ctypes.create_string_buffer(max(n, 1))
+    lib().fpc_ProfileReport(buf, n)
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, cnt, ms, by = line.split()
+        out[name] = (int(cnt), float(ms), float(by))
+    return out
+
+
 def exported_symbols():
     """Names declared in include/*.h (parsed from the headers)."""
     import re
@@ -444,6 +463,12 @@ class Solver:
             w = ctypes.c_double()
             check(L.fpc_StructPFMGGetLevelInfo(self.pc, l, dims, ctypes.byref(cd), ctypes.byref(w)), "level info")
             out.append((tuple(dims), cd.value, w.value))
+        return out
+
+    def level_matrix(self, level):
+        dims = self.levels()[level][0]
+        out = np.empty(7 * dims[0] * dims[1] * dims[2])
+        check(lib().fpc_StructPFMGGetLevelMatrix(self.pc, level, _ptr(out)), "level matrix")
         return out
 
     def __del__(self):

@@ -160,6 +160,9 @@ int fpc_StructPFMGGetNumIterations(fpc_StructSolver solver, int *num_iterations)
 int fpc_StructPFMGGetNumLevels(fpc_StructSolver solver, int *num_levels);
 /* Level l: box dims (3 ints), coarsening direction (-1 on the coarsest), relax weight. */
 int fpc_StructPFMGGetLevelInfo(fpc_StructSolver solver, int level, int *dims, int *cdir, double *weight);
+/* Level l's 7-point operator as stored on the device, AoS (7 per point, entry order
+ * C,-x,+x,-y,+y,-z,+z), FP32 levels widened; values holds 7 * points of the level. */
+int fpc_StructPFMGGetLevelMatrix(fpc_StructSolver solver, int level, double *values);
 int fpc_StructPFMGDestroy(fpc_StructSolver solver);
 
 /* ---- helpers with the reference signatures (hypre_struct_utils.hpp:1-26) ---- */
@@ -188,6 +191,16 @@ int fpc_Free(void *p);
 int fpc_MemcpyToHost(void *dst, const void *src, size_t bytes);
 int fpc_MemcpyToDevice(void *dst, const void *src, size_t bytes);
 int fpc_DeviceSynchronize(void);
+
+/* ---- kernel timing (no reference counterpart) ----
+ * HIP events around the launches of the named kernel classes, on the library
+ * stream: spmv_jacobi, spmv, multidot, orth_update, mg_down, mg_up, mg_restrict,
+ * mg_relax, mg_residual, explicit, divergence, projection ("*" = all, NULL or ""
+ * = off).  Report lines: "class launches total_ms algorithmic_bytes\n"; returns
+ * the length needed including the terminating zero. */
+int fpc_ProfileEnable(const char *names);
+int fpc_ProfileReset(void);
+int fpc_ProfileReport(char *buf, int buflen);
 
 #ifdef __cplusplus
 }

@@ -1740,6 +1740,17 @@ int orc_mg_levels(int nx, int ny, int nz, const double *vals, int rap_type, int 
   return nl;
 }
 
+int orc_mg_level_matrix(int nx, int ny, int nz, const double *vals, int relax_type, int level, double *out) {
+  Box B = box_from_aos(nx, ny, nz, vals);
+  MG M;
+  mg_setup(M, B, relax_type, 0);
+  if (level < 0 || level >= (int)M.L.size()) return -1;
+  const Box &L = M.L[(size_t)level].A;
+  for (size_t r = 0; r < L.n(); ++r)
+    for (int s = 0; s < 7; ++s) out[7 * r + s] = L.a[s][r];
+  return 0;
+}
+
 void *orc_sim_create(const orc_params *par) { return sim_create(*par); }
 void orc_sim_destroy(void *s) {
   Sim *M = static_cast<Sim *>(s);

@@ -65,6 +65,9 @@ int orc_precond_apply(int nx, int ny, int nz, const double *vals, const double *
 int orc_mg_levels(int nx, int ny, int nz, const double *vals, int rap_type, int max_levels,
                   int *dims, int *cdir, double *omega);
 
+/* Level l's operator of that hierarchy as AoS (7 per point). */
+int orc_mg_level_matrix(int nx, int ny, int nz, const double *vals, int relax_type, int level, double *out);
+
 /* Whole simulation (setup + time loop). */
 void *orc_sim_create(const orc_params *par);
 void orc_sim_destroy(void *s);

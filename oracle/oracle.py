@@ -67,6 +67,7 @@ def lib():
         L.orc_gmres.argtypes = [ctypes.c_int] * 3 + [dp, dp, dp, ctypes.c_double] + [ctypes.c_int] * 6 + [ip, dp]
         L.orc_precond_apply.argtypes = [ctypes.c_int] * 3 + [dp, dp, dp] + [ctypes.c_int] * 4
         L.orc_mg_levels.argtypes = [ctypes.c_int] * 3 + [dp, ctypes.c_int, ctypes.c_int, ip, ip, dp]
+        L.orc_mg_level_matrix.argtypes = [ctypes.c_int] * 3 + [dp, ctypes.c_int, ctypes.c_int, dp]
         L.orc_sim_create.argtypes = [ctypes.POINTER(OrcParams)]
         L.orc_sim_create.restype = vp
         L.orc_sim_destroy.argtypes = [vp]
@@ -181,6 +182,16 @@ def mg_levels(nx, ny, nz, vals, rap_type=1, max_levels=0):
     om = np.zeros(64, dtype=np.float64)
     nl = lib().orc_mg_levels(nx, ny, nz, _dp(vals), rap_type, max_levels, _ip(dims), _ip(cdir), _dp(om))
     return [(tuple(dims[3 * l:3 * l + 3]), int(cdir[l]), float(om[l])) for l in range(nl)]
+
+
+def mg_level_matrix(nx, ny, nz, vals, level, relax_type=1):
+    vals = np.ascontiguousarray(vals, dtype=np.float64)
+    dims = mg_levels(nx, ny, nz, vals)[level][0]
+    out = np.empty(7 * int(dims[0]) * int(dims[1]) * int(dims[2]))
+    rc = lib().orc_mg_level_matrix(nx, ny, nz, _dp(vals), relax_type, level, _dp(out))
+    if rc:
+        raise IndexError(level)
+    return out
 
 
 class Sim:

@@ -36,8 +36,8 @@ def test_phases_match_oracle(torch_cuda):
         o.phase(0); g.phase(0)
         assert bits_equal(g.get("uin_step"), o.get("uin_step"))
         o.phase(1); g.phase(1)
-        for f in ("u", "v", "w"):  # BC cleanup in place
-            assert rel_l2(g.get(f), o.get(f)) < 1e-15, f
+        for f in ("u", "v", "w"):  # BC cleanup in place (step 1: exact; later: the previous step's rounding)
+            assert rel_l2(g.get(f), o.get(f)) < (1e-15 if step == 0 else 1e-12), f
         for f in ("rhs_u", "rhs_v", "rhs_w", "x_u", "x_v", "x_w"):
             assert rel_l2(g.get(f), o.get(f)) < 1e-13, f
         o.phase(2); g.phase(2)
@@ -65,7 +65,9 @@ def test_phases_match_oracle(torch_cuda):
 
 
 def test_run_matches_oracle(torch_cuda):
-    kw = dict(SMALL, init_profile=1, noise_amp=1e-3 * 0.45, seed=0, inlet_ramp_steps=4)
+    # rtol 1e-12: two correct GMRES implementations stopping at rtol t differ by up to
+    # cond(A) * t; at 1e-12 that stays below the 1e-9 field bar
+    kw = dict(SMALL, init_profile=1, noise_amp=1e-3 * 0.45, seed=0, inlet_ramp_steps=4, vel_tol=1e-12, p_tol=1e-12)
     o, g = _pair(kw)
     nsteps = 6
     o.step(nsteps)

@@ -95,6 +95,31 @@ def test_gmres_pfmg_matches_oracle(torch_cuda, relax):
         assert abs(a[2] - g[2]) <= 1e-14 * abs(a[2])
 
 
+def test_pfmg_level_operators_match_oracle(torch_cuda):
+    S, (nx, ny, nz), vals = _box(SMALL, "p")
+    box = F.StructBox(nx, ny, nz, vals)
+    sol = F.Solver(box, tol=1e-8, maxit=5, kdim=5, precond="pfmg")
+    lv = sol.levels()
+    assert len(lv) == len(O.mg_levels(nx, ny, nz, vals))
+    for l in range(len(lv)):
+        want = O.mg_level_matrix(nx, ny, nz, vals, l).reshape(-1, 7)
+        got = sol.level_matrix(l).reshape(-1, 7)
+        scale = np.max(np.abs(want), axis=1, keepdims=True) + 1e-300
+        assert np.max(np.abs(got - want) / scale) < 1e-13, l
+
+
+def test_pfmg_vcycle_on_poisson_matches_oracle(torch_cuda):
+    from test_oracle_pins import _poisson
+    nx, ny, nz = 40, 24, 20
+    vals = _poisson(nx, ny, nz, 1.0, 1.1, 0.9)
+    b = np.random.default_rng(12).standard_normal(nx * ny * nz)
+    box = F.StructBox(nx, ny, nz, vals)
+    for relax in (1, 2):
+        want = O.precond_apply(nx, ny, nz, vals, b, precond="pfmg", relax_type=relax)
+        got = F.pfmg_apply(box, b, relax_type=relax)
+        assert rel_l2(got, want) < 1e-12, relax
+
+
 @pytest.mark.parametrize("relax", [0, 1, 2, 3])
 def test_pfmg_vcycle_matches_oracle(torch_cuda, relax):
     S, (nx, ny, nz), vals = _box(SMALL, "p")
