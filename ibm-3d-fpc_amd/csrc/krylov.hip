@@ -28,6 +28,7 @@
 #include <cstring>
 
 #include "fpc_internal.hpp"
+#include "stencil_tile.hpp"
 
 namespace fpc {
 namespace {
@@ -92,6 +93,69 @@ __global__ void k_arnoldi_matvec(const double *__restrict__ c, long long pitch, 
     if (k < nz - 1) s += c[6 * pitch + r] * z[r + pl];
   }
   w[r] = sg * s;
+}
+
+// The same matvec on z-marching tiles (each row of A and z read once).
+template <bool kDinv>
+struct ArnoldiOp {
+  const double *c;
+  long long pitch;
+  int nx, ny;
+  const double *z, *dinv;
+  double *w;
+  double sg;
+  struct Raw {
+    double z, dinv;
+  };
+  struct Row {
+    Coef7<double> a;
+  };
+  __device__ long long at(int i, int j, int k) const { return ((long long)k * ny + j) * nx + i; }
+  __device__ Raw fetch(int i, int j, int k) const {
+    long long t = at(i, j, k);
+    return Raw{z[t], kDinv ? dinv[t] : 1.0};
+  }
+  __device__ double finish(const Raw &r) const { return kDinv ? r.z * r.dinv : r.z; }
+  __device__ Row row(int i, int j, int k) const {
+    Row q;
+    q.a.load(c, pitch, at(i, j, k));
+    return q;
+  }
+  __device__ void out(const Row &q, int i, int j, int k, const Nbr<double> &nb) { w[at(i, j, k)] = sg * q.a.apply(nb); }
+};
+
+template <bool kDinv, int TY>
+__global__ void __launch_bounds__(kTileX *TY)
+    k_arnoldi_matvec_tiled(const double *__restrict__ c, long long pitch, int nx, int ny, int nz,
+                           const double *__restrict__ V, long long vp, const double *__restrict__ zin,
+                           const double *__restrict__ dinv, State *S, int kz) {
+  if (S->st.done) return;
+  const int cur = S->st.i;
+  ArnoldiOp<kDinv> op{c, pitch, nx, ny, zin ? zin : V + (long long)cur * vp, dinv,
+                      (double *)V + (long long)(cur + 1) * vp, S->sigma[cur]};
+  const int k0 = blockIdx.z * kz;
+  march_tile<double, kTileX, TY>(op, nx, ny, nz, k0, min(k0 + kz, nz));
+}
+
+// w = sigma A z (z * 1/diag when dinv is given) for the current basis index.
+int launch_arnoldi_matvec(const fpc_StructMatrix_s *A, const double *V, long long vp, const double *zin,
+                          const double *dinv, State *S) {
+  Dims d = dims_of(A->grid);
+  if (tile_worthwhile(d)) {
+    TileLaunch tl = tile_launch<8>(d);
+    if (dinv)
+      k_arnoldi_matvec_tiled<true, 8><<<tl.grid, tl.block, 0, stream()>>>(A->coef, A->pitch, d.nx, d.ny, d.nz, V, vp,
+                                                                          zin, dinv, S, tl.kz);
+    else
+      k_arnoldi_matvec_tiled<false, 8><<<tl.grid, tl.block, 0, stream()>>>(A->coef, A->pitch, d.nx, d.ny, d.nz, V,
+                                                                           vp, zin, nullptr, S, tl.kz);
+  } else {
+    dim3 g, b;
+    launch_shape(d, g, b);
+    k_arnoldi_matvec<<<g, b, 0, stream()>>>(A->coef, A->pitch, d.nx, d.ny, d.nz, V, vp, zin, dinv, S);
+  }
+  FPC_LAUNCH_CHECK();
+  return 0;
 }
 
 // Partial dots <p_j, w> for j = 0..cur, eight vectors per pass over w.
@@ -361,9 +425,19 @@ struct GmresImpl {
   int pred = 4;  // expected Arnoldi steps per cycle (from the previous solve)
   fpc_StructMatrix_s *A = nullptr;
   fpc_StructVector_s tmp_in, tmp_out;  // callback preconditioner views
+  std::vector<hipGraphExec_t> graphs;  // one captured Arnoldi step per basis index
+  unsigned long long graph_key = 0;
 };
 
+void drop_graphs(GmresImpl *G) {
+  for (hipGraphExec_t g : G->graphs)
+    if (g) (void)hipGraphExecDestroy(g);
+  G->graphs.clear();
+  G->graph_key = 0;
+}
+
 void free_ws(GmresImpl *G) {
+  drop_graphs(G);
   if (G->V) (void)hipFree(G->V);
   if (G->Z) (void)hipFree(G->Z);
   if (G->W) (void)hipFree(G->W);
@@ -517,15 +591,12 @@ static int true_residual(GmresImpl *G, const double *b, const double *x, double 
 // One Arnoldi matvec with no preconditioner or with the fused one-sweep Jacobi.
 static int arnoldi_step(GmresImpl *G) {
   Dims d = dims_of(G->A->grid);
-  dim3 g, b;
-  launch_shape(d, g, b);
   const double *dinv = (G->pc_kind == PC_JACOBI1) ? G->A->dinv : nullptr;
   // bytes per row: 7 coefficients, the input, 1/diag when Jacobi is fused, the output
   FPC_PROF_BEGIN(dinv ? "spmv_jacobi" : "spmv");
-  k_arnoldi_matvec<<<g, b, 0, stream()>>>(G->A->coef, G->A->pitch, d.nx, d.ny, d.nz, G->V, G->vp, nullptr, dinv, G->S);
+  int rc = launch_arnoldi_matvec(G->A, G->V, G->vp, nullptr, dinv, G->S);
   FPC_PROF_END(dinv ? "spmv_jacobi" : "spmv", (double)d.N * (dinv ? 80.0 : 72.0));
-  FPC_LAUNCH_CHECK();
-  return 0;
+  return rc;
 }
 
 // step: host-side index of the current basis vector (for the byte counts only).
@@ -556,17 +627,81 @@ static int arnoldi_tail(GmresImpl *G, int step) {
 // of the current basis vector (used for the flexible Z slot).
 static int arnoldi_step_pc(GmresImpl *G, int step) {
   Dims d = dims_of(G->A->grid);
-  dim3 g, b;
-  launch_shape(d, g, b);
   int *skip = &G->S->st.done;
   double *z = G->flexible ? G->Z + (long long)step * G->vp : G->Z;
   // the preconditioner input is the stored (unnormalised) p_step
   int rc = precond_apply(G, G->V + (long long)step * G->vp, z, skip);
   if (rc) return rc;
   FPC_PROF_BEGIN("spmv");
-  k_arnoldi_matvec<<<g, b, 0, stream()>>>(G->A->coef, G->A->pitch, d.nx, d.ny, d.nz, G->V, G->vp, z, nullptr, G->S);
+  rc = launch_arnoldi_matvec(G->A, G->V, G->vp, z, nullptr, G->S);
   FPC_PROF_END("spmv", (double)d.N * 72.0);
-  FPC_LAUNCH_CHECK();
+  return rc;
+}
+
+// One Arnoldi step (preconditioner, matvec, orthogonalisation, Givens update).
+static int run_step(GmresImpl *G, int step) {
+  int rc = (G->pc_kind == PC_NONE || G->pc_kind == PC_JACOBI1) ? arnoldi_step(G) : arnoldi_step_pc(G, step);
+  if (rc) return rc;
+  return arnoldi_tail(G, step);
+}
+
+// Captured steps are replayed when every kernel argument of the step is fixed by
+// the solver state: no callback preconditioner (its own launches are unknown), a
+// single V-cycle per PFMG apply, no kernel timing.
+static bool graphs_usable(const GmresImpl *G) {
+  if (!tile_tuning().graphs || prof_any()) return false;
+  switch (G->pc_kind) {
+    case PC_NONE:
+    case PC_JACOBI1:
+      return true;
+    case PC_PFMG:
+      return pfmg_graph_safe(G->pc->impl);
+    default:
+      return false;
+  }
+}
+
+static unsigned long long graph_key(const GmresImpl *G) {
+  unsigned long long h = 1469598103934665603ULL;
+  auto mix = [&](unsigned long long v) { h = (h ^ v) * 1099511628211ULL; };
+  mix((unsigned long long)(uintptr_t)G->A);
+  mix(G->A->version);
+  mix((unsigned long long)(uintptr_t)G->A->dinv);
+  mix((unsigned long long)(uintptr_t)G->V);
+  mix((unsigned long long)(uintptr_t)G->Z);
+  mix((unsigned long long)(uintptr_t)G->S);
+  mix((unsigned long long)G->kd);
+  mix((unsigned long long)G->flexible);
+  mix((unsigned long long)G->pc_kind);
+  mix(tile_tuning().generation);
+  if (G->pc_kind == PC_PFMG) mix(pfmg_signature(G->pc->impl));
+  return h | 1ULL;  // never 0 (0 = no graphs held)
+}
+
+// Step `step` as a replay of its captured graph (captured on first use).
+static int launch_step(GmresImpl *G, int step, bool use_graphs) {
+  if (!use_graphs) return run_step(G, step);
+  if ((int)G->graphs.size() < G->kd) G->graphs.resize((size_t)G->kd, nullptr);
+  if (!G->graphs[(size_t)step]) {
+    FPC_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeRelaxed));
+    int rc = run_step(G, step);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(stream(), &g);
+    if (rc || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      if (!rc) set_error(FPC_ERROR_GENERIC, "GMRES: step capture failed: %s", hipGetErrorString(e));
+      return rc ? rc : FPC_ERROR_GENERIC;
+    }
+    hipGraphExec_t ex = nullptr;
+    e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      set_error(FPC_ERROR_GENERIC, "GMRES: graph instantiation failed: %s", hipGetErrorString(e));
+      return FPC_ERROR_GENERIC;
+    }
+    G->graphs[(size_t)step] = ex;
+  }
+  FPC_HIP(hipGraphLaunch(G->graphs[(size_t)step], stream()));
   return 0;
 }
 
@@ -582,6 +717,14 @@ int fpc_StructGMRESSolve(fpc_StructSolver s, fpc_StructMatrix A, fpc_StructVecto
   if (G->pc_kind == PC_PFMG) {
     rc = pfmg_setup(G->pc->impl, A);  // no-op unless A changed
     if (rc) return rc;
+  }
+  const bool use_graphs = graphs_usable(G);
+  if (use_graphs) {
+    const unsigned long long key = graph_key(G);
+    if (key != G->graph_key) {
+      drop_graphs(G);
+      G->graph_key = key;
+    }
   }
   // initial residual into V[0]
   rc = true_residual(G, b, x, G->V, true);
@@ -624,13 +767,7 @@ int fpc_StructGMRESSolve(fpc_StructSolver s, fpc_StructMatrix A, fpc_StructVecto
       if (nb <= 0) nb = 1;
       for (int q = 0; q < nb; ++q) {
         prof_tag(issued + q);
-        if (G->pc_kind == PC_NONE || G->pc_kind == PC_JACOBI1) {
-          rc = arnoldi_step(G);
-        } else {
-          rc = arnoldi_step_pc(G, issued + q);
-        }
-        if (rc) return rc;
-        rc = arnoldi_tail(G, issued + q);
+        rc = launch_step(G, issued + q, use_graphs);
         if (rc) return rc;
       }
       issued += nb;

@@ -35,6 +35,7 @@
 #include <vector>
 
 #include "fpc_internal.hpp"
+#include "stencil_tile.hpp"
 
 namespace fpc {
 namespace {
@@ -372,54 +373,223 @@ __global__ void k_down_jacobi(const T *__restrict__ c, long long pitch, int nx, 
 }
 
 // Pre-smoothed value plus the prolonged coarse correction at fine point p.
-template <typename T>
-__device__ inline T corrected(const T *b, const T *dinv, const T *wm, const T *wp, const T *yc, T w, const int *p,
-                              int d, int fnx, int fny, int cnx, int cny, const int *nc) {
-  long long t = ((long long)p[2] * fny + p[1]) * fnx + p[0];
+// x + P e at fine point (i, j, k) with x = w D^-1 b recomputed: C points (odd
+// index along D) inject the coarse value, F points take wm e(c-1) + wp e(c).
+// D is the coarsened direction (a template so no index array is spilled).
+template <typename T, int D>
+__device__ inline T corrected(const T *b, const T *dinv, const T *wm, const T *wp, const T *yc, T w, int i, int j,
+                              int k, int fnx, int fny, int cnx, int cny, int ncd) {
+  const long long t = ((long long)k * fny + j) * fnx + i;
   T x = w * b[t] * dinv[t];
-  int q[3] = {p[0], p[1], p[2]};
-  long long cstride = (d == 0) ? 1 : (d == 1 ? (long long)cnx : (long long)cnx * cny);
-  if (p[d] & 1) {
-    q[d] = (p[d] - 1) >> 1;
-    x += yc[((long long)q[2] * cny + q[1]) * cnx + q[0]];
+  const int pd = (D == 0) ? i : (D == 1 ? j : k);
+  const long long cstride = (D == 0) ? 1 : (D == 1 ? (long long)cnx : (long long)cnx * cny);
+  const int q = (pd & 1) ? ((pd - 1) >> 1) : (pd >> 1);
+  const int ci = (D == 0) ? q : i, cj = (D == 1) ? q : j, ck = (D == 2) ? q : k;
+  const long long cr = ((long long)ck * cny + cj) * cnx + ci;
+  if (pd & 1) {
+    x += yc[cr];
   } else {
-    q[d] = p[d] >> 1;
-    long long cr = ((long long)q[2] * cny + q[1]) * cnx + q[0];
     T s = (T)0;
-    if (q[d] - 1 >= 0) s += wm[t] * yc[cr - cstride];
-    if (q[d] < nc[d]) s += wp[t] * yc[cr];
+    if (q - 1 >= 0) s += wm[t] * yc[cr - cstride];
+    if (q < ncd) s += wp[t] * yc[cr];
     x += s;
   }
   return x;
 }
 
-// Up-sweep of one level, fused: x = w b dinv + P y_c at the point and its
-// neighbours, then one Jacobi sweep y = x + w (b - A x) dinv.
-template <typename T>
+template <typename T, int D>
 __global__ void k_up_jacobi(const T *__restrict__ c, long long pitch, int nx, int ny, int nz,
                             const T *__restrict__ dinv, const T *__restrict__ b, const T *__restrict__ wm,
-                            const T *__restrict__ wp, const T *__restrict__ yc, int d, int cnx, int cny, int cnz,
+                            const T *__restrict__ wp, const T *__restrict__ yc, int cnx, int cny, int ncd,
                             T *__restrict__ y, T w, const int *skip) {
   if (skip && *skip) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y;
   int k = blockIdx.z;
   if (i >= nx || j >= ny) return;
-  int nc[3] = {cnx, cny, cnz};
   long long t = ((long long)k * ny + j) * nx + i;
-  int p[3] = {i, j, k};
-  T xc = corrected(b, dinv, wm, wp, yc, w, p, d, nx, ny, cnx, cny, nc);
+#define FPC_CORR(ii, jj, kk) corrected<T, D>(b, dinv, wm, wp, yc, w, ii, jj, kk, nx, ny, cnx, cny, ncd)
+  T xc = FPC_CORR(i, j, k);
   T s = c[t] * xc;
-  if (i > 0) { int q[3] = {i - 1, j, k}; s += c[pitch + t] * corrected(b, dinv, wm, wp, yc, w, q, d, nx, ny, cnx, cny, nc); }
-  if (i < nx - 1) { int q[3] = {i + 1, j, k}; s += c[2 * pitch + t] * corrected(b, dinv, wm, wp, yc, w, q, d, nx, ny, cnx, cny, nc); }
-  if (j > 0) { int q[3] = {i, j - 1, k}; s += c[3 * pitch + t] * corrected(b, dinv, wm, wp, yc, w, q, d, nx, ny, cnx, cny, nc); }
-  if (j < ny - 1) { int q[3] = {i, j + 1, k}; s += c[4 * pitch + t] * corrected(b, dinv, wm, wp, yc, w, q, d, nx, ny, cnx, cny, nc); }
-  if (k > 0) { int q[3] = {i, j, k - 1}; s += c[5 * pitch + t] * corrected(b, dinv, wm, wp, yc, w, q, d, nx, ny, cnx, cny, nc); }
-  if (k < nz - 1) { int q[3] = {i, j, k + 1}; s += c[6 * pitch + t] * corrected(b, dinv, wm, wp, yc, w, q, d, nx, ny, cnx, cny, nc); }
+  if (i > 0) s += c[pitch + t] * FPC_CORR(i - 1, j, k);
+  if (i < nx - 1) s += c[2 * pitch + t] * FPC_CORR(i + 1, j, k);
+  if (j > 0) s += c[3 * pitch + t] * FPC_CORR(i, j - 1, k);
+  if (j < ny - 1) s += c[4 * pitch + t] * FPC_CORR(i, j + 1, k);
+  if (k > 0) s += c[5 * pitch + t] * FPC_CORR(i, j, k - 1);
+  if (k < nz - 1) s += c[6 * pitch + t] * FPC_CORR(i, j, k + 1);
+#undef FPC_CORR
   y[t] = xc + w * (b[t] - s) * dinv[t];
 }
 
 // x = Ainv b on the coarsest level (FP64 inverse), one block.
+// ---------------------------------------------------------------- z-marching forms
+// The same arithmetic as k_down_jacobi / k_restrict / k_up_jacobi, on the
+// z-marching tiles: every row read once, z-neighbours carried in registers.
+
+// Jacobi sweep from zero and residual: r = b - A (w D^-1 b).
+template <typename T>
+struct DownOp {
+  const T *c;
+  long long pitch;
+  int nx, ny;
+  const T *dinv, *b;
+  T *r;
+  T w;
+  struct Raw {
+    T b, dinv;
+  };
+  struct Row {
+    Coef7<T> a;
+    T b;
+  };
+  __device__ long long at(int i, int j, int k) const { return ((long long)k * ny + j) * nx + i; }
+  __device__ Raw fetch(int i, int j, int k) const {
+    long long t = at(i, j, k);
+    return Raw{b[t], dinv[t]};
+  }
+  __device__ T finish(const Raw &q) const { return w * q.b * q.dinv; }
+  __device__ Row row(int i, int j, int k) const {
+    long long t = at(i, j, k);
+    Row q;
+    q.a.load(c, pitch, t);
+    q.b = b[t];
+    return q;
+  }
+  __device__ void out(const Row &q, int i, int j, int k, const Nbr<T> &nb) { r[at(i, j, k)] = q.b - q.a.apply(nb); }
+};
+
+// The same, restricted on the fly when the level coarsens z: coarse plane kc
+// (fine C plane 2kc+1) = r(C) + wp(F-) r(F-) + wm(F+) r(F+), in that order; the
+// fine residual is never stored.  Chunks start on even planes and run one plane
+// past their end so the F+ plane of their last coarse plane is seen.
+template <typename T>
+struct DownRestrictZOp : DownOp<T> {
+  const T *wm, *wp;
+  T *bc;
+  int nz, ncz, k0;
+  T r_prev = T(0), pend = T(0);
+  __device__ void out(const typename DownOp<T>::Row &q, int i, int j, int k, const Nbr<T> &nb) {
+    const long long t = this->at(i, j, k), pl = (long long)this->nx * this->ny;
+    const T rk = q.b - q.a.apply(nb);
+    if (k & 1) {
+      const T v = rk + wp[t - pl] * r_prev;
+      const int kc = (k - 1) >> 1;
+      if (k + 1 < nz) pend = v;
+      else if (kc < ncz) bc[((long long)kc * this->ny + j) * this->nx + i] = v;
+    } else if (k > k0) {
+      const int kc = (k >> 1) - 1;
+      if (kc < ncz) bc[((long long)kc * this->ny + j) * this->nx + i] = pend + wm[t] * rk;
+    }
+    r_prev = rk;
+  }
+};
+
+template <typename T, int TY>
+__global__ void __launch_bounds__(kTileX *TY)
+    k_down_tiled(const T *__restrict__ c, long long pitch, int nx, int ny, int nz, const T *__restrict__ dinv,
+                 const T *__restrict__ b, T *__restrict__ r, T w, int kz, const int *skip) {
+  if (skip && *skip) return;
+  DownOp<T> op{c, pitch, nx, ny, dinv, b, r, w};
+  const int k0 = blockIdx.z * kz;
+  march_tile<T, kTileX, TY>(op, nx, ny, nz, k0, min(k0 + kz, nz));
+}
+
+template <typename T, int TY>
+__global__ void __launch_bounds__(kTileX *TY)
+    k_down_restrict_z(const T *__restrict__ c, long long pitch, int nx, int ny, int nz, const T *__restrict__ dinv,
+                      const T *__restrict__ b, const T *__restrict__ wm, const T *__restrict__ wp,
+                      T *__restrict__ bc, int ncz, T w, int kz, const int *skip) {
+  if (skip && *skip) return;
+  DownRestrictZOp<T> op;
+  op.c = c; op.pitch = pitch; op.nx = nx; op.ny = ny; op.dinv = dinv; op.b = b; op.r = nullptr; op.w = w;
+  op.wm = wm; op.wp = wp; op.bc = bc; op.nz = nz; op.ncz = ncz;
+  const int k0 = blockIdx.z * kz, k1 = min(k0 + kz, nz);
+  op.k0 = k0;
+  march_tile<T, kTileX, TY>(op, nx, ny, nz, k0, k1 < nz ? k1 + 1 : k1);
+}
+
+// Prolongation of the coarse correction plus one Jacobi sweep:
+// y = x + P e, then y + w D^-1 (b - A y), with x = w D^-1 b recomputed.
+template <typename T, int D>
+struct UpOp {
+  const T *c;
+  long long pitch;
+  int nx, ny;
+  const T *dinv, *b, *wm, *wp, *yc;
+  int cnx, cny, ncd;
+  T *y;
+  T w;
+  // Loads of corrected<T, D>: C points use e1 only; at F points an absent coarse
+  // neighbour reads as a zero weight and value (0 + wp e1 == wp e1 exactly).
+  struct Raw {
+    T b, dinv, wm, wp, e0, e1;
+    bool f;
+  };
+  struct Row {
+    Coef7<T> a;
+    T b, dinv;
+  };
+  __device__ long long at(int i, int j, int k) const { return ((long long)k * ny + j) * nx + i; }
+  __device__ Raw fetch(int i, int j, int k) const {
+    const long long t = at(i, j, k);
+    const int pd = (D == 0) ? i : (D == 1 ? j : k);
+    const long long cstride = (D == 0) ? 1 : (D == 1 ? (long long)cnx : (long long)cnx * cny);
+    const int q = (pd & 1) ? ((pd - 1) >> 1) : (pd >> 1);
+    const int ci = (D == 0) ? q : i, cj = (D == 1) ? q : j, ck = (D == 2) ? q : k;
+    const long long cr = ((long long)ck * cny + cj) * cnx + ci;
+    Raw r;
+    r.b = b[t];
+    r.dinv = dinv[t];
+    r.f = !(pd & 1);
+    if (pd & 1) {
+      r.wm = r.wp = r.e0 = T(0);
+      r.e1 = yc[cr];
+    } else {
+      const bool lo = q - 1 >= 0, hi = q < ncd;
+      r.wm = lo ? wm[t] : T(0);
+      r.e0 = lo ? yc[cr - cstride] : T(0);
+      r.wp = hi ? wp[t] : T(0);
+      r.e1 = hi ? yc[cr] : T(0);
+    }
+    return r;
+  }
+  __device__ T finish(const Raw &r) const {
+    T x = w * r.b * r.dinv;
+    if (r.f) {
+      T s = (T)0;
+      s += r.wm * r.e0;
+      s += r.wp * r.e1;
+      x += s;
+    } else {
+      x += r.e1;
+    }
+    return x;
+  }
+  __device__ Row row(int i, int j, int k) const {
+    long long t = at(i, j, k);
+    Row q;
+    q.a.load(c, pitch, t);
+    q.b = b[t];
+    q.dinv = dinv[t];
+    return q;
+  }
+  __device__ void out(const Row &q, int i, int j, int k, const Nbr<T> &nb) {
+    y[at(i, j, k)] = nb.c + w * (q.b - q.a.apply(nb)) * q.dinv;
+  }
+};
+
+template <typename T, int D, int TY>
+__global__ void __launch_bounds__(kTileX *TY)
+    k_up_tiled(const T *__restrict__ c, long long pitch, int nx, int ny, int nz, const T *__restrict__ dinv,
+               const T *__restrict__ b, const T *__restrict__ wm, const T *__restrict__ wp,
+               const T *__restrict__ yc, int cnx, int cny, int ncd, T *__restrict__ y, T w, int kz,
+               const int *skip) {
+  if (skip && *skip) return;
+  UpOp<T, D> op{c, pitch, nx, ny, dinv, b, wm, wp, yc, cnx, cny, ncd, y, w};
+  const int k0 = blockIdx.z * kz;
+  march_tile<T, kTileX, TY>(op, nx, ny, nz, k0, min(k0 + kz, nz));
+}
+
 template <typename T>
 __global__ void k_coarse_solve(const double *__restrict__ inv, int n, const T *__restrict__ b, T *__restrict__ x,
                                const int *skip) {
@@ -431,6 +601,121 @@ __global__ void k_coarse_solve(const double *__restrict__ inv, int n, const T *_
     double s = 0.0;
     for (int q = 0; q < n; ++q) s += inv[(size_t)r * n + q] * bs[q];
     x[r] = (T)s;
+  }
+}
+
+// ---------------------------------------------------------------- coarse tail
+// Every level below a few thousand points costs a full kernel launch per
+// stage while its work takes well under a microsecond.  One workgroup of 1024
+// threads runs the whole remaining V-cycle instead: per level the Jacobi sweep
+// from zero with its residual, the restriction, then the dense coarsest solve,
+// then per level prolongation + Jacobi on the way up, with a workgroup barrier
+// between stages.  The arithmetic is that of k_down_jacobi, k_restrict,
+// k_coarse_solve and k_up_jacobi.
+constexpr int kTailMaxLevels = 20;
+constexpr int kTailThreads = 1024;
+
+template <typename T>
+struct TailLevel {
+  const T *c;
+  const T *dinv, *wm, *wp;
+  T *b, *y, *r;
+  long long pitch;
+  int nx, ny, nz, cdir;
+  T w;
+};
+
+template <typename T>
+struct TailArgs {
+  TailLevel<T> lv[kTailMaxLevels];
+  int n;  // levels in the tail, the coarsest included
+};
+
+template <typename T>
+__device__ inline void tail_down(const TailLevel<T> &L) {
+  const long long pl = (long long)L.nx * L.ny, N = pl * L.nz;
+  const T w = L.w;
+  for (long long t = threadIdx.x; t < N; t += kTailThreads) {
+    const int k = (int)(t / pl), rem = (int)(t - (long long)k * pl), j = rem / L.nx, i = rem - j * L.nx;
+    const T *c = L.c, *b = L.b, *dinv = L.dinv;
+    const long long pitch = L.pitch;
+    T s = c[t] * (w * b[t] * dinv[t]);
+    if (i > 0) s += c[pitch + t] * (w * b[t - 1] * dinv[t - 1]);
+    if (i < L.nx - 1) s += c[2 * pitch + t] * (w * b[t + 1] * dinv[t + 1]);
+    if (j > 0) s += c[3 * pitch + t] * (w * b[t - L.nx] * dinv[t - L.nx]);
+    if (j < L.ny - 1) s += c[4 * pitch + t] * (w * b[t + L.nx] * dinv[t + L.nx]);
+    if (k > 0) s += c[5 * pitch + t] * (w * b[t - pl] * dinv[t - pl]);
+    if (k < L.nz - 1) s += c[6 * pitch + t] * (w * b[t + pl] * dinv[t + pl]);
+    L.r[t] = b[t] - s;
+  }
+}
+
+template <typename T>
+__device__ inline void tail_restrict(const TailLevel<T> &F, const TailLevel<T> &C) {
+  const long long cpl = (long long)C.nx * C.ny, NC = cpl * C.nz;
+  const int d = F.cdir;
+  const long long stride = (d == 0) ? 1 : (d == 1 ? (long long)F.nx : (long long)F.nx * F.ny);
+  const int fnd = (d == 0) ? F.nx : (d == 1 ? F.ny : F.nz);
+  for (long long t = threadIdx.x; t < NC; t += kTailThreads) {
+    const int k = (int)(t / cpl), rem = (int)(t - (long long)k * cpl), j = rem / C.nx, i = rem - j * C.nx;
+    const int fi = (d == 0) ? 2 * i + 1 : i, fj = (d == 1) ? 2 * j + 1 : j, fk = (d == 2) ? 2 * k + 1 : k;
+    const int pd = (d == 0) ? fi : (d == 1 ? fj : fk);
+    const long long f0 = ((long long)fk * F.ny + fj) * F.nx + fi;
+    T v = F.r[f0] + F.wp[f0 - stride] * F.r[f0 - stride];
+    if (pd + 1 < fnd) v += F.wm[f0 + stride] * F.r[f0 + stride];
+    C.b[t] = v;
+  }
+}
+
+template <typename T, int D>
+__device__ inline void tail_up_d(const TailLevel<T> &F, const TailLevel<T> &C) {
+  const long long pl = (long long)F.nx * F.ny, N = pl * F.nz;
+  const int nx = F.nx, ny = F.ny, nz = F.nz;
+  const int ncd = (D == 0) ? C.nx : (D == 1 ? C.ny : C.nz);
+  const T *c = F.c, *b = F.b, *dinv = F.dinv, *wm = F.wm, *wp = F.wp, *yc = C.y;
+  const long long pitch = F.pitch;
+  const T w = F.w;
+  for (long long t = threadIdx.x; t < N; t += kTailThreads) {
+    const int k = (int)(t / pl), rem = (int)(t - (long long)k * pl), j = rem / nx, i = rem - j * nx;
+#define FPC_CORR(ii, jj, kk) corrected<T, D>(b, dinv, wm, wp, yc, w, ii, jj, kk, nx, ny, C.nx, C.ny, ncd)
+    T xc = FPC_CORR(i, j, k);
+    T s = c[t] * xc;
+    if (i > 0) s += c[pitch + t] * FPC_CORR(i - 1, j, k);
+    if (i < nx - 1) s += c[2 * pitch + t] * FPC_CORR(i + 1, j, k);
+    if (j > 0) s += c[3 * pitch + t] * FPC_CORR(i, j - 1, k);
+    if (j < ny - 1) s += c[4 * pitch + t] * FPC_CORR(i, j + 1, k);
+    if (k > 0) s += c[5 * pitch + t] * FPC_CORR(i, j, k - 1);
+    if (k < nz - 1) s += c[6 * pitch + t] * FPC_CORR(i, j, k + 1);
+#undef FPC_CORR
+    F.y[t] = xc + w * (b[t] - s) * dinv[t];
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kTailThreads)
+    k_vcycle_tail(const TailArgs<T> a, const double *__restrict__ inv, int ncoarse, const int *skip) {
+  if (skip && *skip) return;
+  for (int l = 0; l + 1 < a.n; ++l) {
+    tail_down<T>(a.lv[l]);
+    __syncthreads();
+    tail_restrict<T>(a.lv[l], a.lv[l + 1]);
+    __syncthreads();
+  }
+  {
+    const TailLevel<T> &L = a.lv[a.n - 1];
+    for (int r = threadIdx.x; r < ncoarse; r += kTailThreads) {
+      double s = 0.0;
+      for (int q = 0; q < ncoarse; ++q) s += inv[(size_t)r * ncoarse + q] * (double)L.b[q];
+      L.y[r] = (T)s;
+    }
+    __syncthreads();
+  }
+  for (int l = a.n - 2; l >= 0; --l) {
+    const int d = a.lv[l].cdir;
+    if (d == 0) tail_up_d<T, 0>(a.lv[l], a.lv[l + 1]);
+    else if (d == 1) tail_up_d<T, 1>(a.lv[l], a.lv[l + 1]);
+    else tail_up_d<T, 2>(a.lv[l], a.lv[l + 1]);
+    __syncthreads();
   }
 }
 
@@ -624,7 +909,17 @@ int vcycle(PfmgImpl *P, const double *in, double *out, const int *skip) {
   // red/black relaxes in place (x == y); Jacobi smooths x into y
   auto xof = [&](int l) -> T * { return rb ? yof(l) : (T *)P->L[(size_t)l].x; };
   auto w_of = [&](int l) -> T { return (T)(P->relax == 0 ? 1.0 : P->L[(size_t)l].omega); };
-  for (int l = 0; l < nl - 1; ++l) {
+  // levels lt .. nl-1 run in the single-workgroup tail (lt == nl: no tail)
+  int lt = nl;
+  const long long tail_max = tile_tuning().tail_max_points;
+  if (fused && tail_max > 0) {
+    for (int l = 1; l < nl; ++l)
+      if (P->L[(size_t)l].d.N <= tail_max) {
+        lt = std::max(l, nl - kTailMaxLevels);
+        break;
+      }
+  }
+  for (int l = 0; l < std::min(lt, nl - 1); ++l) {
     DLevel &F = P->L[(size_t)l];
     DLevel &C = P->L[(size_t)l + 1];
     dim3 g, bl;
@@ -634,17 +929,34 @@ int vcycle(PfmgImpl *P, const double *in, double *out, const int *skip) {
     T *x = xof(l);
     const double sz = (double)sizeof(T);
     if (fused) {
-      {
-        // bytes per point: 7 coefficients, b, 1/diag, r
-        FPC_PROF_BEGIN("mg_down");
+      // bytes per fine point: 7 coefficients, b, 1/diag, then r written (or, when z is
+      // restricted on the fly, half a weight pair read and half a coarse b written)
+      const char *cls = (l == 0) ? "mg_down0" : "mg_down";
+      if (tile_worthwhile(F.d)) {
+        TileLaunch tl = tile_launch<8>(F.d);
+        if (F.cdir == 2) {
+          FPC_PROF_BEGIN(cls);
+          k_down_restrict_z<T, 8><<<tl.grid, tl.block, 0, st>>>(c, F.pitch, F.d.nx, F.d.ny, F.d.nz, dv, bof(l),
+                                                                 (const T *)F.wm, (const T *)F.wp, (T *)C.b,
+                                                                 C.d.nz, w_of(l), tl.kz, skip);
+          FPC_PROF_END(cls, ((double)F.d.N * 10.0 + (double)C.d.N) * sz);
+          FPC_LAUNCH_CHECK();
+          continue;
+        }
+        FPC_PROF_BEGIN(cls);
+        k_down_tiled<T, 8><<<tl.grid, tl.block, 0, st>>>(c, F.pitch, F.d.nx, F.d.ny, F.d.nz, dv, bof(l), (T *)F.r,
+                                                          w_of(l), tl.kz, skip);
+        FPC_PROF_END(cls, (double)F.d.N * 10.0 * sz);
+      } else {
+        FPC_PROF_BEGIN(cls);
         k_down_jacobi<T><<<g, bl, 0, st>>>(c, F.pitch, F.d.nx, F.d.ny, F.d.nz, dv, bof(l), (T *)F.r, w_of(l), skip);
-        FPC_PROF_END("mg_down", (double)F.d.N * 10.0 * sz);
+        FPC_PROF_END(cls, (double)F.d.N * 10.0 * sz);
       }
       FPC_LAUNCH_CHECK();
       dim3 gc, bc;
       launch_shape(C.d, gc, bc);
       {
-        // bytes: r and one interpolation weight per fine point, b of each coarse point
+        // bytes: r at every fine point, the weights at the F points, b of each coarse point
         FPC_PROF_BEGIN("mg_restrict");
         k_restrict<T><<<gc, bc, 0, st>>>(F.d.nx, F.d.ny, F.d.nz, F.cdir, (const T *)F.r, (const T *)F.wm,
                                         (const T *)F.wp, C.d.nx, C.d.ny, C.d.nz, (T *)C.b, skip);
@@ -674,12 +986,24 @@ int vcycle(PfmgImpl *P, const double *in, double *out, const int *skip) {
                                     C.d.nx, C.d.ny, C.d.nz, (T *)C.b, skip);
     FPC_LAUNCH_CHECK();
   }
-  {
+  if (lt < nl) {
+    TailArgs<T> ta;
+    ta.n = nl - lt;
+    for (int q = 0; q < ta.n; ++q) {
+      DLevel &D = P->L[(size_t)(lt + q)];
+      ta.lv[q] = TailLevel<T>{(const T *)D.coef, (const T *)D.dinv, (const T *)D.wm, (const T *)D.wp, (T *)D.b,
+                              (T *)D.y, (T *)D.r, D.pitch, D.d.nx, D.d.ny, D.d.nz, D.cdir, w_of(lt + q)};
+    }
+    FPC_PROF_BEGIN("mg_tail");
+    k_vcycle_tail<T><<<1, kTailThreads, 0, st>>>(ta, P->inv, P->ncoarse, skip);
+    FPC_PROF_END("mg_tail", 0.0);
+    FPC_LAUNCH_CHECK();
+  } else {
     int lc = nl - 1;
     k_coarse_solve<T><<<1, 64, 0, st>>>(P->inv, P->ncoarse, bof(lc), yof(lc), skip);
     FPC_LAUNCH_CHECK();
   }
-  for (int l = nl - 2; l >= 0; --l) {
+  for (int l = std::min(lt, nl - 1) - 1; l >= 0; --l) {
     DLevel &F = P->L[(size_t)l];
     DLevel &C = P->L[(size_t)l + 1];
     dim3 g, bl;
@@ -688,14 +1012,32 @@ int vcycle(PfmgImpl *P, const double *in, double *out, const int *skip) {
     const T *dv = (const T *)F.dinv;
     T *x = xof(l);
     if (fused) {
-      {
-        // bytes per fine point: 7 coefficients, b, 1/diag, two weights, y; plus the coarse y
-        FPC_PROF_BEGIN("mg_up");
-        k_up_jacobi<T><<<g, bl, 0, st>>>(c, F.pitch, F.d.nx, F.d.ny, F.d.nz, dv, bof(l), (const T *)F.wm,
-                                         (const T *)F.wp, (const T *)yof(l + 1), F.cdir, C.d.nx, C.d.ny, C.d.nz,
-                                         yof(l), w_of(l), skip);
-        FPC_PROF_END("mg_up", ((double)F.d.N * 12.0 + (double)C.d.N) * (double)sizeof(T));
+      // bytes per fine point: 7 coefficients, b, 1/diag, y written, a weight pair at
+      // the F points (half of them); plus the coarse correction once
+      const char *cls = (l == 0) ? "mg_up0" : "mg_up";
+      const double bytes = ((double)F.d.N * 11.0 + (double)C.d.N) * (double)sizeof(T);
+      FPC_PROF_BEGIN(cls);
+      const int ncd = (F.cdir == 0) ? C.d.nx : (F.cdir == 1 ? C.d.ny : C.d.nz);
+      const T *wm = (const T *)F.wm, *wp = (const T *)F.wp, *yc = (const T *)yof(l + 1);
+      if (tile_worthwhile(F.d)) {
+        TileLaunch tl = tile_launch<8>(F.d);
+#define FPC_UP_TILED(D)                                                                                     \
+  k_up_tiled<T, D, 8><<<tl.grid, tl.block, 0, st>>>(c, F.pitch, F.d.nx, F.d.ny, F.d.nz, dv, bof(l), wm, wp, yc, \
+                                                    C.d.nx, C.d.ny, ncd, yof(l), w_of(l), tl.kz, skip)
+        if (F.cdir == 0) FPC_UP_TILED(0);
+        else if (F.cdir == 1) FPC_UP_TILED(1);
+        else FPC_UP_TILED(2);
+#undef FPC_UP_TILED
+      } else {
+#define FPC_UP_SIMPLE(D)                                                                                       \
+  k_up_jacobi<T, D><<<g, bl, 0, st>>>(c, F.pitch, F.d.nx, F.d.ny, F.d.nz, dv, bof(l), wm, wp, yc, C.d.nx, C.d.ny, \
+                                      ncd, yof(l), w_of(l), skip)
+        if (F.cdir == 0) FPC_UP_SIMPLE(0);
+        else if (F.cdir == 1) FPC_UP_SIMPLE(1);
+        else FPC_UP_SIMPLE(2);
+#undef FPC_UP_SIMPLE
       }
+      FPC_PROF_END(cls, bytes);
       FPC_LAUNCH_CHECK();
       continue;
     }
@@ -740,6 +1082,27 @@ int pfmg_setup(void *pf, fpc_StructMatrix_s *A) {
   PfmgImpl *P = (PfmgImpl *)pf;
   if (P->A == A && P->version == A->version && P->built_bits == P->bits && !P->L.empty()) return 0;
   return build(P, A);
+}
+
+unsigned long long pfmg_signature(void *pf) {
+  PfmgImpl *P = (PfmgImpl *)pf;
+  unsigned long long h = 1469598103934665603ULL;
+  auto mix = [&](unsigned long long v) { h = (h ^ v) * 1099511628211ULL; };
+  mix(P->version);
+  mix((unsigned long long)P->built_bits);
+  mix((unsigned long long)P->relax);
+  mix((unsigned long long)P->npre);
+  mix((unsigned long long)P->npost);
+  mix((unsigned long long)P->maxit);
+  mix((unsigned long long)P->L.size());
+  mix((unsigned long long)(uintptr_t)(P->L.empty() ? nullptr : P->L[0].coef));
+  mix((unsigned long long)(uintptr_t)P->inv);
+  return h;
+}
+
+bool pfmg_graph_safe(void *pf) {
+  PfmgImpl *P = (PfmgImpl *)pf;
+  return P->maxit == 1 && !P->L.empty();
 }
 
 int pfmg_apply(void *pf, const double *b, double *x, const int *skip) {

@@ -1,0 +1,185 @@
+// stencil_tile.hpp -- z-marching tiles for the 7-point box stencils (libfpc).
+//
+// A workgroup of TX x TY threads owns an x-y tile and walks a chunk of z-planes.
+// Each thread keeps the stencil input of its own column at k-1, k, k+1 in
+// registers, so the z-neighbours cost no extra memory traffic; the x/y
+// neighbours come from the current plane staged in LDS (double-buffered, one
+// barrier per plane).  The ring of 2*TX + 2*TY halo points around the tile is
+// evaluated by the first threads of the workgroup.  The stencil input is any
+// point function ("val"): a plain vector load, v * (1/diag), a Jacobi sweep
+// from zero, or an interpolated coarse-grid correction, so the same walk serves
+// SpMV, the MG smoothers and the Arnoldi matvec.
+//
+// Every global row is read once (x-contiguous, 64 lanes = one 512-byte FP64
+// row segment); the only redundant work is the halo ring (2/TY + 2/TX of the
+// point evaluations, served from L2) and two extra planes per z-chunk.
+#ifndef FPC_STENCIL_TILE_HPP
+#define FPC_STENCIL_TILE_HPP
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdlib>
+
+#include "fpc_internal.hpp"
+
+namespace fpc {
+
+constexpr int kTileX = 64;
+
+// Halo ring point q of a TX x TY tile -> offset (ox, oy) in [-1, TX] x [-1, TY].
+template <int TX, int TY>
+__device__ inline bool tile_ring(int q, int &ox, int &oy) {
+  if (q < TX) { ox = q; oy = -1; return true; }
+  q -= TX;
+  if (q < TX) { ox = q; oy = TY; return true; }
+  q -= TX;
+  if (q < TY) { ox = -1; oy = q; return true; }
+  q -= TY;
+  if (q < TY) { ox = TX; oy = q; return true; }
+  return false;
+}
+
+// Neighbour values of one point, in the stencil's slot order after the centre.
+template <typename T>
+struct Nbr {
+  T c, xm, xp, ym, yp, zm, zp;
+};
+
+// Walk planes k0 .. kend-1 of the tile owned by this workgroup.  The op splits
+// its point function in two so loads run ahead of use: op.fetch(i, j, k) issues
+// the loads (an Op::Raw) and op.finish(raw) does the arithmetic; op.row(i, j, k)
+// loads the point's own data (an Op::Row) and op.out(row, i, j, k, nb) consumes
+// the neighbour values.  Loads for plane k+1 (own row, halo ring) and for the
+// own column at k+2 are issued while plane k is computed; a barrier waits only
+// for LDS (s_waitcnt lgkmcnt), so they stay in flight across it.  Points
+// outside the box read as zero.
+template <typename T, int TX, int TY, class Op>
+__device__ inline void march_tile(Op &op, int nx, int ny, int nz, int k0, int kend) {
+  using Raw = typename Op::Raw;
+  using Row = typename Op::Row;
+  __shared__ T plane[2][TY + 2][TX + 2];
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i = blockIdx.x * TX + tx, j = blockIdx.y * TY + ty;
+  const bool in = (i < nx) && (j < ny);
+  int hx = 0, hy = 0;
+  const bool ring = tile_ring<TX, TY>(ty * TX + tx, hx, hy);
+  const int hi = blockIdx.x * TX + hx, hj = blockIdx.y * TY + hy;
+  const bool ring_in = ring && hi >= 0 && hi < nx && hj >= 0 && hj < ny;
+  T zm = (in && k0 > 0) ? op.finish(op.fetch(i, j, k0 - 1)) : T(0);
+  T zc = (in && k0 < nz) ? op.finish(op.fetch(i, j, k0)) : T(0);
+  Raw own1{}, ring0{};
+  Row row{};
+  if (in && k0 + 1 < nz) own1 = op.fetch(i, j, k0 + 1);
+  if (ring_in && k0 < kend) ring0 = op.fetch(hi, hj, k0);
+  if (in && k0 < kend) row = op.row(i, j, k0);
+  for (int k = k0; k < kend; ++k) {
+    Raw own2{}, ring1{};
+    Row row1{};
+    if (in && k + 2 < nz && k + 1 < kend) own2 = op.fetch(i, j, k + 2);
+    if (ring_in && k + 1 < kend) ring1 = op.fetch(hi, hj, k + 1);
+    if (in && k + 1 < kend) row1 = op.row(i, j, k + 1);
+    const T zp = (in && k + 1 < nz) ? op.finish(own1) : T(0);
+    T(*P)[TX + 2] = plane[(k - k0) & 1];
+    P[ty + 1][tx + 1] = zc;
+    if (ring) P[hy + 1][hx + 1] = ring_in ? op.finish(ring0) : T(0);
+    __syncthreads();
+    if (in) {
+      Nbr<T> nb{zc, P[ty + 1][tx], P[ty + 1][tx + 2], P[ty][tx + 1], P[ty + 2][tx + 1], zm, zp};
+      op.out(row, i, j, k, nb);
+    }
+    zm = zc;
+    zc = zp;
+    own1 = own2;
+    ring0 = ring1;
+    row = row1;
+  }
+}
+
+// The seven coefficients of a row, loaded ahead of the plane barrier; apply()
+// sums in slot order C, -x, +x, -y, +y, -z, +z (out-of-box neighbours are zero).
+template <typename T>
+struct Coef7 {
+  T a[7];
+  __device__ inline void load(const T *c, long long pitch, long long t) {
+#pragma unroll
+    for (int s = 0; s < 7; ++s) a[s] = c[s * pitch + t];
+  }
+  __device__ inline T apply(const Nbr<T> &nb) const {
+    T s = a[0] * nb.c;
+    s += a[1] * nb.xm;
+    s += a[2] * nb.xp;
+    s += a[3] * nb.ym;
+    s += a[4] * nb.yp;
+    s += a[5] * nb.zm;
+    s += a[6] * nb.zp;
+    return s;
+  }
+};
+
+// Launch geometry of a z-marching kernel: tile rows TY and z-chunk length.
+struct TileLaunch {
+  dim3 grid, block;
+  int kz;
+};
+
+// Launch tuning (fpc_SetTuning; environment defaults FPC_TILE_MIN_POINTS,
+// FPC_STENCIL_KZ, FPC_TAIL_MAX_POINTS, FPC_GRAPHS).  0 for kz = automatic.
+struct TileTuning {
+  long long min_points = -1;
+  int kz = -1;
+  long long tail_max_points = -1;  // MG levels this small run in one workgroup (0: never)
+  int graphs = -1;                 // capture GMRES Arnoldi steps as HIP graphs (0: never)
+  unsigned long long generation = 0;  // bumped by every change (captured graphs depend on it)
+};
+inline TileTuning &tile_tuning() {
+  static TileTuning t;
+  if (t.min_points < 0) {
+    const char *e = std::getenv("FPC_TILE_MIN_POINTS");
+    t.min_points = e ? std::max(0LL, std::atoll(e)) : (1LL << 20);
+  }
+  if (t.kz < 0) {
+    const char *e = std::getenv("FPC_STENCIL_KZ");
+    t.kz = e ? std::max(0, std::atoi(e)) : 0;
+  }
+  if (t.tail_max_points < 0) {
+    const char *e = std::getenv("FPC_TAIL_MAX_POINTS");
+    t.tail_max_points = e ? std::max(0LL, std::atoll(e)) : 12288;
+  }
+  if (t.graphs < 0) {
+    const char *e = std::getenv("FPC_GRAPHS");
+    t.graphs = e ? (std::atoi(e) != 0) : 1;
+  }
+  return t;
+}
+
+// Smaller boxes run the one-point-per-thread kernels.
+inline bool tile_worthwhile(const Dims &d) {
+  return d.nx >= 32 && d.ny >= 8 && d.N >= std::max(tile_tuning().min_points, 1LL << 15);
+}
+
+// z-chunk length: about 2048 workgroups in flight, chunks of at least 4 planes,
+// always even (z-restriction pairs stay inside a chunk).
+inline int tile_kz(const Dims &d, int ty) {
+  int kz = tile_tuning().kz;
+  if (kz <= 0) {
+    long long tiles = (long long)((d.nx + kTileX - 1) / kTileX) * ((d.ny + ty - 1) / ty);
+    long long chunks = std::max(1LL, 2048 / std::max(1LL, tiles));
+    kz = std::max((int)((d.nz + chunks - 1) / chunks), 4);
+  }
+  return kz + (kz & 1);
+}
+
+template <int TY>
+inline TileLaunch tile_launch(const Dims &d) {
+  TileLaunch L;
+  L.kz = tile_kz(d, TY);
+  L.block = dim3(kTileX, TY, 1);
+  L.grid = dim3((unsigned)((d.nx + kTileX - 1) / kTileX), (unsigned)((d.ny + TY - 1) / TY),
+                (unsigned)((d.nz + L.kz - 1) / L.kz));
+  return L;
+}
+
+}  // namespace fpc
+
+#endif

@@ -121,6 +121,7 @@ SIGNATURES = {
     "fpc_CopyVecToManagedI": (_i, [_pvp, _vp, _sz]), "fpc_CopyVecToManagedB": (_i, [_pvp, _vp, _sz]),
     "fpc_Free": (_i, [_vp]), "fpc_MemcpyToHost": (_i, [_vp, _vp, _sz]), "fpc_MemcpyToDevice": (_i, [_vp, _vp, _sz]),
     "fpc_DeviceSynchronize": (_i, []),
+    "fpc_SetTuning": (_i, [_cp, ctypes.c_longlong]),
     "fpc_ProfileEnable": (_i, [_cp]), "fpc_ProfileReset": (_i, []), "fpc_ProfileReport": (_i, [_cp, _i]),
     # fpc_dense.h
     "fpc_apply_bc_cleanup": (_i, [_vp] * 7 + [_i] * 3),
@@ -292,6 +293,11 @@ class Sim:
 
     def timer(self, name):
         return float(lib().fpc_sim_timer(self.h, name.encode()))
+
+
+def set_tuning(key, value):
+    """Launch tuning of the z-marching stencil kernels ("tile_min_points", "stencil_kz")."""
+    check(lib().fpc_SetTuning(key.encode(), int(value)), f"SetTuning({key})")
 
 
 def profile_enable(names="*"):

@@ -109,4 +109,27 @@ typedef fpc_PtrToStructSolverFcn HYPRE_PtrToStructSolverFcn;
 #define GetVector3D fpc_GetVector3D
 #define CreateVector3D fpc_CreateVector3D
 
+#ifdef __cplusplus
+#include <vector>
+/* hypre_struct_utils.hpp:27-43 take std::vector sources; these inline forms keep
+ * those call sites unchanged.  The buffers are device memory (hipMalloc): host
+ * code reads them back with fpc_MemcpyToHost. */
+static inline void MallocManagedC(double **p, size_t n) { (void)fpc_MallocManagedC(p, n); }
+static inline void MallocManagedD(double **p, size_t n) { (void)fpc_MallocManagedD(p, n); }
+static inline void MallocManagedI(int **p, size_t n) { (void)fpc_MallocManagedI(p, n); }
+static inline void MallocManagedB(unsigned char **p, size_t n) { (void)fpc_MallocManagedB(p, n); }
+static inline void CopyVecToManagedC(double **p, const std::vector<double> &src) {
+  (void)fpc_CopyVecToManagedC(p, src.data(), src.size());
+}
+static inline void CopyVecToManagedD(double **p, const std::vector<double> &src) {
+  (void)fpc_CopyVecToManagedD(p, src.data(), src.size());
+}
+static inline void CopyVecToManagedI(int **p, const std::vector<int> &src) {
+  (void)fpc_CopyVecToManagedI(p, src.data(), src.size());
+}
+static inline void CopyVecToManagedB(unsigned char **p, const std::vector<char> &src) {
+  (void)fpc_CopyVecToManagedB(p, (const unsigned char *)src.data(), src.size());
+}
+#endif
+
 #endif /* FPC_HYPRE_COMPAT_H */

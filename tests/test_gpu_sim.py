@@ -33,11 +33,15 @@ def test_phases_match_oracle(torch_cuda):
     kw = dict(TINY, init_profile=1, noise_amp=1e-3, seed=7, vel_tol=1e-12, p_tol=1e-12)
     o, g = _pair(kw)
     for step in range(2):
+        # each step's phases start from the oracle's state, so every phase is compared
+        # on identical inputs (the solves leave differences at the solve tolerance)
+        for f in FIELDS:
+            g.set(f, o.get(f))
         o.phase(0); g.phase(0)
         assert bits_equal(g.get("uin_step"), o.get("uin_step"))
         o.phase(1); g.phase(1)
-        for f in ("u", "v", "w"):  # BC cleanup in place (step 1: exact; later: the previous step's rounding)
-            assert rel_l2(g.get(f), o.get(f)) < (1e-15 if step == 0 else 1e-12), f
+        for f in ("u", "v", "w"):  # BC cleanup in place
+            assert rel_l2(g.get(f), o.get(f)) < 1e-15, f
         for f in ("rhs_u", "rhs_v", "rhs_w", "x_u", "x_v", "x_w"):
             assert rel_l2(g.get(f), o.get(f)) < 1e-13, f
         o.phase(2); g.phase(2)
@@ -60,7 +64,13 @@ def test_phases_match_oracle(torch_cuda):
         so, sg = o.scalars(), g.scalars()
         for k in ("umax", "vmax", "wmax"):
             assert abs(so[k] - sg[k]) <= 1e-9 * abs(so[k]), k
-        assert abs(so["div_l2"] - sg["div_l2"]) <= 1e-6 * max(abs(so["div_l2"]), 1e-300) + 1e-14
+        # After the projection D q + ramp*div_bc = -(dt/rho) * (pressure residual), so div_l2 is
+        # solver noise bounded by (dt/rho) * p_tol * ||rhs_p|| / sqrt(Np) for both solvers; two
+        # valid solves need not agree on it beyond that bound.
+        p = O.default_params(**kw)
+        n_p = O.Setup(p).count("Np")
+        bound = 1.5 * p.dt / p.rho * p.p_tol * np.linalg.norm(o.get("rhs_p")) / np.sqrt(n_p)
+        assert so["div_l2"] <= bound and sg["div_l2"] <= bound, (so["div_l2"], sg["div_l2"], bound)
         assert abs(so["cd"] - sg["cd"]) <= 1e-7 * abs(so["cd"])
 
 
