@@ -42,6 +42,10 @@ WORKLOAD = dict(nx=512, ny=96, nz=96, Lx=2.5, Ly=0.41, Lz=0.41, xcirc=0.5, ycirc
 
 
 HBM_PEAK_GBS = 8000.0
+# kernel classes whose every launch covers one fixed box (a roofline figure per
+# launch is meaningful); the other MG classes mix the coarse levels
+ROOFLINE_CLASSES = ("spmv", "spmv_jacobi", "mg_down0", "mg_up0", "multidot", "orth_update", "explicit",
+                    "divergence", "projection")
 
 
 def _classes(rep):
@@ -49,17 +53,28 @@ def _classes(rep):
             for k, (n, ms, b) in rep.items()}
 
 
-def run_sim(F, torch, kw, warmup, steps):
+def run_sim(F, torch, kw, warmup, steps, roofline_timing=True):
     """W untimed steps (the last one with every kernel class timed), then K timed steps
     with the dominant class timed by HIP events."""
     sim = F.Sim(F.default_params(**kw))
+    if not roofline_timing:
+        sim.step(warmup)
+        torch.cuda.synchronize()
+        its = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            sim.step(1)
+            its.append([int(sim.scalar(k)) for k in ("its_u", "its_v", "its_w", "its_p")])
+        torch.cuda.synchronize()
+        return sim, time.perf_counter() - t0, its, None, None, {}
     if warmup > 1:
         sim.step(warmup - 1)
     F.profile_enable("*")
     sim.step(1 if warmup >= 1 else 0)
     torch.cuda.synchronize()
     classes = _classes(F.profile_report()) if warmup >= 1 else {}
-    dom = max(classes, key=lambda k: classes[k]["ms"]) if classes else "*"
+    elig = [k for k in classes if k in ROOFLINE_CLASSES]
+    dom = max(elig, key=lambda k: classes[k]["ms"]) if elig else ",".join(ROOFLINE_CLASSES)
     F.profile_enable(dom)
     torch.cuda.synchronize()
     its = []
@@ -71,7 +86,7 @@ def run_sim(F, torch, kw, warmup, steps):
     dt = time.perf_counter() - t0
     rep = F.profile_report()
     F.profile_enable("")
-    if dom == "*":
+    if dom not in rep:
         classes = _classes(rep)
         dom = max(classes, key=lambda k: classes[k]["ms"])
     return sim, dt, its, dom, rep[dom], classes
@@ -124,6 +139,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-roofline", action="store_true", help="no kernel timing at all (A/B timing runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 2 GiB device-to-device copy 5 times (scale for HBM byte counters)")
     args = ap.parse_args()
@@ -144,7 +160,8 @@ def main():
         return
     import fpc_amd as F
     F.lib()
-    sim64, t64, its64, dom, rec, classes = run_sim(F, torch, WORKLOAD, args.warmup, args.steps)
+    sim64, t64, its64, dom, rec, classes = run_sim(F, torch, WORKLOAD, args.warmup, args.steps,
+                                                   not args.no_roofline)
     res = {
         "metric": "time-steps/s @512x96x96: FP64 vs FP32-MG at 1e-8 resid; stencil HBM GB/s",
         "value": args.steps / t64,
@@ -163,13 +180,13 @@ def main():
         "iterations_mean": [float(x) for x in np.mean(np.array(its64), axis=0)],
         "timers_s": {k: sim64.timer(k) for k in ("t_build", "t_mom", "t_p", "t_proj", "t_diag")},
         "cd": sim64.scalar("cd"), "cl": sim64.scalar("cl"),
-        "roofline": roofline(dom, rec),
+        "roofline": roofline(dom, rec) if dom else None,
         "cpu_baseline": None,
         "kernel_classes": classes,
     }
     if not args.no_fp32:
         sim32, t32, its32, dom32, rec32, cls32 = run_sim(F, torch, dict(WORKLOAD, mg_precision=32), args.warmup,
-                                                         args.steps)
+                                                         args.steps, not args.no_roofline)
         errs = {}
         for f in ("u", "v", "w", "p"):
             a, b = sim64.get(f), sim32.get(f)
@@ -178,7 +195,7 @@ def main():
                          "speedup_vs_fp64": t64 / t32, "field_rel_l2_vs_fp64": errs,
                          "cd_rel_err": abs(sim32.scalar("cd") - sim64.scalar("cd")) / abs(sim64.scalar("cd")),
                          "iterations_mean": [float(x) for x in np.mean(np.array(its32), axis=0)],
-                         "roofline": roofline(dom32, rec32), "kernel_classes": cls32}
+                         "roofline": roofline(dom32, rec32) if dom32 else None, "kernel_classes": cls32}
         sim32.close()
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline()

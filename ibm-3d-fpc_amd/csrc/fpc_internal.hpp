@@ -43,7 +43,6 @@ void ensure_init();
 
 // Kernel timing with HIP events on the library stream (fpc_ProfileEnable).
 bool prof_on(const char *cls);
-bool prof_any();  // any kernel class timed (captured graphs are not used then)
 void prof_begin(const char *cls);
 void prof_end(const char *cls, double bytes);
 // Launches enqueued ahead of a device-side stop flag may return at once; they are
@@ -163,10 +162,6 @@ struct JacobiImpl {
 int jacobi_apply(JacobiImpl *J, const double *b, double *x, const int *skip);
 int pfmg_setup(void *pf, fpc_StructMatrix_s *A);
 int pfmg_apply(void *pf, const double *b, double *x, const int *skip);
-// Identity of a built hierarchy and its cycle settings (changes whenever a
-// captured V-cycle would differ), and whether one apply is a plain V-cycle.
-unsigned long long pfmg_signature(void *pf);
-bool pfmg_graph_safe(void *pf);
 
 inline Dims dims_of(const fpc_StructGrid_s &g) {
   Dims d;

@@ -171,11 +171,24 @@ __global__ void __launch_bounds__(256) k_multidot(const double *__restrict__ V, 
     double acc[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) acc[q] = 0.0;
-    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
-      double wv = w[t];
+    // two points per trip: twice the loads in flight per thread
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    for (; t + stride < n; t += 2 * stride) {
+      const double wa = w[t], wb = w[t + stride];
 #pragma unroll
       for (int q = 0; q < 8; ++q)
-        if (q < nj) acc[q] += V[(long long)(j0 + q) * vp + t] * wv;
+        if (q < nj) {
+          const double *vq = V + (long long)(j0 + q) * vp;
+          acc[q] += vq[t] * wa;
+          acc[q] += vq[t + stride] * wb;
+        }
+    }
+    if (t < n) {
+      const double wa = w[t];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (q < nj) acc[q] += V[(long long)(j0 + q) * vp + t] * wa;
     }
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
@@ -219,7 +232,21 @@ __global__ void __launch_bounds__(256) k_orth_update(double *__restrict__ V, lon
   __syncthreads();
   double *w = V + (long long)(cur + 1) * vp;
   double acc = 0.0;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  for (; t + stride < n; t += 2 * stride) {
+    double va = w[t], vb = w[t + stride];
+    for (int q = 0; q <= cur; ++q) {
+      const double *vq = V + (long long)q * vp;
+      va -= coef[q] * vq[t];
+      vb -= coef[q] * vq[t + stride];
+    }
+    w[t] = va;
+    w[t + stride] = vb;
+    acc += va * va;
+    acc += vb * vb;
+  }
+  if (t < n) {
     double v = w[t];
     for (int q = 0; q <= cur; ++q) v -= coef[q] * V[(long long)q * vp + t];
     w[t] = v;
@@ -425,19 +452,9 @@ struct GmresImpl {
   int pred = 4;  // expected Arnoldi steps per cycle (from the previous solve)
   fpc_StructMatrix_s *A = nullptr;
   fpc_StructVector_s tmp_in, tmp_out;  // callback preconditioner views
-  std::vector<hipGraphExec_t> graphs;  // one captured Arnoldi step per basis index
-  unsigned long long graph_key = 0;
 };
 
-void drop_graphs(GmresImpl *G) {
-  for (hipGraphExec_t g : G->graphs)
-    if (g) (void)hipGraphExecDestroy(g);
-  G->graphs.clear();
-  G->graph_key = 0;
-}
-
 void free_ws(GmresImpl *G) {
-  drop_graphs(G);
   if (G->V) (void)hipFree(G->V);
   if (G->Z) (void)hipFree(G->Z);
   if (G->W) (void)hipFree(G->W);
@@ -645,66 +662,6 @@ static int run_step(GmresImpl *G, int step) {
   return arnoldi_tail(G, step);
 }
 
-// Captured steps are replayed when every kernel argument of the step is fixed by
-// the solver state: no callback preconditioner (its own launches are unknown), a
-// single V-cycle per PFMG apply, no kernel timing.
-static bool graphs_usable(const GmresImpl *G) {
-  if (!tile_tuning().graphs || prof_any()) return false;
-  switch (G->pc_kind) {
-    case PC_NONE:
-    case PC_JACOBI1:
-      return true;
-    case PC_PFMG:
-      return pfmg_graph_safe(G->pc->impl);
-    default:
-      return false;
-  }
-}
-
-static unsigned long long graph_key(const GmresImpl *G) {
-  unsigned long long h = 1469598103934665603ULL;
-  auto mix = [&](unsigned long long v) { h = (h ^ v) * 1099511628211ULL; };
-  mix((unsigned long long)(uintptr_t)G->A);
-  mix(G->A->version);
-  mix((unsigned long long)(uintptr_t)G->A->dinv);
-  mix((unsigned long long)(uintptr_t)G->V);
-  mix((unsigned long long)(uintptr_t)G->Z);
-  mix((unsigned long long)(uintptr_t)G->S);
-  mix((unsigned long long)G->kd);
-  mix((unsigned long long)G->flexible);
-  mix((unsigned long long)G->pc_kind);
-  mix(tile_tuning().generation);
-  if (G->pc_kind == PC_PFMG) mix(pfmg_signature(G->pc->impl));
-  return h | 1ULL;  // never 0 (0 = no graphs held)
-}
-
-// Step `step` as a replay of its captured graph (captured on first use).
-static int launch_step(GmresImpl *G, int step, bool use_graphs) {
-  if (!use_graphs) return run_step(G, step);
-  if ((int)G->graphs.size() < G->kd) G->graphs.resize((size_t)G->kd, nullptr);
-  if (!G->graphs[(size_t)step]) {
-    FPC_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeRelaxed));
-    int rc = run_step(G, step);
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(stream(), &g);
-    if (rc || e != hipSuccess) {
-      if (g) (void)hipGraphDestroy(g);
-      if (!rc) set_error(FPC_ERROR_GENERIC, "GMRES: step capture failed: %s", hipGetErrorString(e));
-      return rc ? rc : FPC_ERROR_GENERIC;
-    }
-    hipGraphExec_t ex = nullptr;
-    e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) {
-      set_error(FPC_ERROR_GENERIC, "GMRES: graph instantiation failed: %s", hipGetErrorString(e));
-      return FPC_ERROR_GENERIC;
-    }
-    G->graphs[(size_t)step] = ex;
-  }
-  FPC_HIP(hipGraphLaunch(G->graphs[(size_t)step], stream()));
-  return 0;
-}
-
 int fpc_StructGMRESSolve(fpc_StructSolver s, fpc_StructMatrix A, fpc_StructVector bv, fpc_StructVector xv) {
   GIMPL(s);
   FPC_CHECK_ARG(A && bv && xv && G->A == A && G->N == A->grid.N, "call Setup with this matrix first");
@@ -717,14 +674,6 @@ int fpc_StructGMRESSolve(fpc_StructSolver s, fpc_StructMatrix A, fpc_StructVecto
   if (G->pc_kind == PC_PFMG) {
     rc = pfmg_setup(G->pc->impl, A);  // no-op unless A changed
     if (rc) return rc;
-  }
-  const bool use_graphs = graphs_usable(G);
-  if (use_graphs) {
-    const unsigned long long key = graph_key(G);
-    if (key != G->graph_key) {
-      drop_graphs(G);
-      G->graph_key = key;
-    }
   }
   // initial residual into V[0]
   rc = true_residual(G, b, x, G->V, true);
@@ -767,7 +716,7 @@ int fpc_StructGMRESSolve(fpc_StructSolver s, fpc_StructMatrix A, fpc_StructVecto
       if (nb <= 0) nb = 1;
       for (int q = 0; q < nb; ++q) {
         prof_tag(issued + q);
-        rc = launch_step(G, issued + q, use_graphs);
+        rc = run_step(G, issued + q);
         if (rc) return rc;
       }
       issued += nb;

@@ -21,7 +21,7 @@
 //  * relaxation: Jacobi (type 0), weighted Jacobi with w = 2/(2 + c_d/sum c)
 //    (type 1), red/black Gauss-Seidel symmetric (2) or forward (3);
 //  * coarsening stops at <= 64 points; that level is solved exactly with a
-//    dense inverse held in FP64.
+//    dense inverse held in FP64 (one GEMV).
 //
 // The hierarchy is built once per matrix on the host in FP64 and uploaded as
 // x-contiguous planes; in FP32 mode every level's coefficients, smoothing,
@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "fpc_internal.hpp"
@@ -178,10 +179,15 @@ HLevel coarsen(const HLevel &F) {
   return C;
 }
 
-// Dense inverse of the coarsest operator (Gauss-Jordan with partial pivoting).
+// Dense inverse of the coarsest operator: PA = LU with partial pivoting, the
+// elimination confined to the band that pivoting can reach (bw below, 2 bw
+// above the diagonal; bw = nx*ny for a 7-point box in x-fastest order), then
+// the unit columns solved in parallel host threads.
 std::vector<double> dense_inverse(const HLevel &L) {
-  int n = (int)L.N;
+  const int n = (int)L.N;
+  const int bw = (L.n[2] > 1) ? L.n[0] * L.n[1] : (L.n[1] > 1 ? L.n[0] : 1);
   std::vector<double> M((size_t)n * n, 0.0), Inv((size_t)n * n, 0.0);
+  std::vector<int> piv((size_t)n, 0);
   int p[3];
   for (p[2] = 0; p[2] < L.n[2]; ++p[2])
     for (p[1] = 0; p[1] < L.n[1]; ++p[1])
@@ -196,31 +202,52 @@ std::vector<double> dense_inverse(const HLevel &L) {
           M[(size_t)r * n + lin(L, q[0], q[1], q[2])] += L.A(s, r);
         }
       }
-  for (int i = 0; i < n; ++i) Inv[(size_t)i * n + i] = 1.0;
   for (int c = 0; c < n; ++c) {
-    int piv = c;
-    for (int r = c + 1; r < n; ++r)
-      if (std::abs(M[(size_t)r * n + c]) > std::abs(M[(size_t)piv * n + c])) piv = r;
-    if (piv != c)
-      for (int t = 0; t < n; ++t) {
-        std::swap(M[(size_t)c * n + t], M[(size_t)piv * n + t]);
-        std::swap(Inv[(size_t)c * n + t], Inv[(size_t)piv * n + t]);
-      }
-    double dg = M[(size_t)c * n + c];
+    const int rmax = std::min(n - 1, c + bw), tmax = std::min(n - 1, c + 2 * bw);
+    int pv = c;
+    for (int r = c + 1; r <= rmax; ++r)
+      if (std::abs(M[(size_t)r * n + c]) > std::abs(M[(size_t)pv * n + c])) pv = r;
+    piv[(size_t)c] = pv;
+    if (pv != c)
+      for (int t = 0; t < n; ++t) std::swap(M[(size_t)c * n + t], M[(size_t)pv * n + t]);
+    const double dg = M[(size_t)c * n + c];
     if (dg == 0.0) continue;
-    for (int t = 0; t < n; ++t) {
-      M[(size_t)c * n + t] /= dg;
-      Inv[(size_t)c * n + t] /= dg;
-    }
-    for (int r = 0; r < n; ++r) {
-      if (r == c) continue;
-      double f = M[(size_t)r * n + c];
+    for (int r = c + 1; r <= rmax; ++r) {
+      const double f = M[(size_t)r * n + c] / dg;
+      M[(size_t)r * n + c] = f;
       if (f == 0.0) continue;
-      for (int t = 0; t < n; ++t) {
-        M[(size_t)r * n + t] -= f * M[(size_t)c * n + t];
-        Inv[(size_t)r * n + t] -= f * Inv[(size_t)c * n + t];
-      }
+      for (int t = c + 1; t <= tmax; ++t) M[(size_t)r * n + t] -= f * M[(size_t)c * n + t];
     }
+  }
+  auto solve_columns = [&](int j0, int j1) {
+    std::vector<double> x((size_t)n);
+    for (int j = j0; j < j1; ++j) {
+      std::fill(x.begin(), x.end(), 0.0);
+      x[(size_t)j] = 1.0;
+      for (int c = 0; c < n; ++c) {
+        const int pv = piv[(size_t)c];
+        if (pv != c) std::swap(x[(size_t)c], x[(size_t)pv]);
+        const double xc = x[(size_t)c];
+        if (xc == 0.0) continue;
+        for (int r = c + 1; r < n; ++r) x[(size_t)r] -= M[(size_t)r * n + c] * xc;
+      }
+      for (int r = n - 1; r >= 0; --r) {
+        double sum = x[(size_t)r];
+        for (int t = r + 1; t < n; ++t) sum -= M[(size_t)r * n + t] * x[(size_t)t];
+        const double dg = M[(size_t)r * n + r];
+        x[(size_t)r] = (dg != 0.0) ? sum / dg : 0.0;
+      }
+      for (int r = 0; r < n; ++r) Inv[(size_t)r * n + j] = x[(size_t)r];
+    }
+  };
+  const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (n < 256 || nth == 1) {
+    solve_columns(0, n);
+  } else {
+    std::vector<std::thread> pool;
+    for (int q = 0; q < nth; ++q) pool.emplace_back(solve_columns, (int)((long long)n * q / nth),
+                                                    (int)((long long)n * (q + 1) / nth));
+    for (auto &t : pool) t.join();
   }
   return Inv;
 }
@@ -372,10 +399,6 @@ __global__ void k_down_jacobi(const T *__restrict__ c, long long pitch, int nx, 
   r[t] = b[t] - s;
 }
 
-// Pre-smoothed value plus the prolonged coarse correction at fine point p.
-// x + P e at fine point (i, j, k) with x = w D^-1 b recomputed: C points (odd
-// index along D) inject the coarse value, F points take wm e(c-1) + wp e(c).
-// D is the coarsened direction (a template so no index array is spilled).
 template <typename T, int D>
 __device__ inline T corrected(const T *b, const T *dinv, const T *wm, const T *wp, const T *yc, T w, int i, int j,
                               int k, int fnx, int fny, int cnx, int cny, int ncd) {
@@ -423,7 +446,7 @@ __global__ void k_up_jacobi(const T *__restrict__ c, long long pitch, int nx, in
 
 // x = Ainv b on the coarsest level (FP64 inverse), one block.
 // ---------------------------------------------------------------- z-marching forms
-// The same arithmetic as k_down_jacobi / k_restrict / k_up_jacobi, on the
+// The same arithmetic as jacobi_resid / k_restrict / k_up_jacobi, on the
 // z-marching tiles: every row read once, z-neighbours carried in registers.
 
 // Jacobi sweep from zero and residual: r = b - A (w D^-1 b).
@@ -590,133 +613,19 @@ __global__ void __launch_bounds__(kTileX *TY)
   march_tile<T, kTileX, TY>(op, nx, ny, nz, k0, min(k0 + kz, nz));
 }
 
+// Coarsest level: x = inv b (FP64 inverse, FP64 sums), one wavefront per row.
 template <typename T>
-__global__ void k_coarse_solve(const double *__restrict__ inv, int n, const T *__restrict__ b, T *__restrict__ x,
-                               const int *skip) {
+__global__ void __launch_bounds__(256)
+    k_coarse_gemv(const double *__restrict__ inv, int n, const T *__restrict__ b, T *__restrict__ x, const int *skip) {
   if (skip && *skip) return;
-  __shared__ double bs[kCoarseMax];
-  for (int t = threadIdx.x; t < n; t += blockDim.x) bs[t] = (double)b[t];
-  __syncthreads();
-  for (int r = threadIdx.x; r < n; r += blockDim.x) {
-    double s = 0.0;
-    for (int q = 0; q < n; ++q) s += inv[(size_t)r * n + q] * bs[q];
-    x[r] = (T)s;
-  }
-}
-
-// ---------------------------------------------------------------- coarse tail
-// Every level below a few thousand points costs a full kernel launch per
-// stage while its work takes well under a microsecond.  One workgroup of 1024
-// threads runs the whole remaining V-cycle instead: per level the Jacobi sweep
-// from zero with its residual, the restriction, then the dense coarsest solve,
-// then per level prolongation + Jacobi on the way up, with a workgroup barrier
-// between stages.  The arithmetic is that of k_down_jacobi, k_restrict,
-// k_coarse_solve and k_up_jacobi.
-constexpr int kTailMaxLevels = 20;
-constexpr int kTailThreads = 1024;
-
-template <typename T>
-struct TailLevel {
-  const T *c;
-  const T *dinv, *wm, *wp;
-  T *b, *y, *r;
-  long long pitch;
-  int nx, ny, nz, cdir;
-  T w;
-};
-
-template <typename T>
-struct TailArgs {
-  TailLevel<T> lv[kTailMaxLevels];
-  int n;  // levels in the tail, the coarsest included
-};
-
-template <typename T>
-__device__ inline void tail_down(const TailLevel<T> &L) {
-  const long long pl = (long long)L.nx * L.ny, N = pl * L.nz;
-  const T w = L.w;
-  for (long long t = threadIdx.x; t < N; t += kTailThreads) {
-    const int k = (int)(t / pl), rem = (int)(t - (long long)k * pl), j = rem / L.nx, i = rem - j * L.nx;
-    const T *c = L.c, *b = L.b, *dinv = L.dinv;
-    const long long pitch = L.pitch;
-    T s = c[t] * (w * b[t] * dinv[t]);
-    if (i > 0) s += c[pitch + t] * (w * b[t - 1] * dinv[t - 1]);
-    if (i < L.nx - 1) s += c[2 * pitch + t] * (w * b[t + 1] * dinv[t + 1]);
-    if (j > 0) s += c[3 * pitch + t] * (w * b[t - L.nx] * dinv[t - L.nx]);
-    if (j < L.ny - 1) s += c[4 * pitch + t] * (w * b[t + L.nx] * dinv[t + L.nx]);
-    if (k > 0) s += c[5 * pitch + t] * (w * b[t - pl] * dinv[t - pl]);
-    if (k < L.nz - 1) s += c[6 * pitch + t] * (w * b[t + pl] * dinv[t + pl]);
-    L.r[t] = b[t] - s;
-  }
-}
-
-template <typename T>
-__device__ inline void tail_restrict(const TailLevel<T> &F, const TailLevel<T> &C) {
-  const long long cpl = (long long)C.nx * C.ny, NC = cpl * C.nz;
-  const int d = F.cdir;
-  const long long stride = (d == 0) ? 1 : (d == 1 ? (long long)F.nx : (long long)F.nx * F.ny);
-  const int fnd = (d == 0) ? F.nx : (d == 1 ? F.ny : F.nz);
-  for (long long t = threadIdx.x; t < NC; t += kTailThreads) {
-    const int k = (int)(t / cpl), rem = (int)(t - (long long)k * cpl), j = rem / C.nx, i = rem - j * C.nx;
-    const int fi = (d == 0) ? 2 * i + 1 : i, fj = (d == 1) ? 2 * j + 1 : j, fk = (d == 2) ? 2 * k + 1 : k;
-    const int pd = (d == 0) ? fi : (d == 1 ? fj : fk);
-    const long long f0 = ((long long)fk * F.ny + fj) * F.nx + fi;
-    T v = F.r[f0] + F.wp[f0 - stride] * F.r[f0 - stride];
-    if (pd + 1 < fnd) v += F.wm[f0 + stride] * F.r[f0 + stride];
-    C.b[t] = v;
-  }
-}
-
-template <typename T, int D>
-__device__ inline void tail_up_d(const TailLevel<T> &F, const TailLevel<T> &C) {
-  const long long pl = (long long)F.nx * F.ny, N = pl * F.nz;
-  const int nx = F.nx, ny = F.ny, nz = F.nz;
-  const int ncd = (D == 0) ? C.nx : (D == 1 ? C.ny : C.nz);
-  const T *c = F.c, *b = F.b, *dinv = F.dinv, *wm = F.wm, *wp = F.wp, *yc = C.y;
-  const long long pitch = F.pitch;
-  const T w = F.w;
-  for (long long t = threadIdx.x; t < N; t += kTailThreads) {
-    const int k = (int)(t / pl), rem = (int)(t - (long long)k * pl), j = rem / nx, i = rem - j * nx;
-#define FPC_CORR(ii, jj, kk) corrected<T, D>(b, dinv, wm, wp, yc, w, ii, jj, kk, nx, ny, C.nx, C.ny, ncd)
-    T xc = FPC_CORR(i, j, k);
-    T s = c[t] * xc;
-    if (i > 0) s += c[pitch + t] * FPC_CORR(i - 1, j, k);
-    if (i < nx - 1) s += c[2 * pitch + t] * FPC_CORR(i + 1, j, k);
-    if (j > 0) s += c[3 * pitch + t] * FPC_CORR(i, j - 1, k);
-    if (j < ny - 1) s += c[4 * pitch + t] * FPC_CORR(i, j + 1, k);
-    if (k > 0) s += c[5 * pitch + t] * FPC_CORR(i, j, k - 1);
-    if (k < nz - 1) s += c[6 * pitch + t] * FPC_CORR(i, j, k + 1);
-#undef FPC_CORR
-    F.y[t] = xc + w * (b[t] - s) * dinv[t];
-  }
-}
-
-template <typename T>
-__global__ void __launch_bounds__(kTailThreads)
-    k_vcycle_tail(const TailArgs<T> a, const double *__restrict__ inv, int ncoarse, const int *skip) {
-  if (skip && *skip) return;
-  for (int l = 0; l + 1 < a.n; ++l) {
-    tail_down<T>(a.lv[l]);
-    __syncthreads();
-    tail_restrict<T>(a.lv[l], a.lv[l + 1]);
-    __syncthreads();
-  }
-  {
-    const TailLevel<T> &L = a.lv[a.n - 1];
-    for (int r = threadIdx.x; r < ncoarse; r += kTailThreads) {
-      double s = 0.0;
-      for (int q = 0; q < ncoarse; ++q) s += inv[(size_t)r * ncoarse + q] * (double)L.b[q];
-      L.y[r] = (T)s;
-    }
-    __syncthreads();
-  }
-  for (int l = a.n - 2; l >= 0; --l) {
-    const int d = a.lv[l].cdir;
-    if (d == 0) tail_up_d<T, 0>(a.lv[l], a.lv[l + 1]);
-    else if (d == 1) tail_up_d<T, 1>(a.lv[l], a.lv[l + 1]);
-    else tail_up_d<T, 2>(a.lv[l], a.lv[l + 1]);
-    __syncthreads();
-  }
+  const int lane = threadIdx.x & 63;
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= n) return;
+  const double *row = inv + (size_t)r * n;
+  double s = 0.0;
+  for (int q = lane; q < n; q += 64) s += row[q] * (double)b[q];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0) x[r] = (T)s;
 }
 
 template <typename S, typename D>
@@ -909,17 +818,7 @@ int vcycle(PfmgImpl *P, const double *in, double *out, const int *skip) {
   // red/black relaxes in place (x == y); Jacobi smooths x into y
   auto xof = [&](int l) -> T * { return rb ? yof(l) : (T *)P->L[(size_t)l].x; };
   auto w_of = [&](int l) -> T { return (T)(P->relax == 0 ? 1.0 : P->L[(size_t)l].omega); };
-  // levels lt .. nl-1 run in the single-workgroup tail (lt == nl: no tail)
-  int lt = nl;
-  const long long tail_max = tile_tuning().tail_max_points;
-  if (fused && tail_max > 0) {
-    for (int l = 1; l < nl; ++l)
-      if (P->L[(size_t)l].d.N <= tail_max) {
-        lt = std::max(l, nl - kTailMaxLevels);
-        break;
-      }
-  }
-  for (int l = 0; l < std::min(lt, nl - 1); ++l) {
+  for (int l = 0; l < nl - 1; ++l) {
     DLevel &F = P->L[(size_t)l];
     DLevel &C = P->L[(size_t)l + 1];
     dim3 g, bl;
@@ -929,29 +828,28 @@ int vcycle(PfmgImpl *P, const double *in, double *out, const int *skip) {
     T *x = xof(l);
     const double sz = (double)sizeof(T);
     if (fused) {
-      // bytes per fine point: 7 coefficients, b, 1/diag, then r written (or, when z is
-      // restricted on the fly, half a weight pair read and half a coarse b written)
+      // bytes per fine point: 7 coefficients, b, 1/diag, and r written -- or, when z is
+      // restricted on the fly, one weight at half the points and the coarse b instead of r
       const char *cls = (l == 0) ? "mg_down0" : "mg_down";
+      if (tile_worthwhile(F.d) && F.cdir == 2) {
+        TileLaunch tl = tile_launch<8>(F.d);
+        FPC_PROF_BEGIN(cls);
+        k_down_restrict_z<T, 8><<<tl.grid, tl.block, 0, st>>>(c, F.pitch, F.d.nx, F.d.ny, F.d.nz, dv, bof(l),
+                                                               (const T *)F.wm, (const T *)F.wp, (T *)C.b, C.d.nz,
+                                                               w_of(l), tl.kz, skip);
+        FPC_PROF_END(cls, ((double)F.d.N * 10.0 + (double)C.d.N) * sz);
+        FPC_LAUNCH_CHECK();
+        continue;
+      }
+      FPC_PROF_BEGIN(cls);
       if (tile_worthwhile(F.d)) {
         TileLaunch tl = tile_launch<8>(F.d);
-        if (F.cdir == 2) {
-          FPC_PROF_BEGIN(cls);
-          k_down_restrict_z<T, 8><<<tl.grid, tl.block, 0, st>>>(c, F.pitch, F.d.nx, F.d.ny, F.d.nz, dv, bof(l),
-                                                                 (const T *)F.wm, (const T *)F.wp, (T *)C.b,
-                                                                 C.d.nz, w_of(l), tl.kz, skip);
-          FPC_PROF_END(cls, ((double)F.d.N * 10.0 + (double)C.d.N) * sz);
-          FPC_LAUNCH_CHECK();
-          continue;
-        }
-        FPC_PROF_BEGIN(cls);
         k_down_tiled<T, 8><<<tl.grid, tl.block, 0, st>>>(c, F.pitch, F.d.nx, F.d.ny, F.d.nz, dv, bof(l), (T *)F.r,
                                                           w_of(l), tl.kz, skip);
-        FPC_PROF_END(cls, (double)F.d.N * 10.0 * sz);
       } else {
-        FPC_PROF_BEGIN(cls);
         k_down_jacobi<T><<<g, bl, 0, st>>>(c, F.pitch, F.d.nx, F.d.ny, F.d.nz, dv, bof(l), (T *)F.r, w_of(l), skip);
-        FPC_PROF_END(cls, (double)F.d.N * 10.0 * sz);
       }
+      FPC_PROF_END(cls, (double)F.d.N * 10.0 * sz);
       FPC_LAUNCH_CHECK();
       dim3 gc, bc;
       launch_shape(C.d, gc, bc);
@@ -986,24 +884,12 @@ int vcycle(PfmgImpl *P, const double *in, double *out, const int *skip) {
                                     C.d.nx, C.d.ny, C.d.nz, (T *)C.b, skip);
     FPC_LAUNCH_CHECK();
   }
-  if (lt < nl) {
-    TailArgs<T> ta;
-    ta.n = nl - lt;
-    for (int q = 0; q < ta.n; ++q) {
-      DLevel &D = P->L[(size_t)(lt + q)];
-      ta.lv[q] = TailLevel<T>{(const T *)D.coef, (const T *)D.dinv, (const T *)D.wm, (const T *)D.wp, (T *)D.b,
-                              (T *)D.y, (T *)D.r, D.pitch, D.d.nx, D.d.ny, D.d.nz, D.cdir, w_of(lt + q)};
-    }
-    FPC_PROF_BEGIN("mg_tail");
-    k_vcycle_tail<T><<<1, kTailThreads, 0, st>>>(ta, P->inv, P->ncoarse, skip);
-    FPC_PROF_END("mg_tail", 0.0);
-    FPC_LAUNCH_CHECK();
-  } else {
+  {
     int lc = nl - 1;
-    k_coarse_solve<T><<<1, 64, 0, st>>>(P->inv, P->ncoarse, bof(lc), yof(lc), skip);
+    k_coarse_gemv<T><<<(unsigned)((P->ncoarse + 3) / 4), 256, 0, st>>>(P->inv, P->ncoarse, bof(lc), yof(lc), skip);
     FPC_LAUNCH_CHECK();
   }
-  for (int l = std::min(lt, nl - 1) - 1; l >= 0; --l) {
+  for (int l = nl - 2; l >= 0; --l) {
     DLevel &F = P->L[(size_t)l];
     DLevel &C = P->L[(size_t)l + 1];
     dim3 g, bl;
@@ -1082,27 +968,6 @@ int pfmg_setup(void *pf, fpc_StructMatrix_s *A) {
   PfmgImpl *P = (PfmgImpl *)pf;
   if (P->A == A && P->version == A->version && P->built_bits == P->bits && !P->L.empty()) return 0;
   return build(P, A);
-}
-
-unsigned long long pfmg_signature(void *pf) {
-  PfmgImpl *P = (PfmgImpl *)pf;
-  unsigned long long h = 1469598103934665603ULL;
-  auto mix = [&](unsigned long long v) { h = (h ^ v) * 1099511628211ULL; };
-  mix(P->version);
-  mix((unsigned long long)P->built_bits);
-  mix((unsigned long long)P->relax);
-  mix((unsigned long long)P->npre);
-  mix((unsigned long long)P->npost);
-  mix((unsigned long long)P->maxit);
-  mix((unsigned long long)P->L.size());
-  mix((unsigned long long)(uintptr_t)(P->L.empty() ? nullptr : P->L[0].coef));
-  mix((unsigned long long)(uintptr_t)P->inv);
-  return h;
-}
-
-bool pfmg_graph_safe(void *pf) {
-  PfmgImpl *P = (PfmgImpl *)pf;
-  return P->maxit == 1 && !P->L.empty();
 }
 
 int pfmg_apply(void *pf, const double *b, double *x, const int *skip) {

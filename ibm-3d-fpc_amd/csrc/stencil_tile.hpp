@@ -123,14 +123,11 @@ struct TileLaunch {
   int kz;
 };
 
-// Launch tuning (fpc_SetTuning; environment defaults FPC_TILE_MIN_POINTS,
-// FPC_STENCIL_KZ, FPC_TAIL_MAX_POINTS, FPC_GRAPHS).  0 for kz = automatic.
+// Launch tuning (fpc_SetTuning; environment defaults FPC_TILE_MIN_POINTS and
+// FPC_STENCIL_KZ).  0 for kz = automatic.
 struct TileTuning {
   long long min_points = -1;
   int kz = -1;
-  long long tail_max_points = -1;  // MG levels this small run in one workgroup (0: never)
-  int graphs = -1;                 // capture GMRES Arnoldi steps as HIP graphs (0: never)
-  unsigned long long generation = 0;  // bumped by every change (captured graphs depend on it)
 };
 inline TileTuning &tile_tuning() {
   static TileTuning t;
@@ -142,14 +139,8 @@ inline TileTuning &tile_tuning() {
     const char *e = std::getenv("FPC_STENCIL_KZ");
     t.kz = e ? std::max(0, std::atoi(e)) : 0;
   }
-  if (t.tail_max_points < 0) {
-    const char *e = std::getenv("FPC_TAIL_MAX_POINTS");
-    t.tail_max_points = e ? std::max(0LL, std::atoll(e)) : 12288;
-  }
-  if (t.graphs < 0) {
-    const char *e = std::getenv("FPC_GRAPHS");
-    t.graphs = e ? (std::atoi(e) != 0) : 1;
-  }
+
+
   return t;
 }
 

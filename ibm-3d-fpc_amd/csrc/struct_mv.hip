@@ -97,8 +97,6 @@ void prof_drop_skipped(size_t from, int ran) {
     if (g_prof_recs[q].tag >= ran) g_prof_recs[q].dropped = true;
 }
 
-bool prof_any() { return g_prof; }
-
 bool prof_on(const char *cls) {
   if (!g_prof) return false;
   if (g_prof_filter == "*") return true;
@@ -768,18 +766,12 @@ int fpc_SetTuning(const char *key, long long value) {
   if (k == "tile_min_points") {
     FPC_CHECK_ARG(value >= 0, "tile_min_points must be >= 0");
     t.min_points = value;
-  } else if (k == "tail_max_points") {
-    FPC_CHECK_ARG(value >= 0, "tail_max_points must be >= 0");
-    t.tail_max_points = value;
   } else if (k == "stencil_kz") {
     FPC_CHECK_ARG(value >= 0 && value <= 1024, "stencil_kz must be in [0, 1024]");
     t.kz = (int)value;
-  } else if (k == "graphs") {
-    t.graphs = value != 0;
   } else {
     FPC_CHECK_ARG(false, "unknown tuning key");
   }
-  t.generation += 1;
   return 0;
 }
 

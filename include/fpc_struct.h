@@ -196,13 +196,7 @@ int fpc_DeviceSynchronize(void);
  * "tile_min_points": boxes with at least this many points (and nx >= 32,
  * ny >= 8) use the z-marching stencil kernels (default 1048576, or the
  * FPC_TILE_MIN_POINTS environment variable); "stencil_kz": z-planes per
- * workgroup of those kernels, 0 = automatic (FPC_STENCIL_KZ);
- * "tail_max_points": PFMG levels (below the finest) with at most this many
- * points run their whole V-cycle part in one workgroup, 0 = never (default
- * 12288, FPC_TAIL_MAX_POINTS); "graphs": 1 = GMRES captures each Arnoldi step
- * (preconditioner included) once as a HIP graph and replays it, 0 = plain
- * launches (default 1, FPC_GRAPHS; steps with a callback preconditioner, a
- * multi-cycle PFMG or kernel timing enabled always launch plainly). */
+ * workgroup of those kernels, 0 = automatic (FPC_STENCIL_KZ). */
 int fpc_SetTuning(const char *key, long long value);
 
 /* ---- kernel timing (no reference counterpart) ----

@@ -691,7 +691,11 @@ void box_matvec(const Box &B, const double *x, double *y) {
 //    Jacobi (type 0), red/black Gauss-Seidel (types 2: symmetric, 3: forward);
 //    one pre and one post sweep;
 //  * coarsening stops at <= 64 points or when no direction has 2+ points; the
-//    coarsest system is solved exactly (LU with partial pivoting).
+//    coarsest system is solved exactly (LU with partial pivoting).  (Stopping at
+//    ~1000 points instead, with the same exact solve, needs 93 rather than 20
+//    GMRES iterations on the 48x16x12 pressure box: the non-Galerkin operator of
+//    such a level misrepresents its smoothest modes, which an exact solve
+//    amplifies and further smoothing levels damp.)
 const int MG_COARSE_MAX = 64;
 
 struct MGLevel {
@@ -838,21 +842,27 @@ Box mg_coarsen(const MGLevel &Lv) {
   return C;
 }
 
-void lu_factor(int n, std::vector<double> &A, std::vector<int> &piv) {
+// Dense LU with partial pivoting, PA = LU, of a matrix whose nonzeros lie within
+// bw of the diagonal (7-point box in x-fastest order: bw = nx*ny, or nx, or 1).
+// Below the pivot only rows c+1..c+bw can be nonzero in column c, and row swaps
+// and updates reach at most column c+2bw, so the loops stop there: the result is
+// the same as the full dense elimination (the skipped terms are exact zeros).
+void lu_factor(int n, int bw, std::vector<double> &A, std::vector<int> &piv) {
   piv.assign((size_t)n, 0);
   for (int c = 0; c < n; ++c) {
+    const int rmax = std::min(n - 1, c + bw), tmax = std::min(n - 1, c + 2 * bw);
     int p = c;
-    for (int r = c + 1; r < n; ++r)
+    for (int r = c + 1; r <= rmax; ++r)
       if (std::abs(A[(size_t)r * n + c]) > std::abs(A[(size_t)p * n + c])) p = r;
     piv[(size_t)c] = p;
     if (p != c)
       for (int t = 0; t < n; ++t) std::swap(A[(size_t)c * n + t], A[(size_t)p * n + t]);
     double d = A[(size_t)c * n + c];
     if (d == 0.0) continue;
-    for (int r = c + 1; r < n; ++r) {
+    for (int r = c + 1; r <= rmax; ++r) {
       double f = A[(size_t)r * n + c] / d;
       A[(size_t)r * n + c] = f;
-      for (int t = c + 1; t < n; ++t) A[(size_t)r * n + t] -= f * A[(size_t)c * n + t];
+      for (int t = c + 1; t <= tmax; ++t) A[(size_t)r * n + t] -= f * A[(size_t)c * n + t];
     }
   }
 }
@@ -901,7 +911,8 @@ void mg_setup(MG &M, const Box &A0, int relax_type, int max_levels) {
         for (int s = 1; s < 7; ++s)
           if (nbr(B, i, j, k, s, q)) M.lu[r * n + q] += B.a[s][r];
       }
-  lu_factor(n, M.lu, M.piv);
+  const int bw = (B.nz > 1) ? B.nx * B.ny : (B.ny > 1 ? B.nx : 1);
+  lu_factor(n, bw, M.lu, M.piv);
 }
 
 // One relaxation sweep on level l.  zero: x is taken as 0 on entry.

@@ -70,3 +70,67 @@ def test_product_module_has_no_oracle_dependency():
             if fn.endswith((".py", ".hip", ".cpp", ".hpp", ".h")):
                 txt = open(os.path.join(d, fn)).read()
                 assert "oracle" not in txt.lower(), fn
+
+
+REFERENCE_STYLE_CALLER = r"""
+// The solver block of a HYPRE Struct caller (the shape of main.cu:151-221,
+// 284-300 and 391-404), compiled against the compat header.
+#include <cstdio>
+#include <vector>
+#include "fpc_hypre_compat.h"
+#include "fpc_dense.h"
+#include "fpc_sim.h"
+
+int main() {
+  const int nx = 8, ny = 4, nz = 4;
+  HYPRE_Initialize();
+  HYPRE_SetMemoryLocation(HYPRE_MEMORY_DEVICE);
+  HYPRE_SetExecutionPolicy(HYPRE_EXEC_DEVICE);
+  std::vector<double> vals(7 * nx * ny * nz, 0.0);
+  for (size_t q = 0; q < vals.size(); q += 7) vals[q] = 1.0;
+  HYPRE_Complex *vals_d = nullptr;
+  CopyVecToManagedC(&vals_d, vals);
+  HYPRE_StructGrid g = CreateGrid3D(nx, ny, nz);
+  HYPRE_StructStencil st = CreateStencil3D();
+  HYPRE_StructMatrix A = CreateMatrix3D(g, st, nx, ny, nz, vals_d);
+  HYPRE_StructVector b = CreateVector3D(g), x = CreateVector3D(g);
+  HYPRE_StructSolver s, pc;
+  HYPRE_StructGMRESCreate(MPI_COMM_WORLD, &s);
+  HYPRE_StructGMRESSetTol(s, 1e-8);
+  HYPRE_StructGMRESSetKDim(s, 30);
+  HYPRE_StructPFMGCreate(MPI_COMM_WORLD, &pc);
+  HYPRE_StructPFMGSetMaxIter(pc, 1);
+  HYPRE_StructPFMGSetZeroGuess(pc);
+  HYPRE_StructGMRESSetPrecond(s, HYPRE_PtrToStructSolverFcn(HYPRE_StructPFMGSolve),
+                              HYPRE_PtrToStructSolverFcn(HYPRE_StructPFMGSetup), pc);
+  HYPRE_StructGMRESSetup(s, A, b, x);
+  int rc = HYPRE_StructGMRESSolve(s, A, b, x);
+  if (rc == HYPRE_ERROR_CONV) HYPRE_ClearError(HYPRE_ERROR_CONV);
+  int its = 0;
+  HYPRE_StructGMRESGetNumIterations(s, &its);
+  fpc_params par;
+  fpc_default_params(&par);
+  std::printf("%d %d %d\n", rc, its, par.nx);
+  HYPRE_StructGMRESDestroy(s);
+  HYPRE_StructPFMGDestroy(pc);
+  HYPRE_StructMatrixDestroy(A);
+  HYPRE_Finalize();
+  return 0;
+}
+"""
+
+
+def test_reference_style_caller_compiles_and_links(tmp_path):
+    # the drop-in boundary: a caller written against HYPRE Struct names builds
+    # against include/ and links against libfpc.so (not run: no GPU here)
+    F.lib()
+    src = tmp_path / "caller.cpp"
+    src.write_text(REFERENCE_STYLE_CALLER)
+    exe = tmp_path / "caller"
+    libdir = os.path.dirname(F.LIB_PATH)
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    cmd = ["g++", "-std=c++17", "-Wall", "-Werror", "-I", inc, str(src), "-o", str(exe), "-L", libdir, "-lfpc",
+           "-Wl,-rpath-link,/opt/rocm/lib", "-Wl,-rpath," + libdir]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert exe.exists()

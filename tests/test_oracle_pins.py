@@ -343,6 +343,8 @@ def _two_level_numpy(nx, ny, nz, vals, d, omega, b):
     return x + omega * (b - A @ x) / Dg
 
 
+# fine levels of 64 < N <= 128 points: exactly one coarsening, then the coarsest
+# level (<= 64 points) is solved exactly
 @pytest.mark.parametrize("dims,h", [((8, 4, 4), (1, 1, 1)), ((4, 8, 4), (1, 0.8, 1.2)), ((4, 4, 8), (1.3, 1, 0.9))])
 def test_two_level_cycle_matches_numpy_definition(dims, h):
     nx, ny, nz = dims
@@ -354,7 +356,7 @@ def test_two_level_cycle_matches_numpy_definition(dims, h):
     A = vals.reshape(-1, 7)
     c = [np.abs(A[:, 1 + 2 * e]).sum() + np.abs(A[:, 2 + 2 * e]).sum() for e in range(3)]
     assert d == int(np.argmax(c))
-    assert abs(omega - 2.0 / (2.0 + c[d] / sum(c))) < 1e-15
+    assert abs(omega - 2.0 / (2.0 + c[d] / sum(c))) < 1e-14  # sums in another order
     b = np.random.default_rng(4).standard_normal(nx * ny * nz)
     want = _two_level_numpy(nx, ny, nz, vals, d, omega, b)
     got = O.precond_apply(nx, ny, nz, vals, b, precond="pfmg", relax_type=1)
