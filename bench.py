@@ -132,6 +132,30 @@ def cpu_baseline():
                       f"({dt:.1f} s), scaled by cells {512 * 96 * nz}/{full}"}
 
 
+def run_ensemble(F, torch, nthreads, n=64, steps=50):
+    """Aggregate cell-updates/s of n seeded uneven channels over `steps` steps: the
+    instances one after another (the plain path looped) and with `nthreads` host
+    workers, one HIP stream per instance.  One untimed step first (solver setup)."""
+    pars = F.ensemble_workload(n=n, seed=0)
+    cells = sum(p.nx * p.ny * p.nz for p in pars)
+    out = {"metric": "cell-updates/s, %d-channel ensemble, %d steps each" % (n, steps), "unit": "cell-updates/s",
+           "instances": n, "cells_total": cells, "steps": steps, "dtype": "f64",
+           "data": "synthetic (seeded geometry, Re 10-100, inlet profile + noise)"}
+    for tag, nth in (("looped", 1), ("concurrent", nthreads)):
+        ens = F.Ensemble(pars, nthreads=nth)
+        ens.step(1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ens.step(steps)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out[tag] = {"value": cells * steps / dt, "seconds": dt, "threads": nth}
+        ens.close()
+    out["value"] = out["concurrent"]["value"]
+    out["speedup_vs_looped"] = out["concurrent"]["value"] / out["looped"]["value"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,6 +164,9 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-roofline", action="store_true", help="no kernel timing at all (A/B timing runs)")
+    ap.add_argument("--ensemble", action="store_true",
+                    help="64 uneven channels, 50 steps: cell-updates/s looped vs concurrent (no headline line)")
+    ap.add_argument("--ensemble-threads", type=int, default=8)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 2 GiB device-to-device copy 5 times (scale for HBM byte counters)")
     args = ap.parse_args()
@@ -160,6 +187,9 @@ def main():
         return
     import fpc_amd as F
     F.lib()
+    if args.ensemble:
+        print(json.dumps(run_ensemble(F, torch, args.ensemble_threads)))
+        return
     sim64, t64, its64, dom, rec, classes = run_sim(F, torch, WORKLOAD, args.warmup, args.steps,
                                                    not args.no_roofline)
     res = {

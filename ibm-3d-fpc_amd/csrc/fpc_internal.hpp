@@ -23,6 +23,21 @@ void set_error(int bits, const char *fmt, ...);
 int error_bits();
 hipStream_t stream();
 void ensure_init();
+// Frees the calling thread's scratch buffers (worker threads, before they exit).
+void release_thread_state();
+
+// Routes this thread's library calls to stream s for the scope's lifetime
+// (each simulation handle runs on a stream of its own).
+class StreamScope {
+ public:
+  explicit StreamScope(hipStream_t s);
+  ~StreamScope();
+  StreamScope(const StreamScope &) = delete;
+  StreamScope &operator=(const StreamScope &) = delete;
+
+ private:
+  hipStream_t prev_;
+};
 
 #define FPC_HIP(call)                                                                                  \
   do {                                                                                                 \

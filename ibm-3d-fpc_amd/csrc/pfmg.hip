@@ -694,14 +694,15 @@ int upload(void **dst, const std::vector<double> &src, size_t count, size_t n_al
   std::vector<T> tmp(n_alloc, (T)0);
   for (size_t q = 0; q < count; ++q) tmp[q] = (T)src[q];
   FPC_HIP(hipMalloc(dst, n_alloc * sizeof(T)));
-  FPC_HIP(hipMemcpy(*dst, tmp.data(), n_alloc * sizeof(T), hipMemcpyHostToDevice));
+  FPC_HIP(hipMemcpyAsync(*dst, tmp.data(), n_alloc * sizeof(T), hipMemcpyHostToDevice, stream()));
+  FPC_HIP(hipStreamSynchronize(stream()));  // tmp is released on return
   return 0;
 }
 
 template <typename T>
 int alloc_t(void **dst, long long n) {
   FPC_HIP(hipMalloc(dst, (size_t)std::max(1LL, n) * sizeof(T)));
-  FPC_HIP(hipMemset(*dst, 0, (size_t)std::max(1LL, n) * sizeof(T)));
+  FPC_HIP(hipMemsetAsync(*dst, 0, (size_t)std::max(1LL, n) * sizeof(T), stream()));
   return 0;
 }
 
@@ -1065,7 +1066,8 @@ int fpc_StructPFMGSolve(fpc_StructSolver s, fpc_StructMatrix A, fpc_StructVector
   if (P->tol > 0.0) {
     if ((rc = dot_partials(b->data, b->data, n, part)) || (rc = reduce_partials(part, kRedBlocks, part + kRedBlocks)))
       return rc;
-    FPC_HIP(hipMemcpy(&bnorm2, part + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost));
+    FPC_HIP(hipMemcpyAsync(&bnorm2, part + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, stream()));
+    FPC_HIP(hipStreamSynchronize(stream()));
   }
   P->its = 0;
   for (int it = 0; it < P->maxit; ++it) {
@@ -1084,7 +1086,8 @@ int fpc_StructPFMGSolve(fpc_StructSolver s, fpc_StructMatrix A, fpc_StructVector
           (rc = reduce_partials(part, kRedBlocks, part + kRedBlocks)))
         return rc;
       double r2 = 0.0;
-      FPC_HIP(hipMemcpy(&r2, part + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost));
+      FPC_HIP(hipMemcpyAsync(&r2, part + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, stream()));
+      FPC_HIP(hipStreamSynchronize(stream()));
       if (std::sqrt(r2) <= P->tol * std::sqrt(bnorm2)) break;
     }
   }
@@ -1126,12 +1129,14 @@ int fpc_StructPFMGGetLevelMatrix(fpc_StructSolver s, int level, double *values) 
   for (int q = 0; q < kSlots; ++q) {
     if (P->built_bits == 32) {
       std::vector<float> t((size_t)N);
-      FPC_HIP(hipMemcpy(t.data(), (const float *)l.coef + (long long)q * l.pitch, (size_t)N * sizeof(float),
-                        hipMemcpyDeviceToHost));
+      FPC_HIP(hipMemcpyAsync(t.data(), (const float *)l.coef + (long long)q * l.pitch, (size_t)N * sizeof(float),
+                             hipMemcpyDeviceToHost, stream()));
+      FPC_HIP(hipStreamSynchronize(stream()));
       for (long long r = 0; r < N; ++r) planes[(size_t)q * (size_t)N + (size_t)r] = t[(size_t)r];
     } else {
-      FPC_HIP(hipMemcpy(planes.data() + (size_t)q * (size_t)N, (const double *)l.coef + (long long)q * l.pitch,
-                        (size_t)N * sizeof(double), hipMemcpyDeviceToHost));
+      FPC_HIP(hipMemcpyAsync(planes.data() + (size_t)q * (size_t)N, (const double *)l.coef + (long long)q * l.pitch,
+                             (size_t)N * sizeof(double), hipMemcpyDeviceToHost, stream()));
+      FPC_HIP(hipStreamSynchronize(stream()));
     }
   }
   for (long long r = 0; r < N; ++r)

@@ -38,7 +38,10 @@ __global__ void k_scale_inlet(const double *uin, double *uin_step, double ramp, 
 template <class T>
 int upload(T **d, const std::vector<T> &h) {
   FPC_HIP(hipMalloc((void **)d, std::max<size_t>(1, h.size()) * sizeof(T)));
-  if (!h.empty()) FPC_HIP(hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  if (!h.empty()) {
+    FPC_HIP(hipMemcpyAsync(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, fpc::stream()));
+    FPC_HIP(hipStreamSynchronize(fpc::stream()));
+  }
   return 0;
 }
 
@@ -49,6 +52,7 @@ using namespace fpc;
 
 struct fpc_sim_s {
   fpc_params par{};
+  hipStream_t stream = nullptr;  // every call on this handle runs on its own stream
   host::SetupData S;
   // device operator data
   unsigned char *au = nullptr, *av = nullptr, *aw = nullptr, *fp = nullptr, *aqu = nullptr, *aqv = nullptr, *aqw = nullptr;
@@ -250,6 +254,12 @@ int fpc_sim_create(const fpc_params *par, fpc_sim *out) {
   ensure_init();
   fpc_sim M = new fpc_sim_s();
   M->par = *par;
+  if (hipStreamCreate(&M->stream) != hipSuccess) {
+    delete M;
+    set_error(FPC_ERROR_GENERIC, "fpc_sim_create: hipStreamCreate failed");
+    return FPC_ERROR_GENERIC;
+  }
+  fpc::StreamScope scope(M->stream);
   M->S.build(*par);
   const host::SetupData &S = M->S;
   int nx = par->nx, ny = par->ny, nz = par->nz, rc = 0;
@@ -353,11 +363,13 @@ int fpc_sim_create(const fpc_params *par, fpc_sim *out) {
 
 int fpc_sim_phase(fpc_sim M, int phase) {
   FPC_CHECK_ARG(M, "null sim");
+  fpc::StreamScope scope(M->stream);
   return sim_phase(M, phase);
 }
 
 int fpc_sim_step(fpc_sim M, int nsteps) {
   FPC_CHECK_ARG(M, "null sim");
+  fpc::StreamScope scope(M->stream);
   for (int n = 0; n < nsteps; ++n)
     for (int ph = 0; ph < 8; ++ph) {
       int rc = sim_phase(M, ph);
@@ -378,6 +390,7 @@ long long fpc_sim_len(fpc_sim M, const char *name) {
 
 int fpc_sim_copy(fpc_sim M, const char *name, void *dst, long long nbytes) {
   FPC_CHECK_ARG(M && name && dst, "null argument");
+  fpc::StreamScope scope(M->stream);
   long long n = -1;
   double *d = field_ptr(M, name, &n);
   if (d) {
@@ -396,6 +409,7 @@ int fpc_sim_copy(fpc_sim M, const char *name, void *dst, long long nbytes) {
 
 int fpc_sim_set(fpc_sim M, const char *name, const void *src, long long nbytes) {
   FPC_CHECK_ARG(M && name && src, "null argument");
+  fpc::StreamScope scope(M->stream);
   long long n = -1;
   double *d = field_ptr(M, name, &n);
   FPC_CHECK_ARG(d, "unknown field");
@@ -413,6 +427,8 @@ int fpc_sim_device_ptr(fpc_sim M, const char *name, void **ptr) {
   *ptr = d;
   return 0;
 }
+
+void *fpc_sim_stream(fpc_sim M) { return M ? (void *)M->stream : nullptr; }
 
 double fpc_sim_scalar(fpc_sim M, const char *name) {
   if (!M) return NAN;
@@ -453,6 +469,8 @@ double fpc_sim_timer(fpc_sim M, const char *name) {
 
 int fpc_sim_destroy(fpc_sim M) {
   if (!M) return 0;
+  fpc::StreamScope scope(M->stream);
+  (void)hipStreamSynchronize(M->stream);
   fpc_StructGMRESDestroy(M->su); fpc_StructJacobiDestroy(M->pu);
   fpc_StructGMRESDestroy(M->sv); fpc_StructJacobiDestroy(M->pv);
   fpc_StructGMRESDestroy(M->sw); fpc_StructJacobiDestroy(M->pw);
@@ -467,6 +485,7 @@ int fpc_sim_destroy(fpc_sim M) {
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (M->hdiag) (void)hipHostFree(M->hdiag);
+  if (M->stream) (void)hipStreamDestroy(M->stream);
   delete M;
   return 0;
 }

@@ -130,17 +130,14 @@ struct TileTuning {
   int kz = -1;
 };
 inline TileTuning &tile_tuning() {
-  static TileTuning t;
-  if (t.min_points < 0) {
+  static TileTuning t = [] {
+    TileTuning x;
     const char *e = std::getenv("FPC_TILE_MIN_POINTS");
-    t.min_points = e ? std::max(0LL, std::atoll(e)) : (1LL << 20);
-  }
-  if (t.kz < 0) {
-    const char *e = std::getenv("FPC_STENCIL_KZ");
-    t.kz = e ? std::max(0, std::atoi(e)) : 0;
-  }
-
-
+    x.min_points = e ? std::max(0LL, std::atoll(e)) : (1LL << 20);
+    e = std::getenv("FPC_STENCIL_KZ");
+    x.kz = e ? std::max(0, std::atoi(e)) : 0;
+    return x;
+  }();
   return t;
 }
 

@@ -10,33 +10,50 @@
 #include <cstring>
 #include <mutex>
 
+#include <atomic>
+#include <mutex>
+
 #include "fpc_internal.hpp"
 #include "stencil_tile.hpp"
 
 namespace fpc {
 
 namespace {
-int g_error = 0;
-char g_msg[512] = "";
+// Library-wide default stream (fpc_SetStream), created on first use.
+std::mutex g_mu;
+std::atomic<bool> g_init{false};
 hipStream_t g_stream = nullptr;
 bool g_own_stream = false;
-bool g_init = false;
-std::vector<double *> g_scratch(8, nullptr);
-std::vector<size_t> g_scratch_n(8, 0);
+
+// Per calling thread: error bits and message (HYPRE keeps one global error
+// state; here each host thread has its own), the stream a handle scoped for
+// its calls, and scratch buffers, so handles can be driven from several
+// threads at once.
+constexpr int kScratchSlots = 8;
+struct ThreadState {
+  int error = 0;
+  char msg[512] = "";
+  hipStream_t scoped = nullptr;
+  double *scratch[kScratchSlots] = {};
+  size_t scratch_n[kScratchSlots] = {};
+};
+thread_local ThreadState t_state;
 }  // namespace
 
 void set_error(int bits, const char *fmt, ...) {
-  g_error |= bits;
+  t_state.error |= bits;
   va_list ap;
   va_start(ap, fmt);
-  vsnprintf(g_msg, sizeof(g_msg), fmt, ap);
+  vsnprintf(t_state.msg, sizeof(t_state.msg), fmt, ap);
   va_end(ap);
 }
 
-int error_bits() { return g_error; }
+int error_bits() { return t_state.error; }
 
 void ensure_init() {
-  if (g_init) return;
+  if (g_init.load(std::memory_order_acquire)) return;
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (g_init.load(std::memory_order_relaxed)) return;
   if (!g_stream) {
     if (hipStreamCreate(&g_stream) != hipSuccess) {
       set_error(FPC_ERROR_GENERIC, "hipStreamCreate failed");
@@ -45,29 +62,46 @@ void ensure_init() {
       g_own_stream = true;
     }
   }
-  g_init = true;
+  g_init.store(true, std::memory_order_release);
 }
 
 hipStream_t stream() {
+  if (t_state.scoped) return t_state.scoped;
   ensure_init();
   return g_stream;
 }
 
+StreamScope::StreamScope(hipStream_t s) : prev_(t_state.scoped) { t_state.scoped = s; }
+StreamScope::~StreamScope() { t_state.scoped = prev_; }
+
 double *scratch_doubles(size_t n, int which) {
-  if (which < 0 || which >= (int)g_scratch.size()) return nullptr;
-  if (g_scratch_n[(size_t)which] < n) {
-    if (g_scratch[(size_t)which]) (void)hipFree(g_scratch[(size_t)which]);
-    g_scratch[(size_t)which] = nullptr;
-    g_scratch_n[(size_t)which] = 0;
+  if (which < 0 || which >= kScratchSlots) return nullptr;
+  ThreadState &ts = t_state;
+  if (ts.scratch_n[which] < n) {
+    if (ts.scratch[which]) {
+      (void)hipStreamSynchronize(stream());  // the old buffer may still be in use
+      (void)hipFree(ts.scratch[which]);
+    }
+    ts.scratch[which] = nullptr;
+    ts.scratch_n[which] = 0;
     void *p = nullptr;
     if (hipMalloc(&p, n * sizeof(double)) != hipSuccess) {
       set_error(FPC_ERROR_MEMORY, "scratch allocation of %zu doubles failed", n);
       return nullptr;
     }
-    g_scratch[(size_t)which] = (double *)p;
-    g_scratch_n[(size_t)which] = n;
+    ts.scratch[which] = (double *)p;
+    ts.scratch_n[which] = n;
   }
-  return g_scratch[(size_t)which];
+  return ts.scratch[which];
+}
+
+void release_thread_state() {
+  ThreadState &ts = t_state;
+  for (int w = 0; w < kScratchSlots; ++w) {
+    if (ts.scratch[w]) (void)hipFree(ts.scratch[w]);
+    ts.scratch[w] = nullptr;
+    ts.scratch_n[w] = 0;
+  }
 }
 
 // ---------------------------------------------------------------- kernel timing
@@ -367,38 +401,34 @@ int fpc_Initialize(void) {
 }
 
 int fpc_Finalize(void) {
-  for (size_t w = 0; w < g_scratch.size(); ++w) {
-    if (g_scratch[w]) (void)hipFree(g_scratch[w]);
-    g_scratch[w] = nullptr;
-    g_scratch_n[w] = 0;
-  }
+  release_thread_state();
+  std::lock_guard<std::mutex> lk(g_mu);
   if (g_stream && g_own_stream) (void)hipStreamDestroy(g_stream);
   g_stream = nullptr;
   g_own_stream = false;
-  g_init = false;
+  g_init.store(false, std::memory_order_release);
   return 0;
 }
 
 int fpc_SetMemoryLocationDevice(void) { return 0; }
 int fpc_SetExecutionPolicyDevice(void) { return 0; }
-int fpc_GetError(void) { return g_error; }
+int fpc_GetError(void) { return t_state.error; }
 int fpc_ClearError(int bits) {
-  g_error &= ~bits;
+  t_state.error &= ~bits;
   return 0;
 }
-const char *fpc_GetErrorMessage(void) { return g_msg; }
+const char *fpc_GetErrorMessage(void) { return t_state.msg; }
 
 int fpc_SetStream(void *s) {
+  std::lock_guard<std::mutex> lk(g_mu);
   if (g_stream && g_own_stream) (void)hipStreamDestroy(g_stream);
   g_stream = (hipStream_t)s;
   g_own_stream = false;
-  if (!s) {
-    g_init = false;
-    ensure_init();
-  }
-  g_init = true;
+  // NULL: a fresh library stream is created on next use
+  g_init.store(s != nullptr, std::memory_order_release);
   return 0;
 }
+
 void *fpc_GetStream(void) { return (void *)stream(); }
 
 // ---------------------------------------------------------------- grid

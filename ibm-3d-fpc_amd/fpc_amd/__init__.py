@@ -155,6 +155,11 @@ SIGNATURES = {
     "fpc_sim_len": (_ll, [_vp, _cp]), "fpc_sim_copy": (_i, [_vp, _cp, _vp, _ll]),
     "fpc_sim_set": (_i, [_vp, _cp, _vp, _ll]), "fpc_sim_device_ptr": (_i, [_vp, _cp, _pvp]),
     "fpc_sim_scalar": (_d, [_vp, _cp]), "fpc_sim_timer": (_d, [_vp, _cp]), "fpc_sim_destroy": (_i, [_vp]),
+    "fpc_sim_stream": (_vp, [_vp]),
+    "fpc_ensemble_create": (_i, [_PP, _i, _i, _pvp]), "fpc_ensemble_step": (_i, [_vp, _i]),
+    "fpc_ensemble_size": (_i, [_vp]), "fpc_ensemble_member": (_vp, [_vp, _i]),
+    "fpc_ensemble_offsets": (_i, [_vp, _cp, ctypes.POINTER(ctypes.c_longlong)]),
+    "fpc_ensemble_pack": (_i, [_vp, _cp, _vp, _ll]), "fpc_ensemble_destroy": (_i, [_vp]),
 }
 
 
@@ -248,11 +253,13 @@ class Sim:
         h = ctypes.c_void_p()
         check(lib().fpc_sim_create(ctypes.byref(params), ctypes.byref(h)), "fpc_sim_create")
         self.h = h
+        self._owned = True
 
     def close(self):
-        if getattr(self, "h", None) and _lib is not None:
+        # ensemble members belong to their ensemble
+        if getattr(self, "h", None) and getattr(self, "_owned", True) and _lib is not None:
             _lib.fpc_sim_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         self.close()
@@ -298,6 +305,72 @@ class Sim:
 def set_tuning(key, value):
     """Launch tuning of the z-marching stencil kernels ("tile_min_points", "stencil_kz")."""
     check(lib().fpc_SetTuning(key.encode(), int(value)), f"SetTuning({key})")
+
+
+def ensemble_workload(n=64, seed=0, nx=(64, 256), nyz=(16, 48), xc=(0.4, 0.7), yc=(0.15, 0.26), re=(10.0, 100.0),
+                      **overrides):
+    """Seeded uneven channel instances: nx in [nx0, nx1], ny = nz in [nyz0, nyz1],
+    cylinder (r = 0.05) centre x in xc, y in yc, Re = (4 Um / 9) D / nu in re with
+    nu = 1e-3; dt = h_min / (8 Um); inlet profile + 1e-3 Um noise; solvers as the
+    production configuration (GMRES(30), rtol 1e-8)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for q in range(n):
+        gx = int(rng.integers(nx[0], nx[1] + 1))
+        g = int(rng.integers(nyz[0], nyz[1] + 1))
+        Re = float(rng.uniform(*re))
+        um = Re * 1e-3 / (4.0 / 9.0 * 0.1)
+        h = min(2.5 / gx, 0.41 / g)
+        kw = dict(nx=gx, ny=g, nz=g, Lx=2.5, Ly=0.41, Lz=0.41, xcirc=float(rng.uniform(*xc)),
+                  ycirc=float(rng.uniform(*yc)), Rcirc=0.05, Um=um, nu=1e-3, rho=1.0, dt=h / (8.0 * um),
+                  vel_kdim=30, p_kdim=30, vel_tol=1e-8, p_tol=1e-8, vel_maxit=250, p_maxit=250, init_profile=1,
+                  noise_amp=1e-3 * um, seed=seed * 1000 + q)
+        kw.update(overrides)
+        out.append(default_params(**kw))
+    return out
+
+
+class Ensemble:
+    """Independent channels stepped concurrently, one HIP stream per instance and a
+    pool of host worker threads (fpc_ensemble_*)."""
+
+    def __init__(self, params_list, nthreads=8):
+        self.params = list(params_list)
+        arr = (Params * len(self.params))(*self.params)
+        self.h = ctypes.c_void_p()
+        check(lib().fpc_ensemble_create(arr, len(self.params), int(nthreads), ctypes.byref(self.h)), "ensemble create")
+        self.members = []
+        for i in range(len(self.params)):
+            m = Sim.__new__(Sim)
+            m.h = ctypes.c_void_p(lib().fpc_ensemble_member(self.h, i))
+            m.params = self.params[i]
+            m._owned = False
+            self.members.append(m)
+
+    def step(self, n=1):
+        check(lib().fpc_ensemble_step(self.h, n), "ensemble step")
+
+    def offsets(self, field):
+        off = (ctypes.c_longlong * (len(self.params) + 1))()
+        check(lib().fpc_ensemble_offsets(self.h, field.encode(), off), "ensemble offsets")
+        return np.array(off[:], dtype=np.int64)
+
+    def pack(self, field):
+        off = self.offsets(field)
+        out = np.empty(int(off[-1]), dtype=np.float64)
+        check(lib().fpc_ensemble_pack(self.h, field.encode(), out.ctypes.data, out.nbytes), "ensemble pack")
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fpc_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def profile_enable(names="*"):

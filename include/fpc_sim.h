@@ -72,14 +72,38 @@ int fpc_sim_phase(fpc_sim sim, int phase);
 long long fpc_sim_len(fpc_sim sim, const char *name);
 int fpc_sim_copy(fpc_sim sim, const char *name, void *dst, long long nbytes);
 int fpc_sim_set(fpc_sim sim, const char *name, const void *src, long long nbytes);
-/* Device pointer of a field (no copy). */
+/* Device pointer of a field (no copy).  Each simulation runs on a HIP stream of
+ * its own (fpc_sim_stream); synchronise with it before using the pointer on
+ * another stream. */
 int fpc_sim_device_ptr(fpc_sim sim, const char *name, void **ptr);
+/* The simulation's HIP stream (a hipStream_t). */
+void *fpc_sim_stream(fpc_sim sim);
 /* step ramp div_l2 umax vmax wmax its_u its_v its_w its_p relres_u relres_v relres_w
  * relres_p fx fy cd cl err_u err_v err_w err_p; NaN if unknown. */
 double fpc_sim_scalar(fpc_sim sim, const char *name);
 /* Per-stage wall time (s) accumulated over steps: t_build t_mom t_p t_proj t_diag */
 double fpc_sim_timer(fpc_sim sim, const char *name);
 int fpc_sim_destroy(fpc_sim sim);
+
+/* ---- ensemble of independent channels (no counterpart in the reference, which
+ * runs one channel per process) ----
+ * n instances with their own parameters (uneven grids, cylinders, Reynolds
+ * numbers), each on its own HIP stream, created and stepped concurrently by a
+ * pool of nthreads host worker threads (largest instance first).  Every
+ * instance computes exactly what it computes alone.  Kernel timing
+ * (fpc_ProfileEnable) must be off while an ensemble steps. */
+typedef struct fpc_ensemble_s *fpc_ensemble;
+int fpc_ensemble_create(const fpc_params *pars, int n, int nthreads, fpc_ensemble *ens);
+/* Advances every instance by nsteps (a solver's maximum-iterations flag is not an error). */
+int fpc_ensemble_step(fpc_ensemble ens, int nsteps);
+int fpc_ensemble_size(fpc_ensemble ens);
+fpc_sim fpc_ensemble_member(fpc_ensemble ens, int i);
+/* Packed offset table of field u, v, w or p over the instances: instance i owns
+ * elements offsets[i] .. offsets[i+1]-1 of a packed array (n+1 entries). */
+int fpc_ensemble_offsets(fpc_ensemble ens, const char *field, long long *offsets);
+/* The field of every instance copied into one packed host array (offset table layout). */
+int fpc_ensemble_pack(fpc_ensemble ens, const char *field, double *dst, long long nbytes);
+int fpc_ensemble_destroy(fpc_ensemble ens);
 
 #ifdef __cplusplus
 }
