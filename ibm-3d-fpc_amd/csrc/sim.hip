@@ -26,6 +26,7 @@
 #include "fpc_setup.hpp"
 #include "fpc_sim.h"
 #include "fpc_struct.h"
+#include "sim_internal.hpp"
 
 namespace fpc {
 namespace {
@@ -49,38 +50,6 @@ int upload(T **d, const std::vector<T> &h) {
 }  // namespace fpc
 
 using namespace fpc;
-
-struct fpc_sim_s {
-  fpc_params par{};
-  hipStream_t stream = nullptr;  // every call on this handle runs on its own stream
-  host::SetupData S;
-  // device operator data
-  unsigned char *au = nullptr, *av = nullptr, *aw = nullptr, *fp = nullptr, *aqu = nullptr, *aqv = nullptr, *aqw = nullptr;
-  double *scu = nullptr, *scv = nullptr, *scw = nullptr, *bcu = nullptr, *dcoef = nullptr, *divbc = nullptr;
-  double *uin = nullptr, *uin_step = nullptr;
-  long long *fu_i = nullptr, *fv_i = nullptr, *px_i = nullptr, *py_i = nullptr;
-  double *fu_w = nullptr, *fv_w = nullptr, *px_w = nullptr, *py_w = nullptr;
-  fpc_flow_ops ops{};
-  // fields
-  double *u = nullptr, *v = nullptr, *w = nullptr, *p = nullptr, *ut = nullptr, *vt = nullptr, *wt = nullptr;
-  double *qout = nullptr, *diag = nullptr, *hdiag = nullptr;
-  // Struct objects
-  fpc_StructGrid gu = nullptr, gv = nullptr, gw = nullptr, gp = nullptr;
-  fpc_StructStencil st = nullptr;
-  fpc_StructMatrix Au = nullptr, Av = nullptr, Aw = nullptr, Ap = nullptr;
-  fpc_StructVector bu = nullptr, xu = nullptr, bv = nullptr, xv = nullptr, bw = nullptr, xw = nullptr, bp = nullptr,
-                   xp = nullptr;
-  fpc_StructSolver su = nullptr, sv = nullptr, sw = nullptr, sp = nullptr, pu = nullptr, pv = nullptr, pw = nullptr,
-                   pp = nullptr;
-  double *pbu = nullptr, *pxu = nullptr, *pbv = nullptr, *pxv = nullptr, *pbw = nullptr, *pxw = nullptr, *pbp = nullptr,
-         *pxp = nullptr;
-  int step = 0;
-  double ramp = 0.0;
-  int its[4] = {0, 0, 0, 0};
-  double rel[4] = {0, 0, 0, 0};
-  int err[4] = {0, 0, 0, 0};
-  double t_build = 0, t_mom = 0, t_p = 0, t_proj = 0, t_diag = 0;
-};
 
 static double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -124,7 +93,7 @@ static int setup_solver_p(fpc_StructSolver *s, fpc_StructSolver *pc, const fpc_p
   return fpc_StructGMRESSetup(*s, A, b, x);
 }
 
-static int sim_phase(fpc_sim M, int ph) {
+int fpc::sim_run_phase(fpc_sim M, int ph) {
   const fpc_params &par = M->par;
   int nx = par.nx, ny = par.ny, nz = par.nz;
   hipStream_t st = fpc::stream();
@@ -277,6 +246,9 @@ int fpc_sim_create(const fpc_params *par, fpc_sim *out) {
   rc |= upload(&M->dcoef, S.dcoef);
   rc |= upload(&M->divbc, S.div_bc_plane);
   rc |= upload(&M->uin, S.uin);
+  M->uin_base = S.uin;
+  M->divbc_base = S.div_bc_plane;
+  M->theta.assign((size_t)ny * nz, 1.0);
   rc |= upload(&M->uin_step, S.uin);
   rc |= upload(&M->fu_i, S.fu_idx);
   rc |= upload(&M->fv_i, S.fv_idx);
@@ -364,7 +336,7 @@ int fpc_sim_create(const fpc_params *par, fpc_sim *out) {
 int fpc_sim_phase(fpc_sim M, int phase) {
   FPC_CHECK_ARG(M, "null sim");
   fpc::StreamScope scope(M->stream);
-  return sim_phase(M, phase);
+  return fpc::sim_run_phase(M, phase);
 }
 
 int fpc_sim_step(fpc_sim M, int nsteps) {
@@ -372,7 +344,7 @@ int fpc_sim_step(fpc_sim M, int nsteps) {
   fpc::StreamScope scope(M->stream);
   for (int n = 0; n < nsteps; ++n)
     for (int ph = 0; ph < 8; ++ph) {
-      int rc = sim_phase(M, ph);
+      int rc = fpc::sim_run_phase(M, ph);
       if (rc) return rc;
     }
   return 0;
@@ -410,6 +382,25 @@ int fpc_sim_copy(fpc_sim M, const char *name, void *dst, long long nbytes) {
 int fpc_sim_set(fpc_sim M, const char *name, const void *src, long long nbytes) {
   FPC_CHECK_ARG(M && name && src, "null argument");
   fpc::StreamScope scope(M->stream);
+  if (std::string(name) == "step") {
+    FPC_CHECK_ARG(nbytes == (long long)sizeof(int), "step is one int");
+    std::memcpy(&M->step, src, sizeof(int));
+    return 0;
+  }
+  if (std::string(name) == "inlet_amplitude") {
+    const size_t n = M->uin_base.size();
+    FPC_CHECK_ARG(nbytes == (long long)(n * sizeof(double)), "size mismatch (ny*nz doubles)");
+    std::memcpy(M->theta.data(), src, n * sizeof(double));
+    std::vector<double> ui(n), db(n);
+    for (size_t q = 0; q < n; ++q) {
+      ui[q] = M->uin_base[q] * M->theta[q];
+      db[q] = M->divbc_base[q] * M->theta[q];
+    }
+    FPC_HIP(hipMemcpyAsync(M->uin, ui.data(), n * sizeof(double), hipMemcpyHostToDevice, fpc::stream()));
+    FPC_HIP(hipMemcpyAsync(M->divbc, db.data(), n * sizeof(double), hipMemcpyHostToDevice, fpc::stream()));
+    FPC_HIP(hipStreamSynchronize(fpc::stream()));
+    return 0;
+  }
   long long n = -1;
   double *d = field_ptr(M, name, &n);
   FPC_CHECK_ARG(d, "unknown field");

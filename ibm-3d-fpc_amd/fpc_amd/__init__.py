@@ -160,6 +160,11 @@ SIGNATURES = {
     "fpc_ensemble_size": (_i, [_vp]), "fpc_ensemble_member": (_vp, [_vp, _i]),
     "fpc_ensemble_offsets": (_i, [_vp, _cp, ctypes.POINTER(ctypes.c_longlong)]),
     "fpc_ensemble_pack": (_i, [_vp, _cp, _vp, _ll]), "fpc_ensemble_destroy": (_i, [_vp]),
+    "fpc_adjoint_create": (_i, [_vp, _pvp]),
+    "fpc_adjoint_step_vjp": (_i, [_vp, _i] + [_vp] * 9 + [ctypes.POINTER(ctypes.c_double)]),
+    "fpc_adjoint_drag_gradient": (_i, [_vp, _i, _i, _i, ctypes.POINTER(ctypes.c_double), _vp,
+                                      ctypes.POINTER(ctypes.c_double)]),
+    "fpc_adjoint_destroy": (_i, [_vp]),
 }
 
 
@@ -287,6 +292,10 @@ class Sim:
         arr = np.ascontiguousarray(arr, dtype=np.float64)
         check(lib().fpc_sim_set(self.h, name.encode(), arr.ctypes.data, arr.nbytes), f"sim set {name}")
 
+    def set_step(self, n):
+        v = ctypes.c_int(int(n))
+        check(lib().fpc_sim_set(self.h, b"step", ctypes.addressof(v), ctypes.sizeof(v)), "sim set step")
+
     def device_ptr(self, name):
         p = ctypes.c_void_p()
         check(lib().fpc_sim_device_ptr(self.h, name.encode(), ctypes.byref(p)), f"device_ptr {name}")
@@ -305,6 +314,44 @@ class Sim:
 def set_tuning(key, value):
     """Launch tuning of the z-marching stencil kernels ("tile_min_points", "stencil_kz")."""
     check(lib().fpc_SetTuning(key.encode(), int(value)), f"SetTuning({key})")
+
+
+class Adjoint:
+    """Discrete adjoint of a simulation's time step (fpc_adjoint_*): gradients of the
+    time-averaged drag coefficient with respect to the inlet amplitude field theta
+    (ny*nz) and nu."""
+
+    def __init__(self, sim):
+        self.sim = sim
+        self.h = ctypes.c_void_p()
+        check(lib().fpc_adjoint_create(sim.h, ctypes.byref(self.h)), "adjoint create")
+
+    def drag_gradient(self, nsteps, avg_from, every):
+        """(J, dJ/dtheta [ny*nz], dJ/dnu) for J = mean c_D over steps avg_from..nsteps."""
+        p = self.sim.params
+        g = np.zeros(p.ny * p.nz)
+        J, gnu = ctypes.c_double(), ctypes.c_double()
+        check(lib().fpc_adjoint_drag_gradient(self.h, nsteps, avg_from, every, ctypes.byref(J), g.ctypes.data,
+                                              ctypes.byref(gnu)), "drag gradient")
+        return J.value, g, gnu.value
+
+    def step_vjp(self, step, state, adj, theta_bar):
+        """One step's VJP on torch CUDA tensors: state = (u, v, w, p) before step `step`;
+        adj = (au, av, aw, ap) adjoint after the step, overwritten with the adjoint
+        before it; theta_bar (ny*nz) accumulated.  Returns the nu adjoint."""
+        nu = ctypes.c_double(0.0)
+        ptr = lambda t: ctypes.c_void_p(t.data_ptr())
+        check(lib().fpc_adjoint_step_vjp(self.h, step, *[ptr(t) for t in state], *[ptr(t) for t in adj],
+                                         ptr(theta_bar), ctypes.byref(nu)), "step vjp")
+        return nu.value
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.fpc_adjoint_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 def ensemble_workload(n=64, seed=0, nx=(64, 256), nyz=(16, 48), xc=(0.4, 0.7), yc=(0.15, 0.26), re=(10.0, 100.0),

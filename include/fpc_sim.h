@@ -68,7 +68,10 @@ int fpc_sim_step(fpc_sim sim, int nsteps);
  * 4 divergence, 5 pressure solve, 6 projection, 7 diagnostics. */
 int fpc_sim_phase(fpc_sim sim, int phase);
 /* Fields: u v w p u_tilde v_tilde w_tilde rhs_u x_u rhs_v x_v rhs_w x_w rhs_p x_p
- * uin_step qout, and every setup array. */
+ * uin_step qout, and every setup array.  fpc_sim_set also takes
+ * "inlet_amplitude" (ny*nz doubles, default 1): the inlet velocity and the
+ * divergence inlet term become theta * (their reference values); and "step"
+ * (one int): the number of steps taken (the next step's ramp follows from it). */
 long long fpc_sim_len(fpc_sim sim, const char *name);
 int fpc_sim_copy(fpc_sim sim, const char *name, void *dst, long long nbytes);
 int fpc_sim_set(fpc_sim sim, const char *name, const void *src, long long nbytes);
@@ -104,6 +107,31 @@ int fpc_ensemble_offsets(fpc_ensemble ens, const char *field, long long *offsets
 /* The field of every instance copied into one packed host array (offset table layout). */
 int fpc_ensemble_pack(fpc_ensemble ens, const char *field, double *dst, long long nbytes);
 int fpc_ensemble_destroy(fpc_ensemble ens);
+
+/* ---- adjoint (no counterpart in the reference) ----
+ * Discrete adjoint of the time step of a simulation, for gradients of the
+ * time-averaged drag coefficient with respect to the inlet amplitude field
+ * theta (fpc_sim_set(sim, "inlet_amplitude", theta[ny*nz]): the inlet is
+ * theta * Um * profile) and the viscosity nu.  The transposed systems use the
+ * simulation's solver settings. */
+typedef struct fpc_adjoint_s *fpc_adjoint;
+int fpc_adjoint_create(fpc_sim sim, fpc_adjoint *adj);
+/* Vector-Jacobian product of step `step` (1-based) taken from the state
+ * (u, v, w, p) (device arrays of the full staggered sizes): on entry (au, av,
+ * aw, ap) hold the adjoint of the state after the step, on return that of the
+ * state before it; theta_bar (device, ny*nz) and *nu_bar (host) are
+ * accumulated.  Leaves the simulation's fields at the recomputed step's start. */
+int fpc_adjoint_step_vjp(fpc_adjoint adj, int step, const double *u, const double *v, const double *w,
+                         const double *p, double *au, double *av, double *aw, double *ap, double *theta_bar,
+                         double *nu_bar);
+/* From the simulation's current state: runs nsteps steps with
+ * J = mean of c_D over steps avg_from..nsteps (1-based, relative), then its
+ * gradient by reverse steps with the state checkpointed every `every` steps and
+ * each segment recomputed.  grad_theta: host, ny*nz.  The simulation ends at the
+ * state after the nsteps steps. */
+int fpc_adjoint_drag_gradient(fpc_adjoint adj, int nsteps, int avg_from, int every, double *J, double *grad_theta,
+                              double *grad_nu);
+int fpc_adjoint_destroy(fpc_adjoint adj);
 
 #ifdef __cplusplus
 }

@@ -1172,6 +1172,7 @@ struct Sim {
   int step = 0;
   double ramp = 0.0;
   std::vector<double> u, v, w, p, ut, vt, wt, uin_step;
+  std::vector<double> uin_base, div_bc_base;  // before an inlet amplitude field was applied
   std::vector<double> conv_u, conv_v, conv_w, gpx, gpy, gpz;
   std::vector<double> rhs_u, x_u, rhs_v, x_v, rhs_w, x_w, rhs_p, x_p;
   std::vector<double> pvec, gp, qt, Dqt, phi, gphi, divv;
@@ -1780,6 +1781,28 @@ int orc_sim_copy(void *s, const char *name, void *dst, long long nbytes) {
   return copy_out(sim_view(*static_cast<Sim *>(s), name), dst, nbytes);
 }
 int orc_sim_set(void *s, const char *name, const void *src, long long nbytes) {
+  if (std::string(name) == "inlet_amplitude") {
+    // theta (ny*nz): the inlet velocity and the divergence inlet term become theta times
+    // their reference values (the adjoint's parameter; no reference counterpart)
+    Sim &M = *static_cast<Sim *>(s);
+    Setup &S = *M.S;
+    const size_t n = (size_t)S.ny * S.nz;
+    if ((long long)(n * sizeof(double)) != nbytes) return -1;
+    if (M.uin_base.empty()) {
+      M.uin_base = S.uin;
+      M.div_bc_base = S.div_bc;
+    }
+    const double *th = static_cast<const double *>(src);
+    for (size_t q = 0; q < n; ++q) S.uin[q] = M.uin_base[q] * th[q];
+    for (int k = 0; k < S.nz; ++k)
+      for (int j = 0; j < S.ny; ++j) {
+        size_t c = ixP(0, j, k, S.nx, S.ny);
+        if (!S.fluid_p[c]) continue;
+        size_t row = (size_t)S.id_p[c];
+        S.div_bc[row] = M.div_bc_base[row] * th[(size_t)j + (size_t)S.ny * k];
+      }
+    return 0;
+  }
   View v = sim_view(*static_cast<Sim *>(s), name);
   if (!v.w || v.n * v.es != nbytes) return -1;
   std::memcpy(v.w, src, (size_t)nbytes);
