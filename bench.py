@@ -33,6 +33,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "ibm-3d-fpc_amd"))
 
@@ -156,6 +158,57 @@ def run_ensemble(F, torch, nthreads, n=64, steps=50):
     return out
 
 
+ADJOINT = dict(nx=256, ny=64, nz=64, Lx=2.5, Ly=0.41, Lz=0.41, xcirc=0.5, ycirc=0.2, Rcirc=0.05, Um=0.45, nu=1e-3,
+               rho=1.0, dt=1e-3, vel_kdim=30, p_kdim=30, vel_tol=1e-12, p_tol=1e-12, vel_maxit=500, p_maxit=500,
+               init_profile=1, noise_amp=1e-3 * 0.45, seed=0)
+
+
+def run_adjoint(F, torch, nsteps=100, avg_from=50, every=10, ndir=16, h=1e-6):
+    """Gradient of the mean drag coefficient over steps avg_from..nsteps with
+    respect to a 64x64 inlet amplitude field (seeded, 1 +- 5%) and nu, by the
+    checkpointed adjoint; checked against central differences on ndir seeded
+    directions.  Solves to rtol 1e-12 so the differences resolve 1e-7."""
+    rng = np.random.default_rng(0)
+    nin = ADJOINT["ny"] * ADJOINT["nz"]
+    theta = 1.0 + 0.05 * (2.0 * rng.random(nin) - 1.0)
+    nu0 = ADJOINT["nu"]
+
+    def J_of(th, nu):
+        s = F.Sim(F.default_params(**dict(ADJOINT, nu=nu)))
+        s.set("inlet_amplitude", th)
+        cds = []
+        for n in range(1, nsteps + 1):
+            s.step(1)
+            if n >= avg_from:
+                cds.append(s.scalar("cd"))
+        s.close()
+        return float(np.mean(cds))
+
+    sim = F.Sim(F.default_params(**ADJOINT))
+    sim.set("inlet_amplitude", theta)
+    adj = F.Adjoint(sim)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J, g_th, g_nu = adj.drag_gradient(nsteps, avg_from, every)
+    t_grad = time.perf_counter() - t0
+    adj.close()
+    sim.close()
+    t0 = time.perf_counter()
+    J_plain = J_of(theta, nu0)
+    t_fwd = time.perf_counter() - t0
+    errs = []
+    for _ in range(ndir):
+        dth, dnu = rng.standard_normal(nin), float(rng.standard_normal())
+        ad = float(g_th @ dth + g_nu * nu0 * dnu)
+        fd = (J_of(theta + h * dth, nu0 * (1 + h * dnu)) - J_of(theta - h * dth, nu0 * (1 - h * dnu))) / (2 * h)
+        errs.append(abs(fd - ad) / abs(ad))
+    return {"metric": "adjoint gradient of mean c_D, 256x64x64, steps %d-%d, checkpoint every %d" % (avg_from, nsteps, every),
+            "J": J, "J_forward_run": J_plain, "dJ_dnu": g_nu, "dJ_dtheta_norm": float(np.linalg.norm(g_th)),
+            "directions": ndir, "h": h, "fd_rel_err_max": max(errs), "fd_rel_err_median": float(np.median(errs)),
+            "gradient_seconds": t_grad, "forward_seconds": t_fwd, "gradient_over_forward": t_grad / t_fwd,
+            "dtype": "f64", "data": "synthetic (seeded inlet amplitude 1 +- 5%, inlet profile + noise)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,13 +220,14 @@ def main():
     ap.add_argument("--ensemble", action="store_true",
                     help="64 uneven channels, 50 steps: cell-updates/s looped vs concurrent (no headline line)")
     ap.add_argument("--ensemble-threads", type=int, default=8)
+    ap.add_argument("--adjoint", action="store_true",
+                    help="256x64x64 drag-gradient run + 16 finite-difference directions (no headline line)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 2 GiB device-to-device copy 5 times (scale for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
-    import numpy as np
     import torch
 
     torch.cuda.set_device(0)
@@ -189,6 +243,9 @@ def main():
     F.lib()
     if args.ensemble:
         print(json.dumps(run_ensemble(F, torch, args.ensemble_threads)))
+        return
+    if args.adjoint:
+        print(json.dumps(run_adjoint(F, torch)))
         return
     sim64, t64, its64, dom, rec, classes = run_sim(F, torch, WORKLOAD, args.warmup, args.steps,
                                                    not args.no_roofline)
