@@ -158,6 +158,47 @@ def run_ensemble(F, torch, nthreads, n=64, steps=50):
     return out
 
 
+MIXED = dict(WORKLOAD, nx=1024, ny=160, nz=160)
+
+
+def run_mixed(F, torch, steps=100):
+    """The 1024x160x160 channel (26M cells, Re ~100), `steps` steps in FP64 and
+    with the FP32 V-cycle inside FP64 flexible GMRES at the same 1e-8 residual:
+    time-steps/s of both, per-field relative L2 error and drag/lift error of the
+    FP32-MG run against the FP64 run."""
+    out = {"metric": "time-steps/s @1024x160x160, %d steps: FP64 vs FP32-MG at 1e-8 resid" % steps,
+           "unit": "time-steps/s", "steps": steps, "dtype": "f64 (FP32 V-cycle in the mixed run)",
+           "data": "synthetic (inlet profile + seeded noise), geometry built on the host"}
+    fields = {}
+    for bits in (64, 32):
+        t0 = time.perf_counter()
+        sim = F.Sim(F.default_params(**dict(MIXED, mg_precision=bits)))
+        print(f"[mixed] FP{bits}: setup {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        its, rel = [], []
+        t0 = time.perf_counter()
+        for n in range(steps):
+            sim.step(1)
+            its.append([sim.scalar(k) for k in ("its_u", "its_v", "its_w", "its_p")])
+            rel.append(sim.scalar("relres_p"))
+            if (n + 1) % 25 == 0:
+                print(f"[mixed] FP{bits}: step {n + 1}", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        fields[bits] = {f: sim.get(f) for f in ("u", "v", "w", "p")}
+        out["fp%d" % bits] = {"value": steps / dt, "ms_per_step": 1e3 * dt / steps,
+                              "iterations_mean": np.mean(its, axis=0).tolist(), "relres_p_max": max(rel),
+                              "cd": sim.scalar("cd"), "cl": sim.scalar("cl")}
+        sim.close()
+    a, b = fields[64], fields[32]
+    out["field_rel_l2_vs_fp64"] = {f: float(np.linalg.norm(b[f] - a[f]) / np.linalg.norm(a[f])) for f in a}
+    out["cd_rel_err"] = abs(out["fp32"]["cd"] - out["fp64"]["cd"]) / abs(out["fp64"]["cd"])
+    out["cl_rel_err"] = abs(out["fp32"]["cl"] - out["fp64"]["cl"]) / abs(out["fp64"]["cd"])
+    out["value"] = out["fp32"]["value"]
+    out["speedup_vs_fp64"] = out["fp32"]["value"] / out["fp64"]["value"]
+    return out
+
+
 ADJOINT = dict(nx=256, ny=64, nz=64, Lx=2.5, Ly=0.41, Lz=0.41, xcirc=0.5, ycirc=0.2, Rcirc=0.05, Um=0.45, nu=1e-3,
                rho=1.0, dt=1e-3, vel_kdim=30, p_kdim=30, vel_tol=1e-12, p_tol=1e-12, vel_maxit=500, p_maxit=500,
                init_profile=1, noise_amp=1e-3 * 0.45, seed=0)
@@ -222,6 +263,8 @@ def main():
     ap.add_argument("--ensemble-threads", type=int, default=8)
     ap.add_argument("--adjoint", action="store_true",
                     help="256x64x64 drag-gradient run + 16 finite-difference directions (no headline line)")
+    ap.add_argument("--mixed", action="store_true",
+                    help="1024x160x160, 100 steps: FP64 against FP32-MG, per-field errors (no headline line)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 2 GiB device-to-device copy 5 times (scale for HBM byte counters)")
     args = ap.parse_args()
@@ -246,6 +289,9 @@ def main():
         return
     if args.adjoint:
         print(json.dumps(run_adjoint(F, torch)))
+        return
+    if args.mixed:
+        print(json.dumps(run_mixed(F, torch)))
         return
     sim64, t64, its64, dom, rec, classes = run_sim(F, torch, WORKLOAD, args.warmup, args.steps,
                                                    not args.no_roofline)

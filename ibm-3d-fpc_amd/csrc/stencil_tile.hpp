@@ -69,16 +69,14 @@ __device__ inline void march_tile(Op &op, int nx, int ny, int nz, int k0, int ke
   T zm = (in && k0 > 0) ? op.finish(op.fetch(i, j, k0 - 1)) : T(0);
   T zc = (in && k0 < nz) ? op.finish(op.fetch(i, j, k0)) : T(0);
   Raw own1{}, ring0{};
-  Row row{};
   if (in && k0 + 1 < nz) own1 = op.fetch(i, j, k0 + 1);
   if (ring_in && k0 < kend) ring0 = op.fetch(hi, hj, k0);
-  if (in && k0 < kend) row = op.row(i, j, k0);
   for (int k = k0; k < kend; ++k) {
     Raw own2{}, ring1{};
-    Row row1{};
+    Row row{};
+    if (in) row = op.row(i, j, k);
     if (in && k + 2 < nz && k + 1 < kend) own2 = op.fetch(i, j, k + 2);
     if (ring_in && k + 1 < kend) ring1 = op.fetch(hi, hj, k + 1);
-    if (in && k + 1 < kend) row1 = op.row(i, j, k + 1);
     const T zp = (in && k + 1 < nz) ? op.finish(own1) : T(0);
     T(*P)[TX + 2] = plane[(k - k0) & 1];
     P[ty + 1][tx + 1] = zc;
@@ -92,7 +90,6 @@ __device__ inline void march_tile(Op &op, int nx, int ny, int nz, int k0, int ke
     zc = zp;
     own1 = own2;
     ring0 = ring1;
-    row = row1;
   }
 }
 
