@@ -136,14 +136,15 @@ def cpu_baseline():
 
 def run_ensemble(F, torch, nthreads, n=64, steps=50):
     """Aggregate cell-updates/s of n seeded uneven channels over `steps` steps: the
-    instances one after another (the plain path looped) and with `nthreads` host
-    workers, one HIP stream per instance.  One untimed step first (solver setup)."""
+    instances one after another (the plain path looped), with `nthreads` host
+    workers and one HIP stream per instance, and batched (one stream, one launch
+    per stage over all instances).  One untimed step first (solver setup)."""
     pars = F.ensemble_workload(n=n, seed=0)
     cells = sum(p.nx * p.ny * p.nz for p in pars)
     out = {"metric": "cell-updates/s, %d-channel ensemble, %d steps each" % (n, steps), "unit": "cell-updates/s",
            "instances": n, "cells_total": cells, "steps": steps, "dtype": "f64",
            "data": "synthetic (seeded geometry, Re 10-100, inlet profile + noise)"}
-    for tag, nth in (("looped", 1), ("concurrent", nthreads)):
+    for tag, nth in (("looped", 1), ("concurrent", nthreads), ("batched", 0)):
         ens = F.Ensemble(pars, nthreads=nth)
         ens.step(1)
         torch.cuda.synchronize()
@@ -153,8 +154,10 @@ def run_ensemble(F, torch, nthreads, n=64, steps=50):
         dt = time.perf_counter() - t0
         out[tag] = {"value": cells * steps / dt, "seconds": dt, "threads": nth}
         ens.close()
-    out["value"] = out["concurrent"]["value"]
-    out["speedup_vs_looped"] = out["concurrent"]["value"] / out["looped"]["value"]
+    best = max(("concurrent", "batched"), key=lambda k: out[k]["value"])
+    out["value"] = out[best]["value"]
+    out["mode"] = best
+    out["speedup_vs_looped"] = out[best]["value"] / out["looped"]["value"]
     return out
 
 

@@ -77,7 +77,15 @@ void prof_drop_skipped(size_t from, int ran);
 
 constexpr int kSlots = 7;
 constexpr int kMaxKDim = 128;       // largest GMRES restart length supported
-constexpr int kRedBlocks = 1024;    // blocks of the deterministic reductions
+constexpr int kRedBlocks = 1024;    // most blocks of the deterministic reductions
+
+// Blocks of a deterministic reduction over n points: one per 2048 points, at
+// most kRedBlocks.  The partition depends on n only, so every reduction of a
+// vector (alone or in a batch) sums the same partials in the same order.
+inline int red_blocks(long long n) {
+  long long b = (n + 2047) / 2048;
+  return (int)(b < 1 ? 1 : (b > kRedBlocks ? kRedBlocks : b));
+}
 
 inline long long pad256(long long n, int elsize) {
   long long per = 256 / elsize;

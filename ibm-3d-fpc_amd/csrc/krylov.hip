@@ -28,36 +28,14 @@
 #include <cstring>
 
 #include "fpc_internal.hpp"
+#include "batched.hpp"
+#include "krylov_device.hpp"
 #include "stencil_tile.hpp"
 
 namespace fpc {
 namespace {
 
-constexpr int KM = kMaxKDim;
-constexpr int NB = kRedBlocks;  // blocks of every partial-sum kernel
-
-// Status record polled by the host (first, small part of the device state).
-struct Status {
-  int i;        // Arnoldi steps completed in the current cycle
-  int done;     // 1: converged estimate, 2: cycle full or maxit reached
-  int iter;     // total steps
-  int maxit;
-  int kdim;
-  int pad;
-  double r_norm;  // current residual estimate |rs[i]|
-  double eps;
-};
-
-struct State {
-  Status st;
-  double sigma[KM + 1];          // scale of stored basis vector j (v_j = sigma_j p_j)
-  double rs[KM + 1];
-  double c[KM], s[KM];
-  double y[KM + 1];              // combination coefficients
-  double hh[(KM + 1) * KM];      // hh(row, col) = hh[col * (KM + 1) + row]
-};
-
-__device__ inline double &H(State *S, int row, int col) { return S->hh[col * (KM + 1) + row]; }
+using namespace kry;
 
 // w = A z with z = sigma * p (* dinv when Jacobi is fused), one thread per point.
 // The current basis index is read from the device status.
@@ -73,25 +51,8 @@ __global__ void k_arnoldi_matvec(const double *__restrict__ c, long long pitch, 
   const double *z = zin ? zin : V + (long long)cur * vp;
   double sg = S->sigma[cur];
   double *w = (double *)V + (long long)(cur + 1) * vp;
-  long long r = ((long long)k * ny + j) * nx + i, pl = (long long)nx * ny;
-  double s;
-  if (dinv) {
-    s = c[r] * (z[r] * dinv[r]);
-    if (i > 0) s += c[pitch + r] * (z[r - 1] * dinv[r - 1]);
-    if (i < nx - 1) s += c[2 * pitch + r] * (z[r + 1] * dinv[r + 1]);
-    if (j > 0) s += c[3 * pitch + r] * (z[r - nx] * dinv[r - nx]);
-    if (j < ny - 1) s += c[4 * pitch + r] * (z[r + nx] * dinv[r + nx]);
-    if (k > 0) s += c[5 * pitch + r] * (z[r - pl] * dinv[r - pl]);
-    if (k < nz - 1) s += c[6 * pitch + r] * (z[r + pl] * dinv[r + pl]);
-  } else {
-    s = c[r] * z[r];
-    if (i > 0) s += c[pitch + r] * z[r - 1];
-    if (i < nx - 1) s += c[2 * pitch + r] * z[r + 1];
-    if (j > 0) s += c[3 * pitch + r] * z[r - nx];
-    if (j < ny - 1) s += c[4 * pitch + r] * z[r + nx];
-    if (k > 0) s += c[5 * pitch + r] * z[r - pl];
-    if (k < nz - 1) s += c[6 * pitch + r] * z[r + pl];
-  }
+  long long r = ((long long)k * ny + j) * nx + i;
+  double s = arnoldi_row(c, pitch, nx, ny, nz, z, dinv, i, j, k, r);
   w[r] = sg * s;
 }
 
@@ -162,146 +123,32 @@ int launch_arnoldi_matvec(const fpc_StructMatrix_s *A, const double *V, long lon
 __global__ void __launch_bounds__(256) k_multidot(const double *__restrict__ V, long long vp, long long n, State *S,
                                                   double *__restrict__ part) {
   if (S->st.done) return;
-  __shared__ double lds[8 * 4];
-  int cur = S->st.i;
-  const double *w = V + (long long)(cur + 1) * vp;
-  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (int j0 = 0; j0 <= cur; j0 += 8) {
-    int nj = min(8, cur + 1 - j0);
-    double acc[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) acc[q] = 0.0;
-    // two points per trip: twice the loads in flight per thread
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    for (; t + stride < n; t += 2 * stride) {
-      const double wa = w[t], wb = w[t + stride];
-#pragma unroll
-      for (int q = 0; q < 8; ++q)
-        if (q < nj) {
-          const double *vq = V + (long long)(j0 + q) * vp;
-          acc[q] += vq[t] * wa;
-          acc[q] += vq[t + stride] * wb;
-        }
-    }
-    if (t < n) {
-      const double wa = w[t];
-#pragma unroll
-      for (int q = 0; q < 8; ++q)
-        if (q < nj) acc[q] += V[(long long)(j0 + q) * vp + t] * wa;
-    }
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      double v = acc[q];
-      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-      acc[q] = v;
-    }
-    __syncthreads();
-    if (lane == 0)
-#pragma unroll
-      for (int q = 0; q < 8; ++q) lds[q * 4 + wid] = acc[q];
-    __syncthreads();
-    if (threadIdx.x < nj) {
-      int q = threadIdx.x;
-      part[(long long)(j0 + q) * NB + blockIdx.x] = ((lds[q * 4] + lds[q * 4 + 1]) + lds[q * 4 + 2]) + lds[q * 4 + 3];
-    }
-  }
+  multidot_body(V, vp, n, S, part, blockIdx.x, gridDim.x);
 }
 
 // h_{j,cur} = sigma_j * sum_b part[j][b]; one block per j.
-__global__ void __launch_bounds__(256) k_reduce_h(State *S, const double *__restrict__ part) {
+__global__ void __launch_bounds__(256) k_reduce_h(State *S, const double *__restrict__ part, int nred) {
   if (S->st.done) return;
-  __shared__ double lds[4];
-  int cur = S->st.i;
-  int jv = blockIdx.x;
-  if (jv > cur) return;
-  double acc = 0.0;
-  for (int t = threadIdx.x; t < NB; t += blockDim.x) acc += part[(long long)jv * NB + t];
-  double tot = block_sum<256>(acc, lds);
-  if (threadIdx.x == 0) H(S, jv, cur) = S->sigma[jv] * tot;
+  reduce_h_body(S, part, nred, blockIdx.x);
 }
 
 // w -= sum_j h_j sigma_j p_j with partial sums of w^2.
 __global__ void __launch_bounds__(256) k_orth_update(double *__restrict__ V, long long vp, long long n, State *S,
                                                      double *__restrict__ part) {
   if (S->st.done) return;
-  __shared__ double coef[KM + 1];
-  __shared__ double lds[4];
-  int cur = S->st.i;
-  for (int q = threadIdx.x; q <= cur; q += blockDim.x) coef[q] = H(S, q, cur) * S->sigma[q];
-  __syncthreads();
-  double *w = V + (long long)(cur + 1) * vp;
-  double acc = 0.0;
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  for (; t + stride < n; t += 2 * stride) {
-    double va = w[t], vb = w[t + stride];
-    for (int q = 0; q <= cur; ++q) {
-      const double *vq = V + (long long)q * vp;
-      va -= coef[q] * vq[t];
-      vb -= coef[q] * vq[t + stride];
-    }
-    w[t] = va;
-    w[t + stride] = vb;
-    acc += va * va;
-    acc += vb * vb;
-  }
-  if (t < n) {
-    double v = w[t];
-    for (int q = 0; q <= cur; ++q) v -= coef[q] * V[(long long)q * vp + t];
-    w[t] = v;
-    acc += v * v;
-  }
-  double tot = block_sum<256>(acc, lds);
-  if (threadIdx.x == 0) part[blockIdx.x] = tot;
+  orth_update_body(V, vp, n, S, part, blockIdx.x, gridDim.x);
 }
 
 // Norm of the new vector, Givens update, convergence flag (HYPRE's Arnoldi tail).
-__global__ void __launch_bounds__(256) k_arnoldi_tail(State *S, const double *__restrict__ part) {
+__global__ void __launch_bounds__(256) k_arnoldi_tail(State *S, const double *__restrict__ part, int nred) {
   if (S->st.done) return;
-  __shared__ double lds[4];
-  double acc = 0.0;
-  for (int t = threadIdx.x; t < NB; t += blockDim.x) acc += part[t];
-  double tot = block_sum<256>(acc, lds);
-  if (threadIdx.x != 0) return;
-  int cur = S->st.i;  // HYPRE's i - 1
-  int i = cur + 1;
-  double t = sqrt(tot);
-  H(S, i, cur) = t;
-  S->sigma[i] = (t != 0.0) ? 1.0 / t : 1.0;
-  for (int j = 1; j < i; ++j) {
-    double tt = H(S, j - 1, cur);
-    H(S, j - 1, cur) = S->s[j - 1] * H(S, j, cur) + S->c[j - 1] * tt;
-    H(S, j, cur) = -S->s[j - 1] * tt + S->c[j - 1] * H(S, j, cur);
-  }
-  double g = sqrt(H(S, i, cur) * H(S, i, cur) + H(S, cur, cur) * H(S, cur, cur));
-  if (g == 0.0) g = 2.220446049250313e-16;
-  S->c[cur] = H(S, cur, cur) / g;
-  S->s[cur] = H(S, i, cur) / g;
-  S->rs[i] = -H(S, i, cur) * S->rs[cur] / g;
-  S->rs[cur] = S->c[cur] * S->rs[cur];
-  H(S, cur, cur) = S->s[cur] * H(S, i, cur) + S->c[cur] * H(S, cur, cur);
-  S->st.r_norm = fabs(S->rs[i]);
-  S->st.i = i;
-  S->st.iter += 1;
-  if (S->st.r_norm <= S->st.eps) S->st.done = 1;
-  else if (i >= S->st.kdim || S->st.iter >= S->st.maxit) S->st.done = 2;
+  arnoldi_tail_body(S, part, nred);
 }
 
 // Upper-triangular solve; y_j = sigma_j * rs_j (coefficients of the stored p_j).
 __global__ void k_backsolve(State *S) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  int i = S->st.i;
-  double rs[KM + 1];
-  for (int q = 0; q <= i; ++q) rs[q] = S->rs[q];
-  rs[i - 1] = rs[i - 1] / H(S, i - 1, i - 1);
-  for (int k = i - 2; k >= 0; --k) {
-    double t = 0.0;
-    for (int j = k + 1; j < i; ++j) t -= H(S, k, j) * rs[j];
-    t += rs[k];
-    rs[k] = t / H(S, k, k);
-  }
-  for (int q = 0; q < i; ++q) S->y[q] = rs[q] * S->sigma[q];
+  backsolve_body(S);
 }
 
 // out (+)= [dinv *] sum_j y_j P_j   (x update or the preconditioner input)
@@ -312,9 +159,7 @@ __global__ void k_combine(const double *__restrict__ P, long long vp, long long 
   for (int q = threadIdx.x; q < i; q += blockDim.x) y[q] = S->y[q];
   __syncthreads();
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
-    double v = y[i - 1] * P[(long long)(i - 1) * vp + t];
-    for (int q = i - 2; q >= 0; --q) v += y[q] * P[(long long)q * vp + t];
-    if (dinv) v *= dinv[t];
+    const double v = combine_point(P, vp, t, y, i, dinv);
     out[t] = accumulate ? out[t] + v : v;
   }
 }
@@ -322,12 +167,7 @@ __global__ void k_combine(const double *__restrict__ P, long long vp, long long 
 // HYPRE's residual at a full restart: rs_j rotated back, p_0 = sum_j rs_j v_j.
 __global__ void k_restart_coefs(State *S) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  int i = S->st.i;
-  for (int j = i; j > 0; --j) {
-    S->rs[j - 1] = -S->s[j - 1] * S->rs[j];
-    S->rs[j] = S->c[j - 1] * S->rs[j];
-  }
-  for (int j = 0; j <= i; ++j) S->y[j] = S->rs[j] * S->sigma[j];
+  restart_coefs_body(S);
 }
 
 __global__ void k_restart_vec(double *__restrict__ V, long long vp, long long n, const State *S) {
@@ -335,25 +175,14 @@ __global__ void k_restart_vec(double *__restrict__ V, long long vp, long long n,
   int i = S->st.i;
   for (int q = threadIdx.x; q <= i; q += blockDim.x) y[q] = S->y[q];
   __syncthreads();
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
-    double v = y[0] * V[t];
-    for (int q = 1; q <= i; ++q) v += y[q] * V[(long long)q * vp + t];
-    V[t] = v;
-  }
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+    V[t] = restart_point(V, vp, t, y, i);
 }
 
 // Start of a cycle: i = 0, rs_0 = r_norm, sigma_0 = 1/r_norm.
 __global__ void k_cycle_start(State *S, double r_norm, double eps, int iter, int maxit, int kdim) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  S->st.i = 0;
-  S->st.done = 0;
-  S->st.iter = iter;
-  S->st.maxit = maxit;
-  S->st.kdim = kdim;
-  S->st.r_norm = r_norm;
-  S->st.eps = eps;
-  S->rs[0] = r_norm;
-  S->sigma[0] = 1.0 / r_norm;
+  cycle_start_body(S, r_norm, eps, iter, maxit, kdim);
 }
 
 __global__ void k_scale_copy(double *__restrict__ dst, const double *__restrict__ src, const State *S, long long n) {
@@ -439,6 +268,7 @@ struct GmresImpl {
   // workspace
   long long N = 0, vp = 0;
   int kd = 0;
+  int nred = 1;  // blocks of the reductions over N points (red_blocks)
   double *V = nullptr;    // kd+1 basis vectors
   double *Z = nullptr;    // flexible: kd preconditioned vectors; standard: one work vector
   double *W = nullptr;    // work vector (true residual / precond input)
@@ -494,6 +324,45 @@ int precond_apply(GmresImpl *G, const double *in, double *out, const int *skip) 
 }
 
 }  // namespace
+
+int gmres_batch_view(fpc_StructSolver s, GmresBatchView *v) {
+  GmresImpl *G = gimpl(s);
+  if (!G || !G->A || G->N == 0 || !G->S) {
+    set_error(FPC_ERROR_ARG, "batched solve: a solver is not set up");
+    return FPC_ERROR_ARG;
+  }
+  if (G->flexible) {
+    set_error(FPC_ERROR_ARG, "batched solve: flexible GMRES is not supported");
+    return FPC_ERROR_ARG;
+  }
+  if (G->pc_kind == PC_NONE) v->pc = 0;
+  else if (G->pc_kind == PC_JACOBI1) v->pc = 1;
+  else if (G->pc_kind == PC_PFMG) v->pc = 3;
+  else {
+    set_error(FPC_ERROR_ARG, "batched solve: only Jacobi (one sweep) and PFMG preconditioners are supported");
+    return FPC_ERROR_ARG;
+  }
+  v->V = G->V;
+  v->Z = G->Z;
+  v->W = G->W;
+  v->part = G->part;
+  v->S = G->S;
+  v->N = G->N;
+  v->vp = G->vp;
+  v->kd = G->kd;
+  v->nred = G->nred;
+  v->maxit = G->maxit;
+  v->tol = G->tol;
+  v->atol = G->atol;
+  v->dinv = (v->pc == 1) ? G->A->dinv : nullptr;
+  v->A = G->A;
+  v->pfmg = (v->pc == 3) ? G->pc->impl : nullptr;
+  v->pred = &G->pred;
+  v->its = &G->its;
+  v->relres = &G->relres;
+  return 0;
+}
+
 }  // namespace fpc
 
 using namespace fpc;
@@ -558,6 +427,7 @@ int fpc_StructGMRESSetup(fpc_StructSolver s, fpc_StructMatrix A, fpc_StructVecto
     G->N = N;
     G->kd = G->kdim;
     G->vp = pad256(std::max(1LL, N), (int)sizeof(double));
+    G->nred = red_blocks(N);
     FPC_HIP(hipMalloc(&G->V, (size_t)(G->kd + 1) * G->vp * sizeof(double)));
     int nz = G->flexible ? G->kd : 1;
     FPC_HIP(hipMalloc(&G->Z, (size_t)nz * G->vp * sizeof(double)));
@@ -593,11 +463,11 @@ int fpc_StructGMRESSetup(fpc_StructSolver s, fpc_StructMatrix A, fpc_StructVecto
 static int true_residual(GmresImpl *G, const double *b, const double *x, double *rout, bool with_b) {
   int rc = residual_norm_partials(G->A, b, x, rout, G->part);
   if (rc) return rc;
-  rc = reduce_partials(G->part, NB, G->dscal);
+  rc = reduce_partials(G->part, G->nred, G->dscal);
   if (rc) return rc;
   if (with_b) {
     rc = dot_partials(b, b, G->N, G->part + NB);
-    if (!rc) rc = reduce_partials(G->part + NB, NB, G->dscal + 1);
+    if (!rc) rc = reduce_partials(G->part + NB, G->nred, G->dscal + 1);
     if (rc) return rc;
   }
   FPC_HIP(hipMemcpyAsync(G->hscal, G->dscal, 2 * sizeof(double), hipMemcpyDeviceToHost, stream()));
@@ -622,20 +492,20 @@ static int arnoldi_tail(GmresImpl *G, int step) {
   {
     // bytes per row: the step+1 basis vectors once, w once per group of eight
     FPC_PROF_BEGIN("multidot");
-    k_multidot<<<NB, 256, 0, stream()>>>(G->V, G->vp, G->N, G->S, G->part);
+    k_multidot<<<G->nred, 256, 0, stream()>>>(G->V, G->vp, G->N, G->S, G->part);
     FPC_PROF_END("multidot", (double)G->N * 8.0 * (nv + std::ceil(nv / 8.0)));
   }
   FPC_LAUNCH_CHECK();
-  k_reduce_h<<<KM + 1, 256, 0, stream()>>>(G->S, G->part);
+  k_reduce_h<<<KM + 1, 256, 0, stream()>>>(G->S, G->part, G->nred);
   FPC_LAUNCH_CHECK();
   {
     // bytes per row: the step+1 basis vectors, w read and written
     FPC_PROF_BEGIN("orth_update");
-    k_orth_update<<<NB, 256, 0, stream()>>>(G->V, G->vp, G->N, G->S, G->part);
+    k_orth_update<<<G->nred, 256, 0, stream()>>>(G->V, G->vp, G->N, G->S, G->part);
     FPC_PROF_END("orth_update", (double)G->N * 8.0 * (nv + 2.0));
   }
   FPC_LAUNCH_CHECK();
-  k_arnoldi_tail<<<1, 256, 0, stream()>>>(G->S, G->part);
+  k_arnoldi_tail<<<1, 256, 0, stream()>>>(G->S, G->part, G->nred);
   FPC_LAUNCH_CHECK();
   return 0;
 }

@@ -1,0 +1,296 @@
+// krylov_device.hpp -- device state and per-block bodies of the GMRES kernels
+// (libfpc).
+//
+// One solver's kernels (krylov.hip, struct_mv.hip) and the batched kernels that
+// advance many solvers in one launch (batched.hip) run these same bodies, so a
+// system solved in a batch computes bit for bit what it computes alone.  Block
+// index and block count are parameters: a batched launch maps blockIdx.y to
+// the system and passes that system's own partition.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fpc_internal.hpp"
+
+namespace fpc {
+namespace kry {
+
+constexpr int KM = kMaxKDim;
+constexpr int NB = kRedBlocks;  // row stride of the partial-sum arrays
+
+// Status record polled by the host (first, small part of the device state).
+struct Status {
+  int i;        // Arnoldi steps completed in the current cycle
+  int done;     // 1: converged estimate, 2: cycle full or maxit reached, 3: solve finished
+  int iter;     // total steps
+  int maxit;
+  int kdim;
+  int pad;
+  double r_norm;  // current residual estimate |rs[i]|
+  double eps;
+};
+
+struct State {
+  Status st;
+  double sigma[KM + 1];          // scale of stored basis vector j (v_j = sigma_j p_j)
+  double rs[KM + 1];
+  double c[KM], s[KM];
+  double y[KM + 1];              // combination coefficients
+  double hh[(KM + 1) * KM];      // hh(row, col) = hh[col * (KM + 1) + row]
+};
+
+__device__ inline double &H(State *S, int row, int col) { return S->hh[col * (KM + 1) + row]; }
+
+// (A z)_r at point (i, j, k) = row r, with z scaled by 1/diag when dinv is given.
+__device__ inline double arnoldi_row(const double *__restrict__ c, long long pitch, int nx, int ny, int nz,
+                                     const double *__restrict__ z, const double *__restrict__ dinv, int i, int j,
+                                     int k, long long r) {
+  const long long pl = (long long)nx * ny;
+  double s;
+  if (dinv) {
+    s = c[r] * (z[r] * dinv[r]);
+    if (i > 0) s += c[pitch + r] * (z[r - 1] * dinv[r - 1]);
+    if (i < nx - 1) s += c[2 * pitch + r] * (z[r + 1] * dinv[r + 1]);
+    if (j > 0) s += c[3 * pitch + r] * (z[r - nx] * dinv[r - nx]);
+    if (j < ny - 1) s += c[4 * pitch + r] * (z[r + nx] * dinv[r + nx]);
+    if (k > 0) s += c[5 * pitch + r] * (z[r - pl] * dinv[r - pl]);
+    if (k < nz - 1) s += c[6 * pitch + r] * (z[r + pl] * dinv[r + pl]);
+  } else {
+    s = c[r] * z[r];
+    if (i > 0) s += c[pitch + r] * z[r - 1];
+    if (i < nx - 1) s += c[2 * pitch + r] * z[r + 1];
+    if (j > 0) s += c[3 * pitch + r] * z[r - nx];
+    if (j < ny - 1) s += c[4 * pitch + r] * z[r + nx];
+    if (k > 0) s += c[5 * pitch + r] * z[r - pl];
+    if (k < nz - 1) s += c[6 * pitch + r] * z[r + pl];
+  }
+  return s;
+}
+
+// Partial dots <p_j, w> for j = 0..cur, eight vectors per pass over w; block bid of nblk.
+__device__ inline void multidot_body(const double *__restrict__ V, long long vp, long long n, State *S,
+                                     double *__restrict__ part, int bid, int nblk) {
+  __shared__ double lds[8 * 4];
+  int cur = S->st.i;
+  const double *w = V + (long long)(cur + 1) * vp;
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int j0 = 0; j0 <= cur; j0 += 8) {
+    int nj = min(8, cur + 1 - j0);
+    double acc[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc[q] = 0.0;
+    // two points per trip: twice the loads in flight per thread
+    const long long stride = (long long)nblk * blockDim.x;
+    long long t = bid * (long long)blockDim.x + threadIdx.x;
+    for (; t + stride < n; t += 2 * stride) {
+      const double wa = w[t], wb = w[t + stride];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (q < nj) {
+          const double *vq = V + (long long)(j0 + q) * vp;
+          acc[q] += vq[t] * wa;
+          acc[q] += vq[t + stride] * wb;
+        }
+    }
+    if (t < n) {
+      const double wa = w[t];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (q < nj) acc[q] += V[(long long)(j0 + q) * vp + t] * wa;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      double v = acc[q];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      acc[q] = v;
+    }
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) lds[q * 4 + wid] = acc[q];
+    __syncthreads();
+    if (threadIdx.x < nj) {
+      int q = threadIdx.x;
+      part[(long long)(j0 + q) * NB + bid] = ((lds[q * 4] + lds[q * 4 + 1]) + lds[q * 4 + 2]) + lds[q * 4 + 3];
+    }
+  }
+}
+
+// h_{jv,cur} = sigma_jv * sum of the nred partials of row jv (one block of 256).
+__device__ inline void reduce_h_body(State *S, const double *__restrict__ part, int nred, int jv) {
+  __shared__ double lds[4];
+  int cur = S->st.i;
+  if (jv > cur) return;
+  double acc = 0.0;
+  for (int t = threadIdx.x; t < nred; t += blockDim.x) acc += part[(long long)jv * NB + t];
+  double tot = block_sum<256>(acc, lds);
+  if (threadIdx.x == 0) H(S, jv, cur) = S->sigma[jv] * tot;
+}
+
+// w -= sum_j h_j sigma_j p_j with partial sums of w^2; block bid of nblk.
+__device__ inline void orth_update_body(double *__restrict__ V, long long vp, long long n, State *S,
+                                        double *__restrict__ part, int bid, int nblk) {
+  __shared__ double coef[KM + 1];
+  __shared__ double lds[4];
+  int cur = S->st.i;
+  for (int q = threadIdx.x; q <= cur; q += blockDim.x) coef[q] = H(S, q, cur) * S->sigma[q];
+  __syncthreads();
+  double *w = V + (long long)(cur + 1) * vp;
+  double acc = 0.0;
+  const long long stride = (long long)nblk * blockDim.x;
+  long long t = bid * (long long)blockDim.x + threadIdx.x;
+  for (; t + stride < n; t += 2 * stride) {
+    double va = w[t], vb = w[t + stride];
+    for (int q = 0; q <= cur; ++q) {
+      const double *vq = V + (long long)q * vp;
+      va -= coef[q] * vq[t];
+      vb -= coef[q] * vq[t + stride];
+    }
+    w[t] = va;
+    w[t + stride] = vb;
+    acc += va * va;
+    acc += vb * vb;
+  }
+  if (t < n) {
+    double v = w[t];
+    for (int q = 0; q <= cur; ++q) v -= coef[q] * V[(long long)q * vp + t];
+    w[t] = v;
+    acc += v * v;
+  }
+  double tot = block_sum<256>(acc, lds);
+  if (threadIdx.x == 0) part[bid] = tot;
+}
+
+// Norm of the new vector, Givens update, convergence flag (HYPRE's Arnoldi
+// tail); one block of 256, nred partials.
+__device__ inline void arnoldi_tail_body(State *S, const double *__restrict__ part, int nred) {
+  __shared__ double lds[4];
+  double acc = 0.0;
+  for (int t = threadIdx.x; t < nred; t += blockDim.x) acc += part[t];
+  double tot = block_sum<256>(acc, lds);
+  if (threadIdx.x != 0) return;
+  int cur = S->st.i;  // HYPRE's i - 1
+  int i = cur + 1;
+  double t = sqrt(tot);
+  H(S, i, cur) = t;
+  S->sigma[i] = (t != 0.0) ? 1.0 / t : 1.0;
+  for (int j = 1; j < i; ++j) {
+    double tt = H(S, j - 1, cur);
+    H(S, j - 1, cur) = S->s[j - 1] * H(S, j, cur) + S->c[j - 1] * tt;
+    H(S, j, cur) = -S->s[j - 1] * tt + S->c[j - 1] * H(S, j, cur);
+  }
+  double g = sqrt(H(S, i, cur) * H(S, i, cur) + H(S, cur, cur) * H(S, cur, cur));
+  if (g == 0.0) g = 2.220446049250313e-16;
+  S->c[cur] = H(S, cur, cur) / g;
+  S->s[cur] = H(S, i, cur) / g;
+  S->rs[i] = -H(S, i, cur) * S->rs[cur] / g;
+  S->rs[cur] = S->c[cur] * S->rs[cur];
+  H(S, cur, cur) = S->s[cur] * H(S, i, cur) + S->c[cur] * H(S, cur, cur);
+  S->st.r_norm = fabs(S->rs[i]);
+  S->st.i = i;
+  S->st.iter += 1;
+  if (S->st.r_norm <= S->st.eps) S->st.done = 1;
+  else if (i >= S->st.kdim || S->st.iter >= S->st.maxit) S->st.done = 2;
+}
+
+// Upper-triangular solve; y_j = sigma_j * rs_j (coefficients of the stored p_j). One thread.
+__device__ inline void backsolve_body(State *S) {
+  int i = S->st.i;
+  double rs[KM + 1];
+  for (int q = 0; q <= i; ++q) rs[q] = S->rs[q];
+  rs[i - 1] = rs[i - 1] / H(S, i - 1, i - 1);
+  for (int k = i - 2; k >= 0; --k) {
+    double t = 0.0;
+    for (int j = k + 1; j < i; ++j) t -= H(S, k, j) * rs[j];
+    t += rs[k];
+    rs[k] = t / H(S, k, k);
+  }
+  for (int q = 0; q < i; ++q) S->y[q] = rs[q] * S->sigma[q];
+}
+
+// [dinv *] sum_j y_j P_j at point t (y staged in LDS, i terms).
+__device__ inline double combine_point(const double *__restrict__ P, long long vp, long long t, const double *y, int i,
+                                       const double *__restrict__ dinv) {
+  double v = y[i - 1] * P[(long long)(i - 1) * vp + t];
+  for (int q = i - 2; q >= 0; --q) v += y[q] * P[(long long)q * vp + t];
+  if (dinv) v *= dinv[t];
+  return v;
+}
+
+// HYPRE's residual at a full restart: rs_j rotated back, p_0 = sum_j rs_j v_j. One thread.
+__device__ inline void restart_coefs_body(State *S) {
+  int i = S->st.i;
+  for (int j = i; j > 0; --j) {
+    S->rs[j - 1] = -S->s[j - 1] * S->rs[j];
+    S->rs[j] = S->c[j - 1] * S->rs[j];
+  }
+  for (int j = 0; j <= i; ++j) S->y[j] = S->rs[j] * S->sigma[j];
+}
+
+__device__ inline double restart_point(const double *__restrict__ V, long long vp, long long t, const double *y, int i) {
+  double v = y[0] * V[t];
+  for (int q = 1; q <= i; ++q) v += y[q] * V[(long long)q * vp + t];
+  return v;
+}
+
+// Start of a cycle: i = 0, rs_0 = r_norm, sigma_0 = 1/r_norm. One thread.
+__device__ inline void cycle_start_body(State *S, double r_norm, double eps, int iter, int maxit, int kdim) {
+  S->st.i = 0;
+  S->st.done = 0;
+  S->st.iter = iter;
+  S->st.maxit = maxit;
+  S->st.kdim = kdim;
+  S->st.r_norm = r_norm;
+  S->st.eps = eps;
+  S->rs[0] = r_norm;
+  S->sigma[0] = 1.0 / r_norm;
+}
+
+// r = b - A x (r optional) with the block's partial sum of r^2 into part[bid]; block bid of nblk.
+__device__ inline void residual_body(const double *__restrict__ c, long long pitch, int nx, int ny, int nz,
+                                     const double *__restrict__ b, const double *__restrict__ x, double *__restrict__ r,
+                                     double *__restrict__ part, int bid, int nblk) {
+  __shared__ double lds[8];
+  long long N = (long long)nx * ny * nz, pl = (long long)nx * ny;
+  double acc = 0.0;
+  for (long long t = bid * (long long)blockDim.x + threadIdx.x; t < N; t += (long long)nblk * blockDim.x) {
+    int i = (int)(t % nx);
+    long long q = t / nx;
+    int j = (int)(q % ny);
+    int k = (int)(q / ny);
+    double s = c[t] * x[t];
+    if (i > 0) s += c[pitch + t] * x[t - 1];
+    if (i < nx - 1) s += c[2 * pitch + t] * x[t + 1];
+    if (j > 0) s += c[3 * pitch + t] * x[t - nx];
+    if (j < ny - 1) s += c[4 * pitch + t] * x[t + nx];
+    if (k > 0) s += c[5 * pitch + t] * x[t - pl];
+    if (k < nz - 1) s += c[6 * pitch + t] * x[t + pl];
+    double rv = b[t] - s;
+    if (r) r[t] = rv;
+    acc += rv * rv;
+  }
+  double tot = block_sum<256>(acc, lds);
+  if (threadIdx.x == 0) part[bid] = tot;
+}
+
+__device__ inline void dot_body(const double *__restrict__ a, const double *__restrict__ b, long long n,
+                                double *__restrict__ part, int bid, int nblk) {
+  __shared__ double lds[8];
+  double acc = 0.0;
+  for (long long t = bid * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)nblk * blockDim.x)
+    acc += a[t] * b[t];
+  double tot = block_sum<256>(acc, lds);
+  if (threadIdx.x == 0) part[bid] = tot;
+}
+
+// Sum of n partials into *out (one block of 1024).
+__device__ inline void reduce_body(const double *__restrict__ part, int n, double *__restrict__ out) {
+  __shared__ double lds[16];
+  double acc = 0.0;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) acc += part[t];
+  double tot = block_sum<1024>(acc, lds);
+  if (threadIdx.x == 0) *out = tot;
+}
+
+}  // namespace kry
+}  // namespace fpc

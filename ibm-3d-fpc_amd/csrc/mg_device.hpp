@@ -1,0 +1,100 @@
+// mg_device.hpp -- point bodies of the one-point-per-thread multigrid kernels
+// (libfpc).
+//
+// The level kernels of one hierarchy (pfmg.hip) and the batched kernels that run
+// a level of many hierarchies in one launch (batched.hip) evaluate these same
+// functions, so a V-cycle in a batch is bit for bit the V-cycle alone.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace fpc {
+namespace mgd {
+
+// Residual of one weighted-Jacobi sweep from zero: b - A (w D^-1 b) at (i, j, k) = row t.
+template <typename T>
+__device__ inline T down_point(const T *__restrict__ c, long long pitch, int nx, int ny, int nz,
+                               const T *__restrict__ dinv, const T *__restrict__ b, T w, int i, int j, int k,
+                               long long t) {
+  const long long pl = (long long)nx * ny;
+  T s = c[t] * (w * b[t] * dinv[t]);
+  if (i > 0) s += c[pitch + t] * (w * b[t - 1] * dinv[t - 1]);
+  if (i < nx - 1) s += c[2 * pitch + t] * (w * b[t + 1] * dinv[t + 1]);
+  if (j > 0) s += c[3 * pitch + t] * (w * b[t - nx] * dinv[t - nx]);
+  if (j < ny - 1) s += c[4 * pitch + t] * (w * b[t + nx] * dinv[t + nx]);
+  if (k > 0) s += c[5 * pitch + t] * (w * b[t - pl] * dinv[t - pl]);
+  if (k < nz - 1) s += c[6 * pitch + t] * (w * b[t + pl] * dinv[t + pl]);
+  return b[t] - s;
+}
+
+// Restriction to coarse point (i, j, k) of a level coarsened in direction d:
+// r(C) + w+(F-) r(F-) + w-(F+) r(F+), C = fine 2c+1.
+template <typename T>
+__device__ inline T restrict_point(int fnx, int fny, int fnz, int d, const T *__restrict__ r, const T *__restrict__ wm,
+                                   const T *__restrict__ wp, int i, int j, int k) {
+  const int fi = (d == 0) ? 2 * i + 1 : i, fj = (d == 1) ? 2 * j + 1 : j, fk = (d == 2) ? 2 * k + 1 : k;
+  const int pd = (d == 0) ? fi : (d == 1 ? fj : fk);
+  const int fnd = (d == 0) ? fnx : (d == 1 ? fny : fnz);
+  const long long stride = (d == 0) ? 1 : (d == 1 ? (long long)fnx : (long long)fnx * fny);
+  const long long f0 = ((long long)fk * fny + fj) * fnx + fi;
+  T v = r[f0] + wp[f0 - stride] * r[f0 - stride];
+  if (pd + 1 < fnd) v += wm[f0 + stride] * r[f0 + stride];
+  return v;
+}
+
+// x = w D^-1 b plus the interpolated coarse correction at fine point (i, j, k),
+// the level coarsened in direction D (C points inject, F points take the
+// weighted neighbours).
+template <typename T, int D>
+__device__ inline T corrected(const T *b, const T *dinv, const T *wm, const T *wp, const T *yc, T w, int i, int j,
+                              int k, int fnx, int fny, int cnx, int cny, int ncd) {
+  const long long t = ((long long)k * fny + j) * fnx + i;
+  T x = w * b[t] * dinv[t];
+  const int pd = (D == 0) ? i : (D == 1 ? j : k);
+  const long long cstride = (D == 0) ? 1 : (D == 1 ? (long long)cnx : (long long)cnx * cny);
+  const int q = (pd & 1) ? ((pd - 1) >> 1) : (pd >> 1);
+  const int ci = (D == 0) ? q : i, cj = (D == 1) ? q : j, ck = (D == 2) ? q : k;
+  const long long cr = ((long long)ck * cny + cj) * cnx + ci;
+  if (pd & 1) {
+    x += yc[cr];
+  } else {
+    T s = (T)0;
+    if (q - 1 >= 0) s += wm[t] * yc[cr - cstride];
+    if (q < ncd) s += wp[t] * yc[cr];
+    x += s;
+  }
+  return x;
+}
+
+// Prolongation + one weighted-Jacobi sweep at (i, j, k) = row t:
+// y = x + w D^-1 (b - A x) with x = corrected() at the point and its neighbours.
+template <typename T, int D>
+__device__ inline T up_point(const T *__restrict__ c, long long pitch, int nx, int ny, int nz,
+                             const T *__restrict__ dinv, const T *__restrict__ b, const T *__restrict__ wm,
+                             const T *__restrict__ wp, const T *__restrict__ yc, int cnx, int cny, int ncd, T w, int i,
+                             int j, int k, long long t) {
+#define FPC_CORR(ii, jj, kk) corrected<T, D>(b, dinv, wm, wp, yc, w, ii, jj, kk, nx, ny, cnx, cny, ncd)
+  T xc = FPC_CORR(i, j, k);
+  T s = c[t] * xc;
+  if (i > 0) s += c[pitch + t] * FPC_CORR(i - 1, j, k);
+  if (i < nx - 1) s += c[2 * pitch + t] * FPC_CORR(i + 1, j, k);
+  if (j > 0) s += c[3 * pitch + t] * FPC_CORR(i, j - 1, k);
+  if (j < ny - 1) s += c[4 * pitch + t] * FPC_CORR(i, j + 1, k);
+  if (k > 0) s += c[5 * pitch + t] * FPC_CORR(i, j, k - 1);
+  if (k < nz - 1) s += c[6 * pitch + t] * FPC_CORR(i, j, k + 1);
+#undef FPC_CORR
+  return xc + w * (b[t] - s) * dinv[t];
+}
+
+// Row r of the coarsest solve x = inv b (FP64 sums), one wavefront; valid in lane 0.
+template <typename T>
+__device__ inline double gemv_row(const double *__restrict__ inv, int n, const T *__restrict__ b, int r, int lane) {
+  const double *row = inv + (size_t)r * n;
+  double s = 0.0;
+  for (int q = lane; q < n; q += 64) s += row[q] * (double)b[q];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  return s;
+}
+
+}  // namespace mgd
+}  // namespace fpc

@@ -36,6 +36,8 @@
 #include <vector>
 
 #include "fpc_internal.hpp"
+#include "batched.hpp"
+#include "mg_device.hpp"
 #include "stencil_tile.hpp"
 
 namespace fpc {
@@ -338,14 +340,7 @@ __global__ void k_restrict(int fnx, int fny, int fnz, int d, const T *__restrict
   int j = blockIdx.y * blockDim.y + threadIdx.y;
   int k = blockIdx.z;
   if (i >= cnx || j >= cny) return;
-  int p[3] = {i, j, k};
-  int fn[3] = {fnx, fny, fnz};
-  p[d] = 2 * p[d] + 1;
-  long long stride = (d == 0) ? 1 : (d == 1 ? (long long)fnx : (long long)fnx * fny);
-  long long f0 = ((long long)p[2] * fny + p[1]) * fnx + p[0];
-  T v = r[f0] + wp[f0 - stride] * r[f0 - stride];
-  if (p[d] + 1 < fn[d]) v += wm[f0 + stride] * r[f0 + stride];
-  bc[((long long)k * cny + j) * cnx + i] = v;
+  bc[((long long)k * cny + j) * cnx + i] = mgd::restrict_point<T>(fnx, fny, fnz, d, r, wm, wp, i, j, k);
 }
 
 // x_f += P x_c, one thread per fine point.
@@ -388,36 +383,8 @@ __global__ void k_down_jacobi(const T *__restrict__ c, long long pitch, int nx, 
   int j = blockIdx.y * blockDim.y + threadIdx.y;
   int k = blockIdx.z;
   if (i >= nx || j >= ny) return;
-  long long t = ((long long)k * ny + j) * nx + i, pl = (long long)nx * ny;
-  T s = c[t] * (w * b[t] * dinv[t]);
-  if (i > 0) s += c[pitch + t] * (w * b[t - 1] * dinv[t - 1]);
-  if (i < nx - 1) s += c[2 * pitch + t] * (w * b[t + 1] * dinv[t + 1]);
-  if (j > 0) s += c[3 * pitch + t] * (w * b[t - nx] * dinv[t - nx]);
-  if (j < ny - 1) s += c[4 * pitch + t] * (w * b[t + nx] * dinv[t + nx]);
-  if (k > 0) s += c[5 * pitch + t] * (w * b[t - pl] * dinv[t - pl]);
-  if (k < nz - 1) s += c[6 * pitch + t] * (w * b[t + pl] * dinv[t + pl]);
-  r[t] = b[t] - s;
-}
-
-template <typename T, int D>
-__device__ inline T corrected(const T *b, const T *dinv, const T *wm, const T *wp, const T *yc, T w, int i, int j,
-                              int k, int fnx, int fny, int cnx, int cny, int ncd) {
-  const long long t = ((long long)k * fny + j) * fnx + i;
-  T x = w * b[t] * dinv[t];
-  const int pd = (D == 0) ? i : (D == 1 ? j : k);
-  const long long cstride = (D == 0) ? 1 : (D == 1 ? (long long)cnx : (long long)cnx * cny);
-  const int q = (pd & 1) ? ((pd - 1) >> 1) : (pd >> 1);
-  const int ci = (D == 0) ? q : i, cj = (D == 1) ? q : j, ck = (D == 2) ? q : k;
-  const long long cr = ((long long)ck * cny + cj) * cnx + ci;
-  if (pd & 1) {
-    x += yc[cr];
-  } else {
-    T s = (T)0;
-    if (q - 1 >= 0) s += wm[t] * yc[cr - cstride];
-    if (q < ncd) s += wp[t] * yc[cr];
-    x += s;
-  }
-  return x;
+  long long t = ((long long)k * ny + j) * nx + i;
+  r[t] = mgd::down_point<T>(c, pitch, nx, ny, nz, dinv, b, w, i, j, k, t);
 }
 
 template <typename T, int D>
@@ -431,17 +398,7 @@ __global__ void k_up_jacobi(const T *__restrict__ c, long long pitch, int nx, in
   int k = blockIdx.z;
   if (i >= nx || j >= ny) return;
   long long t = ((long long)k * ny + j) * nx + i;
-#define FPC_CORR(ii, jj, kk) corrected<T, D>(b, dinv, wm, wp, yc, w, ii, jj, kk, nx, ny, cnx, cny, ncd)
-  T xc = FPC_CORR(i, j, k);
-  T s = c[t] * xc;
-  if (i > 0) s += c[pitch + t] * FPC_CORR(i - 1, j, k);
-  if (i < nx - 1) s += c[2 * pitch + t] * FPC_CORR(i + 1, j, k);
-  if (j > 0) s += c[3 * pitch + t] * FPC_CORR(i, j - 1, k);
-  if (j < ny - 1) s += c[4 * pitch + t] * FPC_CORR(i, j + 1, k);
-  if (k > 0) s += c[5 * pitch + t] * FPC_CORR(i, j, k - 1);
-  if (k < nz - 1) s += c[6 * pitch + t] * FPC_CORR(i, j, k + 1);
-#undef FPC_CORR
-  y[t] = xc + w * (b[t] - s) * dinv[t];
+  y[t] = mgd::up_point<T, D>(c, pitch, nx, ny, nz, dinv, b, wm, wp, yc, cnx, cny, ncd, w, i, j, k, t);
 }
 
 // x = Ainv b on the coarsest level (FP64 inverse), one block.
@@ -621,10 +578,7 @@ __global__ void __launch_bounds__(256)
   const int lane = threadIdx.x & 63;
   const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (r >= n) return;
-  const double *row = inv + (size_t)r * n;
-  double s = 0.0;
-  for (int q = lane; q < n; q += 64) s += row[q] * (double)b[q];
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  const double s = mgd::gemv_row<T>(inv, n, b, r, lane);
   if (lane == 0) x[r] = (T)s;
 }
 
@@ -971,6 +925,42 @@ int pfmg_setup(void *pf, fpc_StructMatrix_s *A) {
   return build(P, A);
 }
 
+int pfmg_batch_view(void *pf, std::vector<MgLevelView> &levels, const double **inv, int *ncoarse) {
+  PfmgImpl *P = (PfmgImpl *)pf;
+  if (!P || P->L.empty()) {
+    set_error(FPC_ERROR_ARG, "batched solve: PFMG used before setup");
+    return FPC_ERROR_ARG;
+  }
+  const bool fused = P->relax < 2 && P->npre == 1 && P->npost == 1;
+  if (P->built_bits != 64 || !fused || P->maxit != 1) {
+    set_error(FPC_ERROR_ARG,
+              "batched solve: only the FP64 V-cycle with one (weighted) Jacobi sweep each way, one cycle, is supported");
+    return FPC_ERROR_ARG;
+  }
+  levels.clear();
+  for (const DLevel &L : P->L) {
+    MgLevelView v;
+    v.nx = L.d.nx;
+    v.ny = L.d.ny;
+    v.nz = L.d.nz;
+    v.N = L.d.N;
+    v.cdir = L.cdir;
+    v.w = (P->relax == 0) ? 1.0 : L.omega;
+    v.pitch = L.pitch;
+    v.coef = (const double *)L.coef;
+    v.dinv = (const double *)L.dinv;
+    v.wm = (const double *)L.wm;
+    v.wp = (const double *)L.wp;
+    v.b = (double *)L.b;
+    v.r = (double *)L.r;
+    v.y = (double *)L.y;
+    levels.push_back(v);
+  }
+  *inv = P->inv;
+  *ncoarse = P->ncoarse;
+  return 0;
+}
+
 int pfmg_apply(void *pf, const double *b, double *x, const int *skip) {
   PfmgImpl *P = (PfmgImpl *)pf;
   if (P->L.empty()) {
@@ -1064,7 +1054,7 @@ int fpc_StructPFMGSolve(fpc_StructSolver s, fpc_StructMatrix A, fpc_StructVector
   if (!part) return FPC_ERROR_MEMORY;
   double bnorm2 = 0.0;
   if (P->tol > 0.0) {
-    if ((rc = dot_partials(b->data, b->data, n, part)) || (rc = reduce_partials(part, kRedBlocks, part + kRedBlocks)))
+    if ((rc = dot_partials(b->data, b->data, n, part)) || (rc = reduce_partials(part, red_blocks(n), part + kRedBlocks)))
       return rc;
     FPC_HIP(hipMemcpyAsync(&bnorm2, part + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, stream()));
     FPC_HIP(hipStreamSynchronize(stream()));
@@ -1083,7 +1073,7 @@ int fpc_StructPFMGSolve(fpc_StructSolver s, fpc_StructMatrix A, fpc_StructVector
     P->its = it + 1;
     if (P->tol > 0.0) {
       if ((rc = residual_norm_partials(A, b->data, x->data, nullptr, part)) ||
-          (rc = reduce_partials(part, kRedBlocks, part + kRedBlocks)))
+          (rc = reduce_partials(part, red_blocks(n), part + kRedBlocks)))
         return rc;
       double r2 = 0.0;
       FPC_HIP(hipMemcpyAsync(&r2, part + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, stream()));

@@ -50,10 +50,12 @@ struct Nbr {
 // its point function in two so loads run ahead of use: op.fetch(i, j, k) issues
 // the loads (an Op::Raw) and op.finish(raw) does the arithmetic; op.row(i, j, k)
 // loads the point's own data (an Op::Row) and op.out(row, i, j, k, nb) consumes
-// the neighbour values.  Loads for plane k+1 (own row, halo ring) and for the
-// own column at k+2 are issued while plane k is computed; a barrier waits only
-// for LDS (s_waitcnt lgkmcnt), so they stay in flight across it.  Points
-// outside the box read as zero.
+// the neighbour values.  The own row of plane k is loaded at the top of its
+// iteration, and the halo ring of plane k+1 and the own column at k+2 are
+// issued then too; a barrier waits only for LDS (s_waitcnt lgkmcnt), so all of
+// them stay in flight across it.  (Loading the row a plane ahead as well costs
+// 16-22 more VGPRs and one wave per SIMD of occupancy, and measured slower.)
+// Points outside the box read as zero.
 template <typename T, int TX, int TY, class Op>
 __device__ inline void march_tile(Op &op, int nx, int ny, int nz, int k0, int kend) {
   using Raw = typename Op::Raw;

@@ -14,6 +14,7 @@
 #include <mutex>
 
 #include "fpc_internal.hpp"
+#include "krylov_device.hpp"
 #include "stencil_tile.hpp"
 
 namespace fpc {
@@ -242,45 +243,16 @@ __global__ void k_matvec(const double *__restrict__ c, long long pitch, int nx, 
 __global__ void __launch_bounds__(256) k_residual(const double *__restrict__ c, long long pitch, int nx, int ny, int nz,
                                                   const double *__restrict__ b, const double *__restrict__ x,
                                                   double *__restrict__ r, double *__restrict__ part) {
-  __shared__ double lds[8];
-  long long N = (long long)nx * ny * nz, pl = (long long)nx * ny;
-  double acc = 0.0;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < N; t += (long long)gridDim.x * blockDim.x) {
-    int i = (int)(t % nx);
-    long long q = t / nx;
-    int j = (int)(q % ny);
-    int k = (int)(q / ny);
-    double s = c[t] * x[t];
-    if (i > 0) s += c[pitch + t] * x[t - 1];
-    if (i < nx - 1) s += c[2 * pitch + t] * x[t + 1];
-    if (j > 0) s += c[3 * pitch + t] * x[t - nx];
-    if (j < ny - 1) s += c[4 * pitch + t] * x[t + nx];
-    if (k > 0) s += c[5 * pitch + t] * x[t - pl];
-    if (k < nz - 1) s += c[6 * pitch + t] * x[t + pl];
-    double rv = b[t] - s;
-    if (r) r[t] = rv;
-    acc += rv * rv;
-  }
-  double tot = block_sum<256>(acc, lds);
-  if (threadIdx.x == 0) part[blockIdx.x] = tot;
+  kry::residual_body(c, pitch, nx, ny, nz, b, x, r, part, blockIdx.x, gridDim.x);
 }
 
 __global__ void __launch_bounds__(256) k_dot(const double *__restrict__ a, const double *__restrict__ b, long long n,
                                              double *__restrict__ part) {
-  __shared__ double lds[8];
-  double acc = 0.0;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
-    acc += a[t] * b[t];
-  double tot = block_sum<256>(acc, lds);
-  if (threadIdx.x == 0) part[blockIdx.x] = tot;
+  kry::dot_body(a, b, n, part, blockIdx.x, gridDim.x);
 }
 
 __global__ void __launch_bounds__(1024) k_reduce(const double *__restrict__ part, int n, double *__restrict__ out) {
-  __shared__ double lds[16];
-  double acc = 0.0;
-  for (int t = threadIdx.x; t < n; t += blockDim.x) acc += part[t];
-  double tot = block_sum<1024>(acc, lds);
-  if (threadIdx.x == 0) *out = tot;
+  kry::reduce_body(part, n, out);
 }
 
 unsigned blocks_for(long long n) {
@@ -325,13 +297,13 @@ int matvec(double alpha, const fpc_StructMatrix_s *A, const double *x, double be
 
 int residual_norm_partials(const fpc_StructMatrix_s *A, const double *b, const double *x, double *r, double *part) {
   Dims d = dims_of(A->grid);
-  k_residual<<<kRedBlocks, 256, 0, stream()>>>(A->coef, A->pitch, d.nx, d.ny, d.nz, b, x, r, part);
+  k_residual<<<red_blocks(d.N), 256, 0, stream()>>>(A->coef, A->pitch, d.nx, d.ny, d.nz, b, x, r, part);
   FPC_LAUNCH_CHECK();
   return 0;
 }
 
 int dot_partials(const double *a, const double *b, long long n, double *part) {
-  k_dot<<<kRedBlocks, 256, 0, stream()>>>(a, b, n, part);
+  k_dot<<<red_blocks(n), 256, 0, stream()>>>(a, b, n, part);
   FPC_LAUNCH_CHECK();
   return 0;
 }

@@ -378,8 +378,9 @@ def ensemble_workload(n=64, seed=0, nx=(64, 256), nyz=(16, 48), xc=(0.4, 0.7), y
 
 
 class Ensemble:
-    """Independent channels stepped concurrently, one HIP stream per instance and a
-    pool of host worker threads (fpc_ensemble_*)."""
+    """Independent channels (fpc_ensemble_*): stepped concurrently, one HIP stream per
+    instance and a pool of `nthreads` host worker threads, or, with nthreads=0,
+    batched on one stream with one launch per stage over every instance."""
 
     def __init__(self, params_list, nthreads=8):
         self.params = list(params_list)

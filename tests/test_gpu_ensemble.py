@@ -1,6 +1,7 @@
-"""GPU: an ensemble of uneven channels stepped concurrently (one HIP stream per
-instance, several host worker threads) computes exactly what each instance
-computes alone, and its packed offset table lays the fields end to end."""
+"""GPU: an ensemble of uneven channels, stepped concurrently (one HIP stream per
+instance, several host worker threads) or batched (one stream, one launch per
+stage over every instance), computes exactly what each instance computes
+alone, and its packed offset table lays the fields end to end."""
 import numpy as np
 import pytest
 
@@ -14,16 +15,37 @@ def _params():
     return F.ensemble_workload(n=6, seed=3, nx=(24, 48), nyz=(8, 14), inlet_ramp_steps=4)
 
 
-def test_ensemble_equals_instances_run_alone(torch_cuda):
+@pytest.mark.parametrize("nthreads", [3, 0], ids=["concurrent", "batched"])
+def test_ensemble_equals_instances_run_alone(torch_cuda, nthreads):
     pars = _params()
-    ens = F.Ensemble(pars, nthreads=3)
+    ens = F.Ensemble(pars, nthreads=nthreads)
     ens.step(3)
     for i, p in enumerate(pars):
         alone = F.Sim(p)
         alone.step(3)
         for f in ("u", "v", "w", "p"):
             assert bits_equal(ens.members[i].get(f), alone.get(f)), (i, f)
-        assert ens.members[i].scalar("cd") == alone.scalar("cd")
+        for k in ("its_u", "its_v", "its_w", "its_p", "relres_p", "cd", "cl", "div_l2"):
+            assert ens.members[i].scalar(k) == alone.scalar(k), (i, k)
+        alone.close()
+    ens.close()
+
+
+def test_batched_ensemble_restarts_and_tight_tolerance(torch_cuda):
+    """Short restart length (kdim 4) and rtol 1e-12: every system restarts several
+    times at its own pace, and the batch still reproduces each instance bitwise."""
+    pars = F.ensemble_workload(n=5, seed=8, nx=(24, 40), nyz=(8, 12), inlet_ramp_steps=3, vel_kdim=4, p_kdim=4,
+                               vel_tol=1e-12, p_tol=1e-12, vel_maxit=400, p_maxit=400)
+    ens = F.Ensemble(pars, nthreads=0)
+    ens.step(2)
+    for i, p in enumerate(pars):
+        alone = F.Sim(p)
+        alone.step(2)
+        assert alone.scalar("its_p") > 4
+        for f in ("u", "v", "w", "p"):
+            assert bits_equal(ens.members[i].get(f), alone.get(f)), (i, f)
+        for k in ("its_u", "its_v", "its_w", "its_p"):
+            assert ens.members[i].scalar(k) == alone.scalar(k), (i, k)
         alone.close()
     ens.close()
 
