@@ -91,9 +91,14 @@ int fpc_sim_destroy(fpc_sim sim);
 /* ---- ensemble of independent channels (no counterpart in the reference, which
  * runs one channel per process) ----
  * n instances with their own parameters (uneven grids, cylinders, Reynolds
- * numbers), each on its own HIP stream, created and stepped concurrently by a
- * pool of nthreads host worker threads (largest instance first).  Every
- * instance computes exactly what it computes alone.  Kernel timing
+ * numbers).  nthreads >= 1: each instance on its own HIP stream, created and
+ * stepped concurrently by a pool of nthreads host worker threads (largest
+ * instance first).  nthreads = 0: batched on one stream (at most 256
+ * instances): the momentum solves of all instances and their pressure solves
+ * each advance with one launch per solver kernel and V-cycle level over every
+ * instance; needs the reference solver settings (Jacobi with one sweep, the
+ * FP64 PFMG-style V-cycle with one sweep each way and one cycle).  Either way
+ * every instance computes exactly what it computes alone.  Kernel timing
  * (fpc_ProfileEnable) must be off while an ensemble steps. */
 typedef struct fpc_ensemble_s *fpc_ensemble;
 int fpc_ensemble_create(const fpc_params *pars, int n, int nthreads, fpc_ensemble *ens);
