@@ -690,13 +690,17 @@ void box_matvec(const Box &B, const double *x, double *y) {
 //  * relaxation: weighted Jacobi with w_l = 2/(2 + c_d/sum c) (type 1), plain
 //    Jacobi (type 0), red/black Gauss-Seidel (types 2: symmetric, 3: forward);
 //    one pre and one post sweep;
-//  * coarsening stops at <= 64 points or when no direction has 2+ points; the
-//    coarsest system is solved exactly (LU with partial pivoting).  (Stopping at
-//    ~1000 points instead, with the same exact solve, needs 93 rather than 20
-//    GMRES iterations on the 48x16x12 pressure box: the non-Galerkin operator of
-//    such a level misrepresents its smoothest modes, which an exact solve
-//    amplifies and further smoothing levels damp.)
-const int MG_COARSE_MAX = 64;
+//  * coarsening stops when a level has at most coarse_max(N0) points, N0 the
+//    finest level's, or when no direction has 2+ points; the coarsest system is
+//    solved exactly (LU with partial pivoting).  coarse_max(N0) =
+//    min(1152, max(64, N0 / 2048)): on the production pressure box (512x96x96)
+//    stopping at <= 1152 points takes 13 instead of 18 levels with the same 21
+//    GMRES iterations (256x48x48 at <= 1200: 18 and 18), while small boxes need
+//    the deep hierarchy (48x16x12 stopped at ~1000 points: 57 instead of 17
+//    iterations).  Boxes up to 128x32x32 keep 64: there a larger exact solve
+//    would put its own rounding (~cond * eps of the coarsest matrix) at the
+//    level of a 1e-12 GMRES tolerance.
+inline size_t mg_coarse_max(size_t n0) { return std::min<size_t>(1152, std::max<size_t>(64, n0 / 2048)); }
 
 struct MGLevel {
   Box A;
@@ -886,12 +890,13 @@ void mg_setup(MG &M, const Box &A0, int relax_type, int max_levels) {
   M.L.clear();
   M.L.emplace_back();
   M.L.back().A = A0;
+  const size_t cmax = mg_coarse_max(A0.n());
   for (;;) {
     MGLevel &Lv = M.L.back();
     mg_choose(Lv.A, Lv.cdir, Lv.omega);
     size_t N = Lv.A.n();
     Lv.x.assign(N, 0.0); Lv.b.assign(N, 0.0); Lv.r.assign(N, 0.0);
-    if (N <= (size_t)MG_COARSE_MAX || Lv.cdir < 0 || (max_levels > 0 && (int)M.L.size() >= max_levels)) {
+    if (N <= cmax || Lv.cdir < 0 || (max_levels > 0 && (int)M.L.size() >= max_levels)) {
       Lv.cdir = -1;
       break;
     }

@@ -363,9 +363,15 @@ def test_two_level_cycle_matches_numpy_definition(dims, h):
     assert rel_l2(got, want) < 1e-12
 
 
-def test_vcycle_converges_on_poisson():
-    nx, ny, nz = 32, 16, 16
+@pytest.mark.parametrize("dims", [(32, 16, 16), (128, 64, 64)])
+def test_vcycle_converges_on_poisson(dims):
+    # 128x64x64: the hierarchy stops at <= 256 points (N0 / 2048), not 64
+    nx, ny, nz = dims
     vals = _poisson(nx, ny, nz)
+    lv = O.mg_levels(nx, ny, nz, vals)
+    n_last = int(np.prod(lv[-1][0]))
+    cmax = min(1152, max(64, nx * ny * nz // 2048))
+    assert n_last <= cmax and int(np.prod(lv[-2][0])) > cmax
     A = box_csr(nx, ny, nz, vals)
     b = np.random.default_rng(6).standard_normal(A.shape[0])
     x = np.zeros_like(b)
