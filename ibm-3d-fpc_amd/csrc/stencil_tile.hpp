@@ -40,11 +40,19 @@ __device__ inline bool tile_ring(int q, int &ox, int &oy) {
   return false;
 }
 
-// Neighbour values of one point, in the stencil's slot order after the centre.
+// Neighbour values of one point, in the stencil's slot order after the centre;
+// bit s-1 of in_box is set when neighbour s (-x, +x, -y, +y, -z, +z) lies in
+// the box.
 template <typename T>
 struct Nbr {
   T c, xm, xp, ym, yp, zm, zp;
+  unsigned in_box;
 };
+
+__device__ inline unsigned nbr_in_box(int i, int j, int k, int nx, int ny, int nz) {
+  return (unsigned)(i > 0) | ((unsigned)(i < nx - 1) << 1) | ((unsigned)(j > 0) << 2) | ((unsigned)(j < ny - 1) << 3) |
+         ((unsigned)(k > 0) << 4) | ((unsigned)(k < nz - 1) << 5);
+}
 
 // Walk planes k0 .. kend-1 of the tile owned by this workgroup.  The op splits
 // its point function in two so loads run ahead of use: op.fetch(i, j, k) issues
@@ -85,7 +93,8 @@ __device__ inline void march_tile(Op &op, int nx, int ny, int nz, int k0, int ke
     if (ring) P[hy + 1][hx + 1] = ring_in ? op.finish(ring0) : T(0);
     __syncthreads();
     if (in) {
-      Nbr<T> nb{zc, P[ty + 1][tx], P[ty + 1][tx + 2], P[ty][tx + 1], P[ty + 2][tx + 1], zm, zp};
+      Nbr<T> nb{zc, P[ty + 1][tx], P[ty + 1][tx + 2], P[ty][tx + 1], P[ty + 2][tx + 1], zm, zp,
+                nbr_in_box(i, j, k, nx, ny, nz)};
       op.out(row, i, j, k, nb);
     }
     zm = zc;
@@ -96,7 +105,10 @@ __device__ inline void march_tile(Op &op, int nx, int ny, int nz, int k0, int ke
 }
 
 // The seven coefficients of a row, loaded ahead of the plane barrier; apply()
-// sums in slot order C, -x, +x, -y, +y, -z, +z (out-of-box neighbours are zero).
+// sums in slot order C, -x, +x, -y, +y, -z, +z over the neighbours in the box.
+// Terms of absent neighbours are skipped rather than added as a*0, so the sum
+// is bit for bit that of the one-point-per-thread kernels (a*0 would turn a -0
+// partial sum into +0).
 template <typename T>
 struct Coef7 {
   T a[7];
@@ -106,12 +118,12 @@ struct Coef7 {
   }
   __device__ inline T apply(const Nbr<T> &nb) const {
     T s = a[0] * nb.c;
-    s += a[1] * nb.xm;
-    s += a[2] * nb.xp;
-    s += a[3] * nb.ym;
-    s += a[4] * nb.yp;
-    s += a[5] * nb.zm;
-    s += a[6] * nb.zp;
+    if (nb.in_box & 1u) s += a[1] * nb.xm;
+    if (nb.in_box & 2u) s += a[2] * nb.xp;
+    if (nb.in_box & 4u) s += a[3] * nb.ym;
+    if (nb.in_box & 8u) s += a[4] * nb.yp;
+    if (nb.in_box & 16u) s += a[5] * nb.zm;
+    if (nb.in_box & 32u) s += a[6] * nb.zp;
     return s;
   }
 };

@@ -13,7 +13,7 @@ import pytest
 
 import fpc_amd as F
 import oracle as O
-from helpers import box_csr, rel_l2
+from helpers import bits_equal, box_csr, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -85,3 +85,31 @@ def test_tiled_matvec_and_gmres_match_oracle(torch_cuda, dims, scale, cdir):
         assert abs(its_g - its_o) <= 1, (pc, its_g, its_o)
         assert rel_l2(x_g, x_o) < 1e-9, pc
         assert np.linalg.norm(b - A @ x_g) / np.linalg.norm(b) < 2e-11, pc
+
+
+@pytest.mark.parametrize("dims,scale,cdir", CASES, ids=["z-odd", "z-even", "y", "x"])
+def test_tiled_equal_one_point_bitwise(torch_cuda, dims, scale, cdir):
+    """The z-marching kernels and the one-point-per-thread kernels give the same bits:
+    one V-cycle (every level tiled or none), and GMRES with the fused Jacobi matvec
+    and with the V-cycle.  Out-of-box coefficients are left non-zero: both forms
+    must skip them, not add them times zero."""
+    nx, ny, nz = dims
+    v = _aniso(nx, ny, nz, scale, seed=nz + 7).reshape(nz, ny, nx, 7)
+    rng = np.random.default_rng(9)
+    v[:, :, 0, 1] = -0.3 - rng.random((nz, ny))
+    v[:, -1, :, 4] = -0.2 - rng.random((nz, nx))
+    v[0, :, :, 5] = -0.1 - rng.random((ny, nx))
+    vals = v.reshape(-1)
+    b = rng.standard_normal(nx * ny * nz)
+    box = F.StructBox(nx, ny, nz, vals)
+    out = {}
+    for mode, tmin in (("tiled", 0), ("one-point", 1 << 40)):
+        F.set_tuning("tile_min_points", tmin)
+        y = F.pfmg_apply(box, b)
+        xp, itp, _, _ = F.Solver(box, tol=1e-10, maxit=200, kdim=30, precond="pfmg").solve(b)
+        xj, itj, _, _ = F.Solver(box, tol=1e-10, maxit=400, kdim=30, precond="jacobi").solve(b)
+        out[mode] = (y, xp, itp, xj, itj)
+    t, o = out["tiled"], out["one-point"]
+    assert bits_equal(t[0], o[0])
+    assert t[2] == o[2] and bits_equal(t[1], o[1])
+    assert t[4] == o[4] and bits_equal(t[3], o[3])
