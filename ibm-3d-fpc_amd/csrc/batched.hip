@@ -31,6 +31,7 @@
 #include "fpc_internal.hpp"
 #include "krylov_device.hpp"
 #include "mg_device.hpp"
+#include "stencil_tile.hpp"
 
 namespace fpc {
 namespace {
@@ -80,6 +81,23 @@ struct Table {
   int *item = nullptr, *first = nullptr;
   int nblocks = 0;
 };
+
+// One workgroup of a z-marching launch: tile (bx, by) = (bxy & 0xffff, bxy >> 16)
+// of item, planes k0 .. k1-1.
+struct TileEnt {
+  int item, bxy, k0, k1;
+};
+
+struct TileTable {
+  TileEnt *ent = nullptr;
+  int nblocks = 0;
+};
+
+// Items of at least this many points (and nx >= 32, ny >= 8) run the z-marching
+// kernels inside a batch: with many systems per launch the tiles never lack
+// workgroups, unlike one small box alone.
+constexpr long long kBatchTileMin = 16384;
+constexpr int kTY = 8;
 
 __device__ inline void ijk_of(long long t, int nx, int ny, int &i, int &j, int &k) {
   i = (int)(t % nx);
@@ -293,6 +311,60 @@ __global__ void __launch_bounds__(256) k_b_up(const BLev *lev, const BSys *sys, 
   (L.lvl == 0 ? s.Z : L.y)[t] = v;
 }
 
+// ------------------------------------------------------------ z-marching forms
+template <bool kDinv>
+__global__ void __launch_bounds__(kTileX *kTY) k_bt_matvec(const BSys *sys, const TileEnt *tab, int use_z) {
+  const TileEnt e = tab[blockIdx.x];
+  const BSys &s = sys[e.item];
+  State *S = s.S;
+  if (S->st.done) return;
+  const int cur = S->st.i;
+  kry::ArnoldiOp<kDinv> op{s.c, s.pitch, s.nx, s.ny, use_z ? s.Z : s.V + (long long)cur * s.vp, s.dinv,
+                           s.V + (long long)(cur + 1) * s.vp, S->sigma[cur]};
+  march_tile<double, kTileX, kTY>(op, s.nx, s.ny, s.nz, e.k0, e.k1, e.bxy & 0xffff, e.bxy >> 16);
+}
+
+__global__ void __launch_bounds__(kTileX *kTY) k_bt_down(const BLev *lev, const BSys *sys, const TileEnt *tab,
+                                                         int in_mode, int step, const int *flags) {
+  const TileEnt e = tab[blockIdx.x];
+  const BLev &L = lev[e.item];
+  const BSys &s = sys[L.sys];
+  if (lev_skip(L, s, flags)) return;
+  const double *b = lev_b(L, s, in_mode, step);
+  const int bx = e.bxy & 0xffff, by = e.bxy >> 16;
+  if (L.cdir == 2) {
+    // residual restricted in z on the fly (no separate restriction)
+    mgd::DownRestrictZOp<double> op;
+    op.c = L.c; op.pitch = L.pitch; op.nx = L.nx; op.ny = L.ny; op.dinv = L.dinv; op.b = b; op.r = nullptr;
+    op.w = L.w; op.wm = L.wm; op.wp = L.wp; op.bc = L.bc; op.nz = L.nz; op.ncz = L.cnz; op.k0 = e.k0;
+    march_tile<double, kTileX, kTY>(op, L.nx, L.ny, L.nz, e.k0, e.k1 < L.nz ? e.k1 + 1 : e.k1, bx, by);
+  } else {
+    mgd::DownOp<double> op{L.c, L.pitch, L.nx, L.ny, L.dinv, b, L.r, L.w};
+    march_tile<double, kTileX, kTY>(op, L.nx, L.ny, L.nz, e.k0, e.k1, bx, by);
+  }
+}
+
+__global__ void __launch_bounds__(kTileX *kTY) k_bt_up(const BLev *lev, const BSys *sys, const TileEnt *tab,
+                                                       int in_mode, int step, const int *flags) {
+  const TileEnt e = tab[blockIdx.x];
+  const BLev &L = lev[e.item];
+  const BSys &s = sys[L.sys];
+  if (lev_skip(L, s, flags)) return;
+  const double *b = lev_b(L, s, in_mode, step);
+  double *y = L.lvl == 0 ? s.Z : L.y;
+  const int bx = e.bxy & 0xffff, by = e.bxy >> 16;
+  if (L.cdir == 0) {
+    mgd::UpOp<double, 0> op{L.c, L.pitch, L.nx, L.ny, L.dinv, b, L.wm, L.wp, L.yc, L.cnx, L.cny, L.ncd, y, L.w};
+    march_tile<double, kTileX, kTY>(op, L.nx, L.ny, L.nz, e.k0, e.k1, bx, by);
+  } else if (L.cdir == 1) {
+    mgd::UpOp<double, 1> op{L.c, L.pitch, L.nx, L.ny, L.dinv, b, L.wm, L.wp, L.yc, L.cnx, L.cny, L.ncd, y, L.w};
+    march_tile<double, kTileX, kTY>(op, L.nx, L.ny, L.nz, e.k0, e.k1, bx, by);
+  } else {
+    mgd::UpOp<double, 2> op{L.c, L.pitch, L.nx, L.ny, L.dinv, b, L.wm, L.wp, L.yc, L.cnx, L.cny, L.ncd, y, L.w};
+    march_tile<double, kTileX, kTY>(op, L.nx, L.ny, L.nz, e.k0, e.k1, bx, by);
+  }
+}
+
 // ------------------------------------------------------------ non-finite check
 __global__ void k_b_nonfinite(const double *const *arr, const long long *len, int *flags) {
   __shared__ int any;
@@ -335,6 +407,35 @@ int make_table(const std::vector<int> &items, const std::vector<long long> &coun
   return rc;
 }
 
+bool tiled_box(int nx, int ny, long long N) { return nx >= 32 && ny >= 8 && N >= kBatchTileMin; }
+
+// z-chunks of at most 4 planes (short chunks: more workgroups per box; measured
+// fastest of 4, 8 and 16 on the 64-channel ensemble), even length (z-restriction
+// pairs stay in a chunk)
+int tile_chunk(int nz) {
+  const int nch = (nz + 3) / 4;
+  int kz = (nz + nch - 1) / nch;
+  return kz + (kz & 1);
+}
+
+int make_tile_table(const std::vector<int> &items, const std::vector<int> &nx, const std::vector<int> &ny,
+                    const std::vector<int> &nz, TileTable &t) {
+  std::vector<TileEnt> ent;
+  for (size_t e = 0; e < items.size(); ++e) {
+    const int tx = (nx[e] + kTileX - 1) / kTileX, ty = (ny[e] + kTY - 1) / kTY, kz = tile_chunk(nz[e]);
+    for (int k0 = 0; k0 < nz[e]; k0 += kz)
+      for (int by = 0; by < ty; ++by)
+        for (int bx = 0; bx < tx; ++bx) ent.push_back(TileEnt{items[e], bx | (by << 16), k0, std::min(k0 + kz, nz[e])});
+  }
+  t.nblocks = (int)ent.size();
+  return dupload(&t.ent, ent);
+}
+
+void free_tiles(TileTable &t) {
+  if (t.ent) (void)hipFree(t.ent);
+  t = TileTable{};
+}
+
 void free_table(Table &t) {
   if (t.item) (void)hipFree(t.item);
   if (t.first) (void)hipFree(t.first);
@@ -352,11 +453,13 @@ class BatchGroup {
   int pc = 0;
   std::vector<GmresBatchView> view;
   BSys *dsys = nullptr;
-  Table pts;
+  Table pts;          // one-point matvec over the small systems
+  TileTable ptsT;     // z-marching matvec over the others
   // V-cycle: per level l, the items (fine levels l of systems deeper than l+1)
   std::vector<BLev> hlev;
   BLev *dlev = nullptr;
-  std::vector<Table> down, restr;
+  std::vector<Table> down, restr;  // one-point down/up, restriction
+  std::vector<TileTable> downT;    // z-marching down/up
   Table coarse;
   int *flags = nullptr;
   // host staging (pinned)
@@ -371,7 +474,9 @@ class BatchGroup {
     if (dsys) (void)hipFree(dsys);
     if (dlev) (void)hipFree(dlev);
     free_table(pts);
+    free_tiles(ptsT);
     for (auto &t : down) free_table(t);
+    for (auto &t : downT) free_tiles(t);
     for (auto &t : restr) free_table(t);
     free_table(coarse);
     if (flags) (void)hipFree(flags);
@@ -408,21 +513,34 @@ class BatchGroup {
   int vcycle(int in_mode, int step, const int *fl) {
     const int nl = (int)down.size();
     for (int l = 0; l < nl; ++l) {
-      if (down[(size_t)l].nblocks == 0) continue;
-      k_b_down<<<down[(size_t)l].nblocks, 256, 0, stream()>>>(dlev, dsys, down[(size_t)l].item, down[(size_t)l].first,
-                                                              in_mode, step, fl);
-      FPC_LAUNCH_CHECK();
-      k_b_restrict<<<restr[(size_t)l].nblocks, 256, 0, stream()>>>(dlev, dsys, restr[(size_t)l].item,
-                                                                   restr[(size_t)l].first, fl);
-      FPC_LAUNCH_CHECK();
+      const Table &d = down[(size_t)l], &r = restr[(size_t)l];
+      const TileTable &dt = downT[(size_t)l];
+      if (d.nblocks) {
+        k_b_down<<<d.nblocks, 256, 0, stream()>>>(dlev, dsys, d.item, d.first, in_mode, step, fl);
+        FPC_LAUNCH_CHECK();
+      }
+      if (dt.nblocks) {
+        k_bt_down<<<dt.nblocks, dim3(kTileX, kTY), 0, stream()>>>(dlev, dsys, dt.ent, in_mode, step, fl);
+        FPC_LAUNCH_CHECK();
+      }
+      if (r.nblocks) {
+        k_b_restrict<<<r.nblocks, 256, 0, stream()>>>(dlev, dsys, r.item, r.first, fl);
+        FPC_LAUNCH_CHECK();
+      }
     }
     k_b_gemv<<<coarse.nblocks, 256, 0, stream()>>>(dlev, dsys, coarse.item, coarse.first, in_mode, step, fl);
     FPC_LAUNCH_CHECK();
     for (int l = nl - 1; l >= 0; --l) {
-      if (down[(size_t)l].nblocks == 0) continue;
-      k_b_up<<<down[(size_t)l].nblocks, 256, 0, stream()>>>(dlev, dsys, down[(size_t)l].item, down[(size_t)l].first,
-                                                            in_mode, step, fl);
-      FPC_LAUNCH_CHECK();
+      const Table &d = down[(size_t)l];
+      const TileTable &dt = downT[(size_t)l];
+      if (d.nblocks) {
+        k_b_up<<<d.nblocks, 256, 0, stream()>>>(dlev, dsys, d.item, d.first, in_mode, step, fl);
+        FPC_LAUNCH_CHECK();
+      }
+      if (dt.nblocks) {
+        k_bt_up<<<dt.nblocks, dim3(kTileX, kTY), 0, stream()>>>(dlev, dsys, dt.ent, in_mode, step, fl);
+        FPC_LAUNCH_CHECK();
+      }
     }
     return 0;
   }
@@ -433,8 +551,17 @@ class BatchGroup {
       int rc = vcycle(0, step, nullptr);
       if (rc) return rc;
     }
-    k_b_matvec<<<pts.nblocks, 256, 0, stream()>>>(dsys, pts.item, pts.first, pc == 3 ? 1 : 0);
-    FPC_LAUNCH_CHECK();
+    if (pts.nblocks) {
+      k_b_matvec<<<pts.nblocks, 256, 0, stream()>>>(dsys, pts.item, pts.first, pc == 3 ? 1 : 0);
+      FPC_LAUNCH_CHECK();
+    }
+    if (ptsT.nblocks) {
+      if (pc == 1)
+        k_bt_matvec<true><<<ptsT.nblocks, dim3(kTileX, kTY), 0, stream()>>>(dsys, ptsT.ent, 0);
+      else
+        k_bt_matvec<false><<<ptsT.nblocks, dim3(kTileX, kTY), 0, stream()>>>(dsys, ptsT.ent, pc == 3 ? 1 : 0);
+      FPC_LAUNCH_CHECK();
+    }
     k_b_multidot<<<dim3(maxnred, n), 256, 0, stream()>>>(dsys);
     FPC_LAUNCH_CHECK();
     k_b_reduce_h<<<dim3(std::min(step + 1, KM + 1), n), 256, 0, stream()>>>(dsys);
@@ -458,8 +585,6 @@ int batch_group_create(const std::vector<fpc_StructSolver> &solvers, const std::
   G->n = n;
   G->view.resize((size_t)n);
   std::vector<BSys> hs((size_t)n);
-  std::vector<long long> npts((size_t)n);
-  std::vector<int> all_ids((size_t)n);
   int rc = 0;
   for (int q = 0; q < n && !rc; ++q) {
     GmresBatchView &v = G->view[(size_t)q];
@@ -489,13 +614,28 @@ int batch_group_create(const std::vector<fpc_StructSolver> &solvers, const std::
     s.S = v.S;
     s.b = b[(size_t)q];
     s.x = x[(size_t)q];
-    npts[(size_t)q] = d.N;
-    all_ids[(size_t)q] = q;
     G->maxnred = std::max(G->maxnred, v.nred);
     G->maxN = std::max(G->maxN, d.N);
   }
   if (!rc) rc = dupload(&G->dsys, hs);
-  if (!rc) rc = make_table(all_ids, npts, 256, n, G->pts);
+  if (!rc) {
+    std::vector<int> one, til, tnx, tny, tnz;
+    std::vector<long long> onen;
+    for (int q = 0; q < n; ++q) {
+      const BSys &e = hs[(size_t)q];
+      if (tiled_box(e.nx, e.ny, e.N)) {
+        til.push_back(q);
+        tnx.push_back(e.nx);
+        tny.push_back(e.ny);
+        tnz.push_back(e.nz);
+      } else {
+        one.push_back(q);
+        onen.push_back(e.N);
+      }
+    }
+    rc = make_table(one, onen, 256, n, G->pts);
+    if (!rc) rc = make_tile_table(til, tnx, tny, tnz, G->ptsT);
+  }
   if (!rc && G->pc == 3) {
     // level descriptors: system q's levels are contiguous, coarsest last
     std::vector<std::vector<MgLevelView>> lv((size_t)n);
@@ -549,18 +689,35 @@ int batch_group_create(const std::vector<fpc_StructSolver> &solvers, const std::
     // per level: items = fine level l of the systems with a level below it
     G->down.resize((size_t)std::max(0, maxl - 1));
     G->restr.resize(G->down.size());
+    G->downT.resize(G->down.size());
     for (int l = 0; l + 1 < maxl && !rc; ++l) {
-      std::vector<int> items;
-      for (int q = 0; q < n; ++q)
-        if ((int)lv[(size_t)q].size() > l + 1) items.push_back(base[(size_t)q] + l);
+      // fine level l of every system with a level below it: one-point or tiled;
+      // the tiled z-coarsened ones restrict inside their down kernel
+      std::vector<int> one, til, rit, tnx, tny, tnz;
       std::vector<long long> fine, crs;
-      for (int it : items) {
-        fine.push_back(G->hlev[(size_t)it].N);
-        crs.push_back(G->hlev[(size_t)it].Nc);
+      for (int q = 0; q < n; ++q) {
+        if ((int)lv[(size_t)q].size() <= l + 1) continue;
+        const int it = base[(size_t)q] + l;
+        const BLev &e = G->hlev[(size_t)it];
+        const bool tiled = tiled_box(e.nx, e.ny, e.N);
+        if (tiled) {
+          til.push_back(it);
+          tnx.push_back(e.nx);
+          tny.push_back(e.ny);
+          tnz.push_back(e.nz);
+        } else {
+          one.push_back(it);
+          fine.push_back(e.N);
+        }
+        if (!tiled || e.cdir != 2) {
+          rit.push_back(it);
+          crs.push_back(e.Nc);
+        }
       }
       const int nids = (int)G->hlev.size();
-      rc = make_table(items, fine, 256, nids, G->down[(size_t)l]);
-      if (!rc) rc = make_table(items, crs, 256, nids, G->restr[(size_t)l]);
+      rc = make_table(one, fine, 256, nids, G->down[(size_t)l]);
+      if (!rc) rc = make_tile_table(til, tnx, tny, tnz, G->downT[(size_t)l]);
+      if (!rc) rc = make_table(rit, crs, 256, nids, G->restr[(size_t)l]);
     }
     // coarsest levels: ceil(ncoarse / 4) blocks each
     if (!rc) {

@@ -56,35 +56,6 @@ __global__ void k_arnoldi_matvec(const double *__restrict__ c, long long pitch, 
   w[r] = sg * s;
 }
 
-// The same matvec on z-marching tiles (each row of A and z read once).
-template <bool kDinv>
-struct ArnoldiOp {
-  const double *c;
-  long long pitch;
-  int nx, ny;
-  const double *z, *dinv;
-  double *w;
-  double sg;
-  struct Raw {
-    double z, dinv;
-  };
-  struct Row {
-    Coef7<double> a;
-  };
-  __device__ long long at(int i, int j, int k) const { return ((long long)k * ny + j) * nx + i; }
-  __device__ Raw fetch(int i, int j, int k) const {
-    long long t = at(i, j, k);
-    return Raw{z[t], kDinv ? dinv[t] : 1.0};
-  }
-  __device__ double finish(const Raw &r) const { return kDinv ? r.z * r.dinv : r.z; }
-  __device__ Row row(int i, int j, int k) const {
-    Row q;
-    q.a.load(c, pitch, at(i, j, k));
-    return q;
-  }
-  __device__ void out(const Row &q, int i, int j, int k, const Nbr<double> &nb) { w[at(i, j, k)] = sg * q.a.apply(nb); }
-};
-
 template <bool kDinv, int TY>
 __global__ void __launch_bounds__(kTileX *TY)
     k_arnoldi_matvec_tiled(const double *__restrict__ c, long long pitch, int nx, int ny, int nz,
@@ -95,7 +66,7 @@ __global__ void __launch_bounds__(kTileX *TY)
   ArnoldiOp<kDinv> op{c, pitch, nx, ny, zin ? zin : V + (long long)cur * vp, dinv,
                       (double *)V + (long long)(cur + 1) * vp, S->sigma[cur]};
   const int k0 = blockIdx.z * kz;
-  march_tile<double, kTileX, TY>(op, nx, ny, nz, k0, min(k0 + kz, nz));
+  march_tile<double, kTileX, TY>(op, nx, ny, nz, k0, min(k0 + kz, nz), blockIdx.x, blockIdx.y);
 }
 
 // w = sigma A z (z * 1/diag when dinv is given) for the current basis index.

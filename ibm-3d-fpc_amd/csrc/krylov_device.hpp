@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include "fpc_internal.hpp"
+#include "stencil_tile.hpp"
 
 namespace fpc {
 namespace kry {
@@ -291,6 +292,35 @@ __device__ inline void reduce_body(const double *__restrict__ part, int n, doubl
   double tot = block_sum<1024>(acc, lds);
   if (threadIdx.x == 0) *out = tot;
 }
+
+// The same matvec on z-marching tiles (each row of A and z read once).
+template <bool kDinv>
+struct ArnoldiOp {
+  const double *c;
+  long long pitch;
+  int nx, ny;
+  const double *z, *dinv;
+  double *w;
+  double sg;
+  struct Raw {
+    double z, dinv;
+  };
+  struct Row {
+    Coef7<double> a;
+  };
+  __device__ long long at(int i, int j, int k) const { return ((long long)k * ny + j) * nx + i; }
+  __device__ Raw fetch(int i, int j, int k) const {
+    long long t = at(i, j, k);
+    return Raw{z[t], kDinv ? dinv[t] : 1.0};
+  }
+  __device__ double finish(const Raw &r) const { return kDinv ? r.z * r.dinv : r.z; }
+  __device__ Row row(int i, int j, int k) const {
+    Row q;
+    q.a.load(c, pitch, at(i, j, k));
+    return q;
+  }
+  __device__ void out(const Row &q, int i, int j, int k, const Nbr<double> &nb) { w[at(i, j, k)] = sg * q.a.apply(nb); }
+};
 
 }  // namespace kry
 }  // namespace fpc

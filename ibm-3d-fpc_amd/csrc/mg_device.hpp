@@ -1,12 +1,15 @@
 // mg_device.hpp -- point bodies of the one-point-per-thread multigrid kernels
-// (libfpc).
+// and the z-marching tile operations of the same stages (libfpc).
 //
 // The level kernels of one hierarchy (pfmg.hip) and the batched kernels that run
 // a level of many hierarchies in one launch (batched.hip) evaluate these same
-// functions, so a V-cycle in a batch is bit for bit the V-cycle alone.
+// functions, so a V-cycle in a batch is bit for bit the V-cycle alone, whichever
+// of the two kernel forms a level runs.
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include "stencil_tile.hpp"
 
 namespace fpc {
 namespace mgd {
@@ -95,6 +98,138 @@ __device__ inline double gemv_row(const double *__restrict__ inv, int n, const T
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
   return s;
 }
+
+// ---------------------------------------------------------------- z-marching forms
+// The same arithmetic on the z-marching tiles (stencil_tile.hpp): every row read
+// once, z-neighbours carried in registers; bit for bit the one-point results.
+
+// Jacobi sweep from zero and residual: r = b - A (w D^-1 b).
+template <typename T>
+struct DownOp {
+  const T *c;
+  long long pitch;
+  int nx, ny;
+  const T *dinv, *b;
+  T *r;
+  T w;
+  struct Raw {
+    T b, dinv;
+  };
+  struct Row {
+    Coef7<T> a;
+    T b;
+  };
+  __device__ long long at(int i, int j, int k) const { return ((long long)k * ny + j) * nx + i; }
+  __device__ Raw fetch(int i, int j, int k) const {
+    long long t = at(i, j, k);
+    return Raw{b[t], dinv[t]};
+  }
+  __device__ T finish(const Raw &q) const { return w * q.b * q.dinv; }
+  __device__ Row row(int i, int j, int k) const {
+    long long t = at(i, j, k);
+    Row q;
+    q.a.load(c, pitch, t);
+    q.b = b[t];
+    return q;
+  }
+  __device__ void out(const Row &q, int i, int j, int k, const Nbr<T> &nb) { r[at(i, j, k)] = q.b - q.a.apply(nb); }
+};
+
+// The same, restricted on the fly when the level coarsens z: coarse plane kc
+// (fine C plane 2kc+1) = r(C) + wp(F-) r(F-) + wm(F+) r(F+), in that order; the
+// fine residual is never stored.  Chunks start on even planes and run one plane
+// past their end so the F+ plane of their last coarse plane is seen.
+template <typename T>
+struct DownRestrictZOp : DownOp<T> {
+  const T *wm, *wp;
+  T *bc;
+  int nz, ncz, k0;
+  T r_prev = T(0), pend = T(0);
+  __device__ void out(const typename DownOp<T>::Row &q, int i, int j, int k, const Nbr<T> &nb) {
+    const long long t = this->at(i, j, k), pl = (long long)this->nx * this->ny;
+    const T rk = q.b - q.a.apply(nb);
+    if (k & 1) {
+      const T v = rk + wp[t - pl] * r_prev;
+      const int kc = (k - 1) >> 1;
+      if (k + 1 < nz) pend = v;
+      else if (kc < ncz) bc[((long long)kc * this->ny + j) * this->nx + i] = v;
+    } else if (k > k0) {
+      const int kc = (k >> 1) - 1;
+      if (kc < ncz) bc[((long long)kc * this->ny + j) * this->nx + i] = pend + wm[t] * rk;
+    }
+    r_prev = rk;
+  }
+};
+
+// Prolongation of the coarse correction plus one Jacobi sweep:
+// y = x + P e, then y + w D^-1 (b - A y), with x = w D^-1 b recomputed.
+template <typename T, int D>
+struct UpOp {
+  const T *c;
+  long long pitch;
+  int nx, ny;
+  const T *dinv, *b, *wm, *wp, *yc;
+  int cnx, cny, ncd;
+  T *y;
+  T w;
+  // Loads of corrected<T, D>: C points use e1 only; at F points an absent coarse
+  // neighbour reads as a zero weight and value (0 + wp e1 == wp e1 exactly).
+  struct Raw {
+    T b, dinv, wm, wp, e0, e1;
+    bool f;
+  };
+  struct Row {
+    Coef7<T> a;
+    T b, dinv;
+  };
+  __device__ long long at(int i, int j, int k) const { return ((long long)k * ny + j) * nx + i; }
+  __device__ Raw fetch(int i, int j, int k) const {
+    const long long t = at(i, j, k);
+    const int pd = (D == 0) ? i : (D == 1 ? j : k);
+    const long long cstride = (D == 0) ? 1 : (D == 1 ? (long long)cnx : (long long)cnx * cny);
+    const int q = (pd & 1) ? ((pd - 1) >> 1) : (pd >> 1);
+    const int ci = (D == 0) ? q : i, cj = (D == 1) ? q : j, ck = (D == 2) ? q : k;
+    const long long cr = ((long long)ck * cny + cj) * cnx + ci;
+    Raw r;
+    r.b = b[t];
+    r.dinv = dinv[t];
+    r.f = !(pd & 1);
+    if (pd & 1) {
+      r.wm = r.wp = r.e0 = T(0);
+      r.e1 = yc[cr];
+    } else {
+      const bool lo = q - 1 >= 0, hi = q < ncd;
+      r.wm = lo ? wm[t] : T(0);
+      r.e0 = lo ? yc[cr - cstride] : T(0);
+      r.wp = hi ? wp[t] : T(0);
+      r.e1 = hi ? yc[cr] : T(0);
+    }
+    return r;
+  }
+  __device__ T finish(const Raw &r) const {
+    T x = w * r.b * r.dinv;
+    if (r.f) {
+      T s = (T)0;
+      s += r.wm * r.e0;
+      s += r.wp * r.e1;
+      x += s;
+    } else {
+      x += r.e1;
+    }
+    return x;
+  }
+  __device__ Row row(int i, int j, int k) const {
+    long long t = at(i, j, k);
+    Row q;
+    q.a.load(c, pitch, t);
+    q.b = b[t];
+    q.dinv = dinv[t];
+    return q;
+  }
+  __device__ void out(const Row &q, int i, int j, int k, const Nbr<T> &nb) {
+    y[at(i, j, k)] = nb.c + w * (q.b - q.a.apply(nb)) * q.dinv;
+  }
+};
 
 }  // namespace mgd
 }  // namespace fpc

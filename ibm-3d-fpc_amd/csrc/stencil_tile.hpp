@@ -54,7 +54,7 @@ __device__ inline unsigned nbr_in_box(int i, int j, int k, int nx, int ny, int n
          ((unsigned)(k > 0) << 4) | ((unsigned)(k < nz - 1) << 5);
 }
 
-// Walk planes k0 .. kend-1 of the tile owned by this workgroup.  The op splits
+// Walk planes k0 .. kend-1 of tile (bx, by), owned by this workgroup.  The op splits
 // its point function in two so loads run ahead of use: op.fetch(i, j, k) issues
 // the loads (an Op::Raw) and op.finish(raw) does the arithmetic; op.row(i, j, k)
 // loads the point's own data (an Op::Row) and op.out(row, i, j, k, nb) consumes
@@ -65,16 +65,16 @@ __device__ inline unsigned nbr_in_box(int i, int j, int k, int nx, int ny, int n
 // 16-22 more VGPRs and one wave per SIMD of occupancy, and measured slower.)
 // Points outside the box read as zero.
 template <typename T, int TX, int TY, class Op>
-__device__ inline void march_tile(Op &op, int nx, int ny, int nz, int k0, int kend) {
+__device__ inline void march_tile(Op &op, int nx, int ny, int nz, int k0, int kend, int bx, int by) {
   using Raw = typename Op::Raw;
   using Row = typename Op::Row;
   __shared__ T plane[2][TY + 2][TX + 2];
   const int tx = threadIdx.x, ty = threadIdx.y;
-  const int i = blockIdx.x * TX + tx, j = blockIdx.y * TY + ty;
+  const int i = bx * TX + tx, j = by * TY + ty;
   const bool in = (i < nx) && (j < ny);
   int hx = 0, hy = 0;
   const bool ring = tile_ring<TX, TY>(ty * TX + tx, hx, hy);
-  const int hi = blockIdx.x * TX + hx, hj = blockIdx.y * TY + hy;
+  const int hi = bx * TX + hx, hj = by * TY + hy;
   const bool ring_in = ring && hi >= 0 && hi < nx && hj >= 0 && hj < ny;
   T zm = (in && k0 > 0) ? op.finish(op.fetch(i, j, k0 - 1)) : T(0);
   T zc = (in && k0 < nz) ? op.finish(op.fetch(i, j, k0)) : T(0);

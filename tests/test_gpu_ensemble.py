@@ -61,3 +61,21 @@ def test_ensemble_packed_offsets(torch_cuda):
     for i in range(len(pars)):
         assert bits_equal(packed[off[i]:off[i + 1]], ens.members[i].get("p"))
     ens.close()
+
+
+def test_batched_ensemble_tiled_instances(torch_cuda):
+    """Instances large enough that the batch runs their matvecs and V-cycle levels
+    on z-marching tiles (>= 16384 points, nx >= 32, ny >= 8), while each instance
+    alone runs one-point kernels: the two kernel forms give the same bits."""
+    pars = F.ensemble_workload(n=4, seed=12, nx=(64, 96), nyz=(16, 24), inlet_ramp_steps=3)
+    ens = F.Ensemble(pars, nthreads=0)
+    ens.step(2)
+    for i, p in enumerate(pars):
+        alone = F.Sim(p)
+        alone.step(2)
+        for f in ("u", "v", "w", "p"):
+            assert bits_equal(ens.members[i].get(f), alone.get(f)), (i, f)
+        for k in ("its_u", "its_v", "its_w", "its_p"):
+            assert ens.members[i].scalar(k) == alone.scalar(k), (i, k)
+        alone.close()
+    ens.close()
