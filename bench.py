@@ -15,7 +15,9 @@ launch / its mean launch duration, against the 8 TB/s HBM3E peak.  Launches
 enqueued ahead of a converged GMRES cycle return at once and are not counted.
 traffic = HBM bytes per launch read from profiles/hbm_traffic.txt (FETCH_SIZE +
 WRITE_SIZE per dispatch, calibrated on a copy kernel) when that file names the
-class, else null.
+class, else null.  copy_gbs = a 1 GiB device-to-device copy timed the same way
+(read + written bytes / s): the streaming rate reached in practice, and
+frac_of_copy = achieved / copy_gbs.
 
 cpu_baseline: the oracle (single-threaded C++ restatement of the same path)
 on a 16-deep z-slab of the same channel (512x96x16, Lz scaled so the grid
@@ -117,6 +119,21 @@ def roofline(dom, rec):
             "kernel": dom, "launches": n, "mean_us": mean_s * 1e6, "algorithmic_bytes_per_launch": per_launch}
 
 
+def copy_gbs(torch, nbytes=1 << 30, reps=10):
+    """Device-to-device copy bandwidth (bytes read + written per second, GB/s): the
+    streaming rate this GPU reaches in practice, beside the 8 TB/s nominal peak."""
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    e1.synchronize()
+    return 2.0 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+
+
 def cpu_baseline():
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as O
@@ -153,6 +170,9 @@ def run_ensemble(F, torch, nthreads, n=64, steps=50):
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         out[tag] = {"value": cells * steps / dt, "seconds": dt, "threads": nth}
+        if nth == 0:
+            kz, tmin = ens.tiles()
+            out[tag]["autotuned_tiles"] = {"kz": kz, "tile_min_points": tmin}
         ens.close()
     best = max(("concurrent", "batched"), key=lambda k: out[k]["value"])
     out["value"] = out[best]["value"]
@@ -333,6 +353,10 @@ def main():
                          "iterations_mean": [float(x) for x in np.mean(np.array(its32), axis=0)],
                          "roofline": roofline(dom32, rec32) if dom32 else None, "kernel_classes": cls32}
         sim32.close()
+    if res["roofline"]:
+        cg = copy_gbs(torch)
+        res["roofline"]["copy_gbs"] = cg
+        res["roofline"]["frac_of_copy"] = res["roofline"]["achieved"] / cg
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline()
     print(json.dumps(res))

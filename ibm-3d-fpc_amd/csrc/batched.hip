@@ -93,10 +93,16 @@ struct TileTable {
   int nblocks = 0;
 };
 
-// Items of at least this many points (and nx >= 32, ny >= 8) run the z-marching
-// kernels inside a batch: with many systems per launch the tiles never lack
-// workgroups, unlike one small box alone.
-constexpr long long kBatchTileMin = 16384;
+// Tile configurations tried by the batch autotuner (BatchGroup::autotune): boxes
+// of at least tile_min points (and nx >= 32, ny >= 8) run the z-marching kernels
+// in z-chunks of at most kz planes, the others the one-point kernels.  With many
+// systems per launch the tiles do not lack workgroups, unlike one small box
+// alone.  Every configuration computes the same bits.
+struct TileConfig {
+  int kz;
+  long long tile_min;
+};
+constexpr TileConfig kTileConfigs[] = {{4, 16384}, {8, 16384}, {16, 16384}, {2, 16384}, {4, 65536}, {4, 1LL << 40}};
 constexpr int kTY = 8;
 
 __device__ inline void ijk_of(long long t, int nx, int ny, int &i, int &j, int &k) {
@@ -407,22 +413,20 @@ int make_table(const std::vector<int> &items, const std::vector<long long> &coun
   return rc;
 }
 
-bool tiled_box(int nx, int ny, long long N) { return nx >= 32 && ny >= 8 && N >= kBatchTileMin; }
+bool tiled_box(int nx, int ny, long long N, long long tile_min) { return nx >= 32 && ny >= 8 && N >= tile_min; }
 
-// z-chunks of at most 4 planes (short chunks: more workgroups per box; measured
-// fastest of 4, 8 and 16 on the 64-channel ensemble), even length (z-restriction
-// pairs stay in a chunk)
-int tile_chunk(int nz) {
-  const int nch = (nz + 3) / 4;
+// z-chunks of at most kzmax planes, of even length (z-restriction pairs stay in a chunk)
+int tile_chunk(int nz, int kzmax) {
+  const int nch = (nz + kzmax - 1) / kzmax;
   int kz = (nz + nch - 1) / nch;
   return kz + (kz & 1);
 }
 
 int make_tile_table(const std::vector<int> &items, const std::vector<int> &nx, const std::vector<int> &ny,
-                    const std::vector<int> &nz, TileTable &t) {
+                    const std::vector<int> &nz, int kzmax, TileTable &t) {
   std::vector<TileEnt> ent;
   for (size_t e = 0; e < items.size(); ++e) {
-    const int tx = (nx[e] + kTileX - 1) / kTileX, ty = (ny[e] + kTY - 1) / kTY, kz = tile_chunk(nz[e]);
+    const int tx = (nx[e] + kTileX - 1) / kTileX, ty = (ny[e] + kTY - 1) / kTY, kz = tile_chunk(nz[e], kzmax);
     for (int k0 = 0; k0 < nz[e]; k0 += kz)
       for (int by = 0; by < ty; ++by)
         for (int bx = 0; bx < tx; ++bx) ent.push_back(TileEnt{items[e], bx | (by << 16), k0, std::min(k0 + kz, nz[e])});
@@ -469,6 +473,9 @@ class BatchGroup {
   double *hargs = nullptr, *dargs = nullptr;
   long long maxN = 0;
   int maxnred = 1;
+  std::vector<BSys> hsys;
+  std::vector<int> nlev, base;  // levels per system, first level index
+  TileConfig tiles{4, 16384};
 
   ~BatchGroup() {
     if (dsys) (void)hipFree(dsys);
@@ -574,6 +581,105 @@ class BatchGroup {
   }
 
   int solve(int *its_out, double *rel_out, int *err_out);
+
+  // (Re)builds the launch tables of the matvec and V-cycle levels for tile configuration cfg.
+  int build_tables(const TileConfig &cfg) {
+    free_table(pts);
+    free_tiles(ptsT);
+    for (auto &t : down) free_table(t);
+    for (auto &t : restr) free_table(t);
+    for (auto &t : downT) free_tiles(t);
+    tiles = cfg;
+    std::vector<int> one, til, tnx, tny, tnz;
+    std::vector<long long> onen;
+    for (int q = 0; q < n; ++q) {
+      const BSys &e = hsys[(size_t)q];
+      if (tiled_box(e.nx, e.ny, e.N, cfg.tile_min)) {
+        til.push_back(q);
+        tnx.push_back(e.nx);
+        tny.push_back(e.ny);
+        tnz.push_back(e.nz);
+      } else {
+        one.push_back(q);
+        onen.push_back(e.N);
+      }
+    }
+    int rc = make_table(one, onen, 256, n, pts);
+    if (!rc) rc = make_tile_table(til, tnx, tny, tnz, cfg.kz, ptsT);
+    if (rc || pc != 3) return rc;
+    const int maxl = *std::max_element(nlev.begin(), nlev.end());
+    down.assign((size_t)std::max(0, maxl - 1), Table{});
+    restr.assign(down.size(), Table{});
+    downT.assign(down.size(), TileTable{});
+    for (int l = 0; l + 1 < maxl && !rc; ++l) {
+      // fine level l of every system with a level below it: one-point or tiled;
+      // the tiled z-coarsened ones restrict inside their down kernel
+      std::vector<int> onel, tl, rit, lnx, lny, lnz;
+      std::vector<long long> fine, crs;
+      for (int q = 0; q < n; ++q) {
+        if (nlev[(size_t)q] <= l + 1) continue;
+        const int it = base[(size_t)q] + l;
+        const BLev &e = hlev[(size_t)it];
+        const bool tiled = tiled_box(e.nx, e.ny, e.N, cfg.tile_min);
+        if (tiled) {
+          tl.push_back(it);
+          lnx.push_back(e.nx);
+          lny.push_back(e.ny);
+          lnz.push_back(e.nz);
+        } else {
+          onel.push_back(it);
+          fine.push_back(e.N);
+        }
+        if (!tiled || e.cdir != 2) {
+          rit.push_back(it);
+          crs.push_back(e.Nc);
+        }
+      }
+      const int nids = (int)hlev.size();
+      rc = make_table(onel, fine, 256, nids, down[(size_t)l]);
+      if (!rc) rc = make_tile_table(tl, lnx, lny, lnz, cfg.kz, downT[(size_t)l]);
+      if (!rc) rc = make_table(rit, crs, 256, nids, restr[(size_t)l]);
+    }
+    return rc;
+  }
+
+  // Autotuner: times one V-cycle of every system (input W = 0) under each tile
+  // configuration and keeps the fastest (all give the same bits).  The V-cycle's
+  // buffers hold scratch results afterwards; every solve overwrites them.
+  int autotune() {
+    if (pc != 3) return 0;
+    for (const BSys &e : hsys) FPC_HIP(hipMemsetAsync(e.W, 0, (size_t)e.N * sizeof(double), stream()));
+    Sub all;
+    all.n = n;
+    for (int q = 0; q < n; ++q) all.idx[q] = (unsigned short)q;
+    k_b_flags<<<1, 1024, 0, stream()>>>(all, n, flags);
+    FPC_LAUNCH_CHECK();
+    hipEvent_t e0, e1;
+    FPC_HIP(hipEventCreate(&e0));
+    FPC_HIP(hipEventCreate(&e1));
+    float best = 0.0f;
+    TileConfig pick = kTileConfigs[0];
+    int rc = 0;
+    for (const TileConfig &cfg : kTileConfigs) {
+      if ((rc = build_tables(cfg)) || (rc = vcycle(1, 0, flags))) break;  // warm-up
+      if (hipEventRecord(e0, stream()) != hipSuccess) { rc = FPC_ERROR_GENERIC; break; }
+      for (int r = 0; r < 3 && !rc; ++r) rc = vcycle(1, 0, flags);
+      if (rc || hipEventRecord(e1, stream()) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+        rc = rc ? rc : FPC_ERROR_GENERIC;
+        break;
+      }
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (best == 0.0f || ms < best) {
+        best = ms;
+        pick = cfg;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    return build_tables(pick);
+  }
 };
 
 int batch_group_create(const std::vector<fpc_StructSolver> &solvers, const std::vector<const double *> &b,
@@ -618,24 +724,7 @@ int batch_group_create(const std::vector<fpc_StructSolver> &solvers, const std::
     G->maxN = std::max(G->maxN, d.N);
   }
   if (!rc) rc = dupload(&G->dsys, hs);
-  if (!rc) {
-    std::vector<int> one, til, tnx, tny, tnz;
-    std::vector<long long> onen;
-    for (int q = 0; q < n; ++q) {
-      const BSys &e = hs[(size_t)q];
-      if (tiled_box(e.nx, e.ny, e.N)) {
-        til.push_back(q);
-        tnx.push_back(e.nx);
-        tny.push_back(e.ny);
-        tnz.push_back(e.nz);
-      } else {
-        one.push_back(q);
-        onen.push_back(e.N);
-      }
-    }
-    rc = make_table(one, onen, 256, n, G->pts);
-    if (!rc) rc = make_tile_table(til, tnx, tny, tnz, G->ptsT);
-  }
+  G->hsys = hs;
   if (!rc && G->pc == 3) {
     // level descriptors: system q's levels are contiguous, coarsest last
     std::vector<std::vector<MgLevelView>> lv((size_t)n);
@@ -686,39 +775,8 @@ int batch_group_create(const std::vector<fpc_StructSolver> &solvers, const std::
       }
     }
     if (!rc) rc = dupload(&G->dlev, G->hlev);
-    // per level: items = fine level l of the systems with a level below it
-    G->down.resize((size_t)std::max(0, maxl - 1));
-    G->restr.resize(G->down.size());
-    G->downT.resize(G->down.size());
-    for (int l = 0; l + 1 < maxl && !rc; ++l) {
-      // fine level l of every system with a level below it: one-point or tiled;
-      // the tiled z-coarsened ones restrict inside their down kernel
-      std::vector<int> one, til, rit, tnx, tny, tnz;
-      std::vector<long long> fine, crs;
-      for (int q = 0; q < n; ++q) {
-        if ((int)lv[(size_t)q].size() <= l + 1) continue;
-        const int it = base[(size_t)q] + l;
-        const BLev &e = G->hlev[(size_t)it];
-        const bool tiled = tiled_box(e.nx, e.ny, e.N);
-        if (tiled) {
-          til.push_back(it);
-          tnx.push_back(e.nx);
-          tny.push_back(e.ny);
-          tnz.push_back(e.nz);
-        } else {
-          one.push_back(it);
-          fine.push_back(e.N);
-        }
-        if (!tiled || e.cdir != 2) {
-          rit.push_back(it);
-          crs.push_back(e.Nc);
-        }
-      }
-      const int nids = (int)G->hlev.size();
-      rc = make_table(one, fine, 256, nids, G->down[(size_t)l]);
-      if (!rc) rc = make_tile_table(til, tnx, tny, tnz, G->downT[(size_t)l]);
-      if (!rc) rc = make_table(rit, crs, 256, nids, G->restr[(size_t)l]);
-    }
+    G->base = base;
+    for (int q = 0; q < n; ++q) G->nlev.push_back((int)lv[(size_t)q].size());
     // coarsest levels: ceil(ncoarse / 4) blocks each
     if (!rc) {
       std::vector<int> items;
@@ -739,6 +797,8 @@ int batch_group_create(const std::vector<fpc_StructSolver> &solvers, const std::
       rc = FPC_ERROR_MEMORY;
     }
   }
+  if (!rc) rc = G->autotune();
+  if (!rc && G->pc != 3) rc = G->build_tables(G->tiles);
   if (!rc) {
     // every state starts finished, so only systems in a cycle run Arnoldi steps
     k_b_set_done<<<(n + 255) / 256, 256, 0, stream()>>>(G->dsys, n, 3);
@@ -902,6 +962,11 @@ int BatchGroup::solve(int *its_out, double *rel_out, int *err_out) {
 int batch_group_solve(BatchGroup *g, int *its, double *rel, int *err) { return g->solve(its, rel, err); }
 
 void batch_group_destroy(BatchGroup *g) { delete g; }
+
+void batch_group_tiles(const BatchGroup *g, int *kz, long long *tile_min) {
+  *kz = g->tiles.kz;
+  *tile_min = g->tiles.tile_min;
+}
 
 // ------------------------------------------------------------ non-finite check
 class NonfiniteCheck {

@@ -59,6 +59,9 @@ int batch_group_create(const std::vector<fpc_StructSolver> &solvers, const std::
 // return code (0 or FPC_ERROR_CONV) land in its/rel/err.
 int batch_group_solve(BatchGroup *g, int *its, double *rel, int *err);
 void batch_group_destroy(BatchGroup *g);
+// The tile configuration the group's autotuner picked: z-chunk planes and the
+// smallest box on z-marching tiles.
+void batch_group_tiles(const BatchGroup *g, int *kz, long long *tile_min);
 
 // Non-finite check of many device arrays at once: flags[q] = 1 when arrays[q]
 // (len[q] doubles) holds a NaN or Inf.  Enqueued only; the caller reads flags

@@ -280,6 +280,13 @@ int fpc_ensemble_step(fpc_ensemble E, int nsteps) {
 
 int fpc_ensemble_size(fpc_ensemble E) { return E ? (int)E->sims.size() : 0; }
 
+int fpc_ensemble_tiles(fpc_ensemble E, int *kz, long long *tile_min) {
+  FPC_CHECK_ARG(E && kz && tile_min, "null argument");
+  FPC_CHECK_ARG(E->batched && E->pres, "not a batched ensemble");
+  fpc::batch_group_tiles(E->pres, kz, tile_min);
+  return 0;
+}
+
 fpc_sim fpc_ensemble_member(fpc_ensemble E, int i) {
   if (!E || i < 0 || i >= (int)E->sims.size()) return nullptr;
   return E->sims[(size_t)i];

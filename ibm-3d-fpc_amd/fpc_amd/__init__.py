@@ -158,6 +158,7 @@ SIGNATURES = {
     "fpc_sim_stream": (_vp, [_vp]),
     "fpc_ensemble_create": (_i, [_PP, _i, _i, _pvp]), "fpc_ensemble_step": (_i, [_vp, _i]),
     "fpc_ensemble_size": (_i, [_vp]), "fpc_ensemble_member": (_vp, [_vp, _i]),
+    "fpc_ensemble_tiles": (_i, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_longlong)]),
     "fpc_ensemble_offsets": (_i, [_vp, _cp, ctypes.POINTER(ctypes.c_longlong)]),
     "fpc_ensemble_pack": (_i, [_vp, _cp, _vp, _ll]), "fpc_ensemble_destroy": (_i, [_vp]),
     "fpc_adjoint_create": (_i, [_vp, _pvp]),
@@ -397,6 +398,12 @@ class Ensemble:
 
     def step(self, n=1):
         check(lib().fpc_ensemble_step(self.h, n), "ensemble step")
+
+    def tiles(self):
+        """(z-chunk planes, smallest tiled box) picked by the autotuner of a batched ensemble."""
+        kz, tmin = ctypes.c_int(), ctypes.c_longlong()
+        check(lib().fpc_ensemble_tiles(self.h, ctypes.byref(kz), ctypes.byref(tmin)), "ensemble tiles")
+        return kz.value, tmin.value
 
     def offsets(self, field):
         off = (ctypes.c_longlong * (len(self.params) + 1))()

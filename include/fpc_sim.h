@@ -105,6 +105,11 @@ int fpc_ensemble_create(const fpc_params *pars, int n, int nthreads, fpc_ensembl
 /* Advances every instance by nsteps (a solver's maximum-iterations flag is not an error). */
 int fpc_ensemble_step(fpc_ensemble ens, int nsteps);
 int fpc_ensemble_size(fpc_ensemble ens);
+/* Batched ensembles: the launch configuration chosen by the library's autotuner
+ * at creation (it times one V-cycle of every pressure system under each
+ * candidate): z-chunk length of the z-marching kernels and the smallest box
+ * (points) that runs them.  The choice changes speed only, never results. */
+int fpc_ensemble_tiles(fpc_ensemble ens, int *kz, long long *tile_min);
 fpc_sim fpc_ensemble_member(fpc_ensemble ens, int i);
 /* Packed offset table of field u, v, w or p over the instances: instance i owns
  * elements offsets[i] .. offsets[i+1]-1 of a packed array (n+1 entries). */
