@@ -27,35 +27,7 @@ namespace {
 __global__ void k_bc(double *u, double *v, double *w, const u8 *au, const u8 *av, const u8 *aw, const double *uin,
                      int nx, int ny, int nz) {
   TID_1D;
-  long long Nu = (long long)(nx + 1) * ny * nz, Nv = (long long)nx * (ny + 1) * nz, Nw = (long long)nx * ny * (nz + 1);
-  if (tid < Nu) {
-    long long pl = (long long)(nx + 1) * ny;
-    int k = (int)(tid / pl);
-    long long rem = tid - k * pl;
-    int j = (int)(rem / (nx + 1));
-    int i = (int)(rem - (long long)j * (nx + 1));
-    if (i == 0) u[tid] = uin[j + (long long)ny * k];
-    else if (i == nx) {
-      long long s = iU(nx - 1, j, k, nx, ny);
-      u[tid] = (nx - 1 == 0) ? uin[j + (long long)ny * k] : (au[s] ? u[s] : 0.0);
-    } else if (!au[tid]) u[tid] = 0.0;
-  }
-  if (tid < Nv) {
-    long long pl = (long long)nx * (ny + 1);
-    int k = (int)(tid / pl);
-    long long rem = tid - k * pl;
-    int j = (int)(rem / nx);
-    if (j == 0 || j == ny || !av[tid]) v[tid] = 0.0;
-  }
-  if (tid < Nw) {
-    long long pl = (long long)nx * ny;
-    int k = (int)(tid / pl);
-    long long rem = tid - k * pl;
-    int j = (int)(rem / nx);
-    int i = (int)(rem - (long long)j * nx);
-    if (k == 0 || k == nz) w[tid] = 0.0;
-    else if (!aw[iW(i, j, k - 1, nx, ny)]) w[tid] = 0.0;  // the reference tests active_w[k-1]
-  }
+  bc_point(tid, u, v, w, au, av, aw, uin, nx, ny, nz);
 }
 
 __global__ void k_conv(int comp, double *conv, const double *u, const double *v, const double *w, const u8 *act, int nx,
@@ -104,18 +76,7 @@ __global__ void k_build_full(int comp, double *rhsf, double *x0, const double *f
 
 __global__ void k_scatter(int comp, double *o, const double *x, const double *base, int nx, int ny, int nz) {
   TID_1D;
-  int sx = nx + (comp == 0), sy = ny + (comp == 1), sz = nz + (comp == 2);
-  long long N = (long long)sx * sy * sz;
-  if (tid >= N) return;
-  if (base) o[tid] = base[tid];
-  long long pl = (long long)sx * sy;
-  int k = (int)(tid / pl);
-  long long rem = tid - k * pl;
-  int j = (int)(rem / sx);
-  int i = (int)(rem - (long long)j * sx);
-  if (comp == 0 && i >= 1 && i <= nx - 1) o[tid] = x[((long long)k * ny + j) * (nx - 1) + (i - 1)];
-  if (comp == 1 && j >= 1 && j <= ny - 1) o[tid] = x[((long long)k * (ny - 1) + (j - 1)) * nx + i];
-  if (comp == 2 && k >= 1 && k <= nz - 1) o[tid] = x[((long long)(k - 1) * ny + j) * nx + i];
+  scatter_point(comp, tid, o, x, base, nx, ny, nz);
 }
 
 __global__ void k_csr(int nrows, const int *rowptr, const int *colind, const double *vals, const double *x, double *y) {
@@ -192,48 +153,12 @@ __global__ void k_zero(double *p, long long n) {
 // Momentum rhs and x0 on a u/v/w box row, one thread per row.
 __global__ void k_explicit(int comp, fpc_flow_ops o, const double *u, const double *v, const double *w, const double *p,
                            double *rhs, double *x0, int zero) {
-  int nx = o.nx, ny = o.ny, nz = o.nz;
-  int bx = nx - (comp == 0), by = ny - (comp == 1), bz = nz - (comp == 2);
+  int bx = o.nx - (comp == 0), by = o.ny - (comp == 1), bz = o.nz - (comp == 2);
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y;
   int k = blockIdx.z;
   if (i >= bx || j >= by || k >= bz) return;
-  long long row = ((long long)k * by + j) * bx + i;
-  double rr = 0.0, xv;
-  double at = o.rho / o.dt;
-  if (comp == 0) {
-    int fi = i + 1;
-    long long t = iU(fi, j, k, nx, ny);
-    double f = u[t];
-    xv = f;
-    if (o.active_u[t]) {
-      double cv = conv_u_at(u, v, w, fi, j, k, nx, ny, nz, o.dx, o.dy, o.dz);
-      double g = o.active_qu[t] ? gradx_at(p, o.fluid_p, fi, j, k, nx, ny, o.dx) : 0.0;
-      rr = (at * f - o.rho * cv - g + o.rhs_u_bc[row]) * o.scale_u[row];
-    }
-  } else if (comp == 1) {
-    int fj = j + 1;
-    long long t = iV(i, fj, k, nx, ny);
-    double f = v[t];
-    xv = f;
-    if (o.active_v[t]) {
-      double cv = conv_v_at(u, v, w, i, fj, k, nx, ny, nz, o.dx, o.dy, o.dz);
-      double g = o.active_qv[t] ? grady_at(p, o.fluid_p, i, fj, k, nx, ny, o.dy) : 0.0;
-      rr = (at * f - o.rho * cv - g + 0.0) * o.scale_v[row];
-    }
-  } else {
-    int fk = k + 1;
-    long long t = iW(i, j, fk, nx, ny);
-    double f = w[t];
-    xv = f;
-    if (o.active_w[t]) {
-      double cv = o.active_w[iW(i, j, fk - 1, nx, ny)] ? conv_w_at(u, v, w, i, j, fk, nx, ny, nz, o.dx, o.dy, o.dz) : 0.0;
-      double g = o.active_qw[t] ? gradz_at(p, o.fluid_p, i, j, fk, nx, ny, o.dz) : 0.0;
-      rr = (at * f - o.rho * cv - g + 0.0) * o.scale_w[row];
-    }
-  }
-  rhs[row] = rr;
-  x0[row] = zero ? 0.0 : xv;
+  explicit_row(comp, o, u, v, w, p, rhs, x0, zero, i, j, k);
 }
 
 // Pressure rhs on every cell of the p box.
@@ -243,57 +168,14 @@ __global__ void k_divergence(fpc_flow_ops o, const double *ut, const double *vt,
   int j = blockIdx.y * blockDim.y + threadIdx.y;
   int k = blockIdx.z;
   if (i >= o.nx || j >= o.ny) return;
-  long long c = iP(i, j, k, o.nx, o.ny);
-  double r = 0.0;
-  if (o.fluid_p[c]) {
-    double s = div_at(o, ut, vt, wt, nullptr, i, j, k);
-    double bc = (i == 0) ? o.div_bc[(long long)j + (long long)o.ny * k] : 0.0;
-    r = -(o.rho / o.dt) * (s + ramp * bc);
-  }
-  rhs[c] = r;
-  x0[c] = 0.0;
+  divergence_cell(o, ut, vt, wt, ramp, rhs, x0, i, j, k);
 }
 
 // Projection of the face velocities; thread t covers face t of each of u, v, w.
 __global__ void k_project(fpc_flow_ops o, const double *phi, const double *ut, const double *vt, const double *wt,
                           double *u, double *v, double *w, double *qout) {
   TID_1D;
-  int nx = o.nx, ny = o.ny, nz = o.nz;
-  double f = o.dt / o.rho;
-  long long Nu = (long long)(nx + 1) * ny * nz, Nv = (long long)nx * (ny + 1) * nz, Nw = (long long)nx * ny * (nz + 1);
-  if (tid < Nu) {
-    long long pl = (long long)(nx + 1) * ny;
-    int k = (int)(tid / pl);
-    long long rem = tid - k * pl;
-    int j = (int)(rem / (nx + 1));
-    int i = (int)(rem - (long long)j * (nx + 1));
-    double val = ut[tid];
-    if (o.active_qu[tid] && i >= 1) {
-      val = val - f * gradx_at(phi, o.fluid_p, i, j, k, nx, ny, o.dx);
-      if (i == nx) qout[(long long)j + (long long)ny * k] = val;
-    }
-    u[tid] = val;
-  }
-  if (tid < Nv) {
-    long long pl = (long long)nx * (ny + 1);
-    int k = (int)(tid / pl);
-    long long rem = tid - k * pl;
-    int j = (int)(rem / nx);
-    int i = (int)(rem - (long long)j * nx);
-    double val = vt[tid];
-    if (o.active_qv[tid] && j >= 1 && j <= ny - 1) val = val - f * grady_at(phi, o.fluid_p, i, j, k, nx, ny, o.dy);
-    v[tid] = val;
-  }
-  if (tid < Nw) {
-    long long pl = (long long)nx * ny;
-    int k = (int)(tid / pl);
-    long long rem = tid - k * pl;
-    int j = (int)(rem / nx);
-    int i = (int)(rem - (long long)j * nx);
-    double val = wt[tid];
-    if (o.active_qw[tid] && k >= 1 && k <= nz - 1) val = val - f * gradz_at(phi, o.fluid_p, i, j, k, nx, ny, o.dz);
-    w[tid] = val;
-  }
+  project_point(o, tid, phi, ut, vt, wt, u, v, w, qout);
 }
 
 __global__ void k_add_phi_box(double *p, const double *phi, const u8 *fluid, long long N) {
@@ -306,69 +188,11 @@ __global__ void k_add_phi_box(double *p, const double *phi, const u8 *fluid, lon
 __global__ void __launch_bounds__(256) k_diag_partials(fpc_flow_ops o, const double *u, const double *v, const double *w,
                                                        const double *p, const double *qout, double ramp,
                                                        double *part) {
-  __shared__ double lds[8];
-  int nx = o.nx, ny = o.ny, nz = o.nz;
-  long long Np = (long long)nx * ny * nz;
-  long long Nu = (long long)(nx + 1) * ny * nz, Nv = (long long)nx * (ny + 1) * nz, Nw = (long long)nx * ny * (nz + 1);
-  long long stride = (long long)gridDim.x * blockDim.x, t0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  double d2 = 0.0, mu = 0.0, mv = 0.0, mw = 0.0, fx = 0.0, fy = 0.0;
-  for (long long c = t0; c < Np; c += stride) {
-    if (!o.fluid_p[c]) continue;
-    int k = (int)(c / ((long long)nx * ny));
-    long long rem = c - (long long)k * nx * ny;
-    int j = (int)(rem / nx);
-    int i = (int)(rem - (long long)j * nx);
-    double s = div_at(o, u, v, w, qout, i, j, k);
-    double bc = (i == 0) ? o.div_bc[(long long)j + (long long)ny * k] : 0.0;
-    double d = s + ramp * bc;
-    d2 += d * d;
-  }
-  for (long long t = t0; t < Nu; t += stride) mu = fmax(mu, fabs(u[t]));
-  for (long long t = t0; t < Nv; t += stride) mv = fmax(mv, fabs(v[t]));
-  for (long long t = t0; t < Nw; t += stride) mw = fmax(mw, fabs(w[t]));
-  for (long long t = t0; t < o.n_fu; t += stride) fx += o.fu_w[t] * u[o.fu_idx[t]];
-  for (long long t = t0; t < o.n_px; t += stride) fx += o.px_w[t] * p[o.px_idx[t]];
-  for (long long t = t0; t < o.n_fv; t += stride) fy += o.fv_w[t] * v[o.fv_idx[t]];
-  for (long long t = t0; t < o.n_py; t += stride) fy += o.py_w[t] * p[o.py_idx[t]];
-  double vals[6] = {d2, mu, mv, mw, fx, fy};
-  for (int q = 0; q < 6; ++q) {
-    double x = vals[q];
-    bool isMax = (q >= 1 && q <= 3);
-    for (int off = 32; off > 0; off >>= 1) {
-      double y = __shfl_xor(x, off, 64);
-      x = isMax ? fmax(x, y) : x + y;
-    }
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = x;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double r = lds[0];
-      for (int wv = 1; wv < 4; ++wv) r = isMax ? fmax(r, lds[wv]) : r + lds[wv];
-      part[(long long)q * gridDim.x + blockIdx.x] = r;
-    }
-  }
+  diag_partials_body(o, u, v, w, p, qout, ramp, part, blockIdx.x, gridDim.x);
 }
 
 __global__ void __launch_bounds__(1024) k_diag_final(const double *part, int nb, int Np, double *out) {
-  __shared__ double lds[16];
-  for (int q = 0; q < 6; ++q) {
-    bool isMax = (q >= 1 && q <= 3);
-    double x = isMax ? 0.0 : 0.0;
-    for (int t = threadIdx.x; t < nb; t += blockDim.x) x = isMax ? fmax(x, part[(long long)q * nb + t]) : x + part[(long long)q * nb + t];
-    for (int off = 32; off > 0; off >>= 1) {
-      double y = __shfl_xor(x, off, 64);
-      x = isMax ? fmax(x, y) : x + y;
-    }
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = x;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double r = lds[0];
-      for (int wv = 1; wv < (int)(blockDim.x >> 6); ++wv) r = isMax ? fmax(r, lds[wv]) : r + lds[wv];
-      if (q == 0) r = sqrt(r / (double)(Np > 1 ? Np : 1));
-      out[q] = r;
-    }
-  }
+  diag_final_body(part, nb, Np, out);
 }
 
 __global__ void k_nonfinite(const double *x, long long n, int *flag) {

@@ -31,10 +31,20 @@
 #include <vector>
 
 #include "batched.hpp"
+#include "dense_device.hpp"
 #include "fpc_internal.hpp"
 #include "fpc_setup.hpp"
 #include "fpc_sim.h"
 #include "sim_internal.hpp"
+
+namespace fpc {
+struct BStage;
+int stage_create(const std::vector<fpc_sim> &sims, BStage **out);
+void stage_destroy(BStage *S);
+int stage_pre(BStage *S, const std::vector<fpc_sim> &sims);
+int stage_mid(BStage *S);
+int stage_post(BStage *S);
+}  // namespace fpc
 
 struct fpc_ensemble_s {
   std::vector<fpc_params> par;
@@ -59,6 +69,8 @@ struct fpc_ensemble_s {
   std::vector<int> its;
   std::vector<double> rel;
   std::vector<int> err;
+  // batched dense stages: per-instance descriptors and packed block tables
+  fpc::BStage *stage = nullptr;
 };
 
 namespace {
@@ -113,6 +125,261 @@ int dispatch(fpc_ensemble_s *E, int job, int nsteps) {
   return E->first_error.load();
 }
 
+}  // namespace
+
+namespace fpc {
+
+// One instance as the batched dense stages see it.
+struct BInst {
+  fpc_flow_ops o;
+  double *u, *v, *w, *p, *ut, *vt, *wt, *qout, *diag, *uin_step;
+  const double *uin;
+  double *rb[3], *xb[3];
+  double *pbp, *pxp;
+  int zero, Np;      // Np: fluid cells (the divL2 mean)
+  long long ncell;   // nx * ny * nz
+};
+
+constexpr int kDiagBlocks = 512;  // the partition of stage_diagnostics_async
+
+struct BTable {
+  int *item = nullptr, *first = nullptr;
+  int nblocks = 0;
+};
+
+struct BStage {
+  int n = 0;
+  BInst *inst = nullptr;
+  BTable inlet, faces, rows, scat, cells;
+  double *ramp = nullptr, *hramp = nullptr;     // per instance, device / pinned host
+  double *part = nullptr;                       // n * 6 * kDiagBlocks
+  double *ddiag = nullptr, *hdiag = nullptr;    // n * 8, device / pinned host
+};
+
+namespace {
+
+#define BS_POINT(count)                                              \
+  const int q = item[blockIdx.x];                                    \
+  const long long t = (long long)(blockIdx.x - first[q]) * 256 + threadIdx.x; \
+  if (t >= (count)) return
+
+__device__ inline long long faces_of(const fpc_flow_ops &o) {
+  const long long Nu = (long long)(o.nx + 1) * o.ny * o.nz, Nv = (long long)o.nx * (o.ny + 1) * o.nz,
+                  Nw = (long long)o.nx * o.ny * (o.nz + 1);
+  return Nu > Nv ? (Nu > Nw ? Nu : Nw) : (Nv > Nw ? Nv : Nw);
+}
+
+__global__ void kb_inlet(const BInst *I, const int *item, const int *first, const double *ramp) {
+  BS_POINT((long long)I[q].o.ny * I[q].o.nz);
+  I[q].uin_step[t] = ramp[q] * I[q].uin[t];
+}
+
+// which 0: BC cleanup of u, v, w; 1: of u~, v~, w~
+__global__ void kb_bc(const BInst *I, const int *item, const int *first, int which) {
+  BS_POINT(faces_of(I[q].o));
+  const BInst &B = I[q];
+  const fpc_flow_ops &o = B.o;
+  if (which == 0)
+    bc_point(t, B.u, B.v, B.w, o.active_u, o.active_v, o.active_w, B.uin_step, o.nx, o.ny, o.nz);
+  else
+    bc_point(t, B.ut, B.vt, B.wt, o.active_u, o.active_v, o.active_w, B.uin_step, o.nx, o.ny, o.nz);
+}
+
+// items 3 q + c: momentum box rows of component c of instance q
+__global__ void kb_explicit(const BInst *I, const int *item, const int *first) {
+  const int e = item[blockIdx.x];
+  const long long t = (long long)(blockIdx.x - first[e]) * 256 + threadIdx.x;
+  const BInst &B = I[e / 3];
+  const int c = e % 3;
+  const int bx = B.o.nx - (c == 0), by = B.o.ny - (c == 1), bz = B.o.nz - (c == 2);
+  if (t >= (long long)bx * by * bz) return;
+  const int i = (int)(t % bx), j = (int)((t / bx) % by), k = (int)(t / ((long long)bx * by));
+  explicit_row(c, B.o, B.u, B.v, B.w, B.p, B.rb[c], B.xb[c], B.zero, i, j, k);
+}
+
+// items 3 q + c: full face array of component c of instance q
+__global__ void kb_scatter(const BInst *I, const int *item, const int *first) {
+  const int e = item[blockIdx.x];
+  const long long t = (long long)(blockIdx.x - first[e]) * 256 + threadIdx.x;
+  const BInst &B = I[e / 3];
+  const int c = e % 3;
+  double *out = c == 0 ? B.ut : (c == 1 ? B.vt : B.wt);
+  const double *base = c == 0 ? B.u : (c == 1 ? B.v : B.w);
+  scatter_point(c, t, out, B.xb[c], base, B.o.nx, B.o.ny, B.o.nz);
+}
+
+__global__ void kb_divergence(const BInst *I, const int *item, const int *first, const double *ramp) {
+  BS_POINT(I[q].ncell);
+  const BInst &B = I[q];
+  const int nx = B.o.nx, ny = B.o.ny;
+  divergence_cell(B.o, B.ut, B.vt, B.wt, ramp[q], B.pbp, B.pxp, (int)(t % nx), (int)((t / nx) % ny),
+                  (int)(t / ((long long)nx * ny)));
+}
+
+__global__ void kb_project(const BInst *I, const int *item, const int *first) {
+  BS_POINT(faces_of(I[q].o));
+  const BInst &B = I[q];
+  project_point(B.o, t, B.pxp, B.ut, B.vt, B.wt, B.u, B.v, B.w, B.qout);
+}
+
+__global__ void kb_add_phi(const BInst *I, const int *item, const int *first) {
+  BS_POINT(I[q].ncell);
+  const BInst &B = I[q];
+  if (B.o.fluid_p[t]) B.p[t] += B.pxp[t];
+}
+
+__global__ void __launch_bounds__(256) kb_diag_partials(const BInst *I, const double *ramp, double *part) {
+  const BInst &B = I[blockIdx.y];
+  diag_partials_body(B.o, B.u, B.v, B.w, B.p, B.qout, ramp[blockIdx.y], part + (long long)blockIdx.y * 6 * kDiagBlocks,
+                     blockIdx.x, kDiagBlocks);
+}
+
+__global__ void __launch_bounds__(1024) kb_diag_final(const BInst *I, const double *part, double *out) {
+  const BInst &B = I[blockIdx.x];
+  double *o = out + (long long)blockIdx.x * 8;
+  diag_final_body(part + (long long)blockIdx.x * 6 * kDiagBlocks, kDiagBlocks, B.Np, o);
+  __syncthreads();
+  if (threadIdx.x < 6) B.diag[threadIdx.x] = o[threadIdx.x];
+}
+
+int btable(const std::vector<long long> &count, BTable &t) {
+  std::vector<int> item, first(count.size(), 0);
+  for (size_t e = 0; e < count.size(); ++e) {
+    first[e] = (int)item.size();
+    for (long long b = 0; b < (count[e] + 255) / 256; ++b) item.push_back((int)e);
+  }
+  t.nblocks = (int)item.size();
+  FPC_HIP(hipMalloc(&t.item, std::max<size_t>(1, item.size()) * sizeof(int)));
+  FPC_HIP(hipMalloc(&t.first, std::max<size_t>(1, first.size()) * sizeof(int)));
+  if (!item.empty()) FPC_HIP(hipMemcpy(t.item, item.data(), item.size() * sizeof(int), hipMemcpyHostToDevice));
+  FPC_HIP(hipMemcpy(t.first, first.data(), first.size() * sizeof(int), hipMemcpyHostToDevice));
+  return 0;
+}
+
+void bfree(BTable &t) {
+  if (t.item) (void)hipFree(t.item);
+  if (t.first) (void)hipFree(t.first);
+  t = BTable{};
+}
+
+}  // namespace
+
+void stage_destroy(BStage *S) {
+  if (!S) return;
+  for (BTable *t : {&S->inlet, &S->faces, &S->rows, &S->scat, &S->cells}) bfree(*t);
+  if (S->inst) (void)hipFree(S->inst);
+  if (S->ramp) (void)hipFree(S->ramp);
+  if (S->part) (void)hipFree(S->part);
+  if (S->ddiag) (void)hipFree(S->ddiag);
+  if (S->hramp) (void)hipHostFree(S->hramp);
+  if (S->hdiag) (void)hipHostFree(S->hdiag);
+  delete S;
+}
+
+int stage_create(const std::vector<fpc_sim> &sims, BStage **out) {
+  BStage *S = new BStage();
+  const int n = (int)sims.size();
+  S->n = n;
+  std::vector<BInst> hi((size_t)n);
+  std::vector<long long> inlet, faces, rows, scat, cells;
+  for (int q = 0; q < n; ++q) {
+    fpc_sim M = sims[(size_t)q];
+    BInst &B = hi[(size_t)q];
+    B.o = M->ops;
+    B.u = M->u; B.v = M->v; B.w = M->w; B.p = M->p; B.ut = M->ut; B.vt = M->vt; B.wt = M->wt;
+    B.qout = M->qout; B.diag = M->diag; B.uin_step = M->uin_step; B.uin = M->uin;
+    B.rb[0] = M->pbu; B.rb[1] = M->pbv; B.rb[2] = M->pbw;
+    B.xb[0] = M->pxu; B.xb[1] = M->pxv; B.xb[2] = M->pxw;
+    B.pbp = M->pbp; B.pxp = M->pxp;
+    B.zero = M->par.use_zero_initial_guess != 0;
+    B.Np = M->S.Np_;
+    const long long nx = M->par.nx, ny = M->par.ny, nz = M->par.nz;
+    B.ncell = nx * ny * nz;
+    const long long Nu = (nx + 1) * ny * nz, Nv = nx * (ny + 1) * nz, Nw = nx * ny * (nz + 1);
+    inlet.push_back(ny * nz);
+    faces.push_back(std::max(Nu, std::max(Nv, Nw)));
+    rows.push_back((nx - 1) * ny * nz);
+    rows.push_back(nx * (ny - 1) * nz);
+    rows.push_back(nx * ny * (nz - 1));
+    scat.push_back(Nu);
+    scat.push_back(Nv);
+    scat.push_back(Nw);
+    cells.push_back(nx * ny * nz);
+  }
+  int rc = btable(inlet, S->inlet);
+  if (!rc) rc = btable(faces, S->faces);
+  if (!rc) rc = btable(rows, S->rows);
+  if (!rc) rc = btable(scat, S->scat);
+  if (!rc) rc = btable(cells, S->cells);
+  if (!rc && (hipMalloc(&S->inst, (size_t)n * sizeof(BInst)) != hipSuccess ||
+              hipMalloc(&S->ramp, (size_t)n * sizeof(double)) != hipSuccess ||
+              hipMalloc(&S->part, (size_t)n * 6 * kDiagBlocks * sizeof(double)) != hipSuccess ||
+              hipMalloc(&S->ddiag, (size_t)n * 8 * sizeof(double)) != hipSuccess ||
+              hipHostMalloc(&S->hramp, (size_t)n * sizeof(double)) != hipSuccess ||
+              hipHostMalloc(&S->hdiag, (size_t)n * 8 * sizeof(double)) != hipSuccess)) {
+    set_error(FPC_ERROR_MEMORY, "batched stages: allocation failed");
+    rc = FPC_ERROR_MEMORY;
+  }
+  if (!rc && hipMemcpy(S->inst, hi.data(), (size_t)n * sizeof(BInst), hipMemcpyHostToDevice) != hipSuccess)
+    rc = FPC_ERROR_GENERIC;
+  if (!rc && hipMemset(S->ddiag, 0, (size_t)n * 8 * sizeof(double)) != hipSuccess) rc = FPC_ERROR_GENERIC;
+  if (rc) {
+    stage_destroy(S);
+    return rc;
+  }
+  *out = S;
+  return 0;
+}
+
+// Phase 0 (ramp, inlet) and phase 1 (BC cleanup, momentum rhs and x0) of every instance.
+int stage_pre(BStage *S, const std::vector<fpc_sim> &sims) {
+  for (int q = 0; q < S->n; ++q) {
+    fpc_sim M = sims[(size_t)q];
+    M->step += 1;  // main.cu:250-251
+    M->ramp = std::min(1.0, (double)M->step / (double)M->par.inlet_ramp_steps);
+    S->hramp[q] = M->ramp;
+  }
+  FPC_HIP(hipMemcpyAsync(S->ramp, S->hramp, (size_t)S->n * sizeof(double), hipMemcpyHostToDevice, stream()));
+  kb_inlet<<<S->inlet.nblocks, 256, 0, stream()>>>(S->inst, S->inlet.item, S->inlet.first, S->ramp);
+  FPC_LAUNCH_CHECK();
+  kb_bc<<<S->faces.nblocks, 256, 0, stream()>>>(S->inst, S->faces.item, S->faces.first, 0);
+  FPC_LAUNCH_CHECK();
+  kb_explicit<<<S->rows.nblocks, 256, 0, stream()>>>(S->inst, S->rows.item, S->rows.first);
+  FPC_LAUNCH_CHECK();
+  return 0;
+}
+
+// Phases 3 and 4: scatter, BC cleanup of the tilde fields, divergence and pressure rhs.
+int stage_mid(BStage *S) {
+  kb_scatter<<<S->scat.nblocks, 256, 0, stream()>>>(S->inst, S->scat.item, S->scat.first);
+  FPC_LAUNCH_CHECK();
+  kb_bc<<<S->faces.nblocks, 256, 0, stream()>>>(S->inst, S->faces.item, S->faces.first, 1);
+  FPC_LAUNCH_CHECK();
+  kb_divergence<<<S->cells.nblocks, 256, 0, stream()>>>(S->inst, S->cells.item, S->cells.first, S->ramp);
+  FPC_LAUNCH_CHECK();
+  return 0;
+}
+
+// Phases 6 and 7: projection, BC cleanup, p += phi, diagnostics (copied to the host).
+int stage_post(BStage *S) {
+  kb_project<<<S->faces.nblocks, 256, 0, stream()>>>(S->inst, S->faces.item, S->faces.first);
+  FPC_LAUNCH_CHECK();
+  kb_bc<<<S->faces.nblocks, 256, 0, stream()>>>(S->inst, S->faces.item, S->faces.first, 0);
+  FPC_LAUNCH_CHECK();
+  kb_add_phi<<<S->cells.nblocks, 256, 0, stream()>>>(S->inst, S->cells.item, S->cells.first);
+  FPC_LAUNCH_CHECK();
+  kb_diag_partials<<<dim3(kDiagBlocks, S->n), 256, 0, stream()>>>(S->inst, S->ramp, S->part);
+  FPC_LAUNCH_CHECK();
+  kb_diag_final<<<S->n, 1024, 0, stream()>>>(S->inst, S->part, S->ddiag);
+  FPC_LAUNCH_CHECK();
+  FPC_HIP(hipMemcpyAsync(S->hdiag, S->ddiag, (size_t)S->n * 8 * sizeof(double), hipMemcpyDeviceToHost, stream()));
+  return 0;
+}
+
+}  // namespace fpc
+
+namespace {
+
 void stop_workers(fpc_ensemble_s *E) {
   {
     std::lock_guard<std::mutex> lk(E->mu);
@@ -158,6 +425,7 @@ int build_batched(fpc_ensemble_s *E) {
   if (!rc) rc = batch_group_create(sp, bp, xp, &E->pres);
   if (!rc) rc = nonfinite_create(xm_c, lm, &E->nf_mom);
   if (!rc) rc = nonfinite_create(xp_c, lp, &E->nf_p);
+  if (!rc) rc = stage_create(E->sims, &E->stage);
   E->its.assign(sm.size(), 0);
   E->rel.assign(sm.size(), 0.0);
   E->err.assign(sm.size(), 0);
@@ -169,13 +437,8 @@ int batched_step(fpc_ensemble_s *E) {
   using namespace fpc;
   StreamScope scope(E->stream);
   const int n = (int)E->sims.size();
-  int rc = 0;
-  for (fpc_sim M : E->sims)
-    if ((rc = sim_run_phase(M, 0))) return rc;
-  for (fpc_sim M : E->sims)
-    if ((rc = stage_explicit_async(&M->ops, M->u, M->v, M->w, M->p, M->uin_step, M->pbu, M->pxu, M->pbv, M->pxv,
-                                   M->pbw, M->pxw, M->par.use_zero_initial_guess != 0)))
-      return rc;
+  int rc = stage_pre(E->stage, E->sims);
+  if (rc) return rc;
   if ((rc = batch_group_solve(E->mom, E->its.data(), E->rel.data(), E->err.data()))) return rc;
   for (int i = 0; i < n; ++i)
     for (int c = 0; c < 3; ++c) {
@@ -193,8 +456,7 @@ int batched_step(fpc_ensemble_s *E) {
       return 911;
     }
   }
-  for (fpc_sim M : E->sims)
-    if ((rc = sim_run_phase(M, 3)) || (rc = sim_run_phase(M, 4))) return rc;
+  if ((rc = stage_mid(E->stage))) return rc;
   if ((rc = batch_group_solve(E->pres, E->its.data(), E->rel.data(), E->err.data()))) return rc;
   for (int i = 0; i < n; ++i) {
     E->sims[(size_t)i]->its[3] = E->its[(size_t)i];
@@ -202,15 +464,9 @@ int batched_step(fpc_ensemble_s *E) {
     E->sims[(size_t)i]->err[3] = E->err[(size_t)i];
   }
   if ((rc = nonfinite_enqueue(E->nf_p))) return rc;
-  for (fpc_sim M : E->sims) {
-    if ((rc = stage_projection_async(&M->ops, M->pxp, M->ut, M->vt, M->wt, M->u, M->v, M->w, M->p, M->uin_step,
-                                     M->qout)))
-      return rc;
-    if ((rc = stage_diagnostics_async(&M->ops, M->u, M->v, M->w, M->p, M->qout, M->ramp, M->S.Np_, M->diag)))
-      return rc;
-    FPC_HIP(hipMemcpyAsync(M->hdiag, M->diag, 8 * sizeof(double), hipMemcpyDeviceToHost, stream()));
-  }
+  if ((rc = stage_post(E->stage))) return rc;
   FPC_HIP(hipStreamSynchronize(stream()));
+  for (int i = 0; i < n; ++i) std::memcpy(E->sims[(size_t)i]->hdiag, E->stage->hdiag + 8 * i, 8 * sizeof(double));
   for (int i = 0; i < n; ++i)
     if (nonfinite_flags(E->nf_p)[i]) {
       set_error(FPC_ERROR_GENERIC, "non-finite pressure solve output at step %d of instance %d (reference exit 912)",
@@ -323,6 +579,7 @@ int fpc_ensemble_destroy(fpc_ensemble E) {
   if (E->pres) fpc::batch_group_destroy(E->pres);
   if (E->nf_mom) fpc::nonfinite_destroy(E->nf_mom);
   if (E->nf_p) fpc::nonfinite_destroy(E->nf_p);
+  if (E->stage) fpc::stage_destroy(E->stage);
   if (E->stream) (void)hipStreamDestroy(E->stream);
   for (fpc_sim s : E->sims)
     if (s) fpc_sim_destroy(s);
