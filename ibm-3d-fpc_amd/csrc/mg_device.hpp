@@ -90,11 +90,20 @@ __device__ inline T up_point(const T *__restrict__ c, long long pitch, int nx, i
 }
 
 // Row r of the coarsest solve x = inv b (FP64 sums), one wavefront; valid in lane 0.
+// Four independent partial sums per lane keep four row loads in flight.
 template <typename T>
 __device__ inline double gemv_row(const double *__restrict__ inv, int n, const T *__restrict__ b, int r, int lane) {
   const double *row = inv + (size_t)r * n;
-  double s = 0.0;
-  for (int q = lane; q < n; q += 64) s += row[q] * (double)b[q];
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int q = lane;
+  for (; q + 192 < n; q += 256) {
+    s0 += row[q] * (double)b[q];
+    s1 += row[q + 64] * (double)b[q + 64];
+    s2 += row[q + 128] * (double)b[q + 128];
+    s3 += row[q + 192] * (double)b[q + 192];
+  }
+  for (; q < n; q += 64) s0 += row[q] * (double)b[q];
+  double s = (s0 + s1) + (s2 + s3);
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
   return s;
 }
