@@ -211,7 +211,7 @@ __global__ void __launch_bounds__(256) k_b_combine(const BSys *sys, Sub sub, int
       const double v = kry::combine_point(s.V, s.vp, t, y, i, s.dinv);
       s.x[t] = s.x[t] + v;
     } else {
-      s.W[t] = kry::combine_point(s.V, s.vp, t, y, i, nullptr);
+      s.W[t] = kry::combine_point<double>(s.V, s.vp, t, y, i, nullptr);
     }
   }
 }

@@ -176,6 +176,7 @@ enum SolverKind { SK_NONE = 0, SK_GMRES = 1, SK_FGMRES = 2, SK_JACOBI = 3, SK_PF
 
 // Native preconditioners used inside GMRES (jacobi.hip, pfmg.hip).
 struct JacobiImpl {
+  int bits = 64;  // 32: 1/diag stored in FP32 (the products stay FP64)
   int maxit = 1;
   double tol = 0.0;
   bool zero_guess = false;

@@ -39,9 +39,10 @@ using namespace kry;
 
 // w = A z with z = sigma * p (* dinv when Jacobi is fused), one thread per point.
 // The current basis index is read from the device status.
+template <typename DT>
 __global__ void k_arnoldi_matvec(const double *__restrict__ c, long long pitch, int nx, int ny, int nz,
                                  const double *__restrict__ V, long long vp, const double *__restrict__ zin,
-                                 const double *__restrict__ dinv, State *S) {
+                                 const DT *__restrict__ dinv, State *S) {
   if (S->st.done) return;
   int cur = S->st.i;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -56,35 +57,36 @@ __global__ void k_arnoldi_matvec(const double *__restrict__ c, long long pitch, 
   w[r] = sg * s;
 }
 
-template <bool kDinv, int TY>
+template <bool kDinv, int TY, typename DT>
 __global__ void __launch_bounds__(kTileX *TY)
     k_arnoldi_matvec_tiled(const double *__restrict__ c, long long pitch, int nx, int ny, int nz,
                            const double *__restrict__ V, long long vp, const double *__restrict__ zin,
-                           const double *__restrict__ dinv, State *S, int kz) {
+                           const DT *__restrict__ dinv, State *S, int kz) {
   if (S->st.done) return;
   const int cur = S->st.i;
-  ArnoldiOp<kDinv> op{c, pitch, nx, ny, zin ? zin : V + (long long)cur * vp, dinv,
-                      (double *)V + (long long)(cur + 1) * vp, S->sigma[cur]};
+  ArnoldiOp<kDinv, DT> op{c, pitch, nx, ny, zin ? zin : V + (long long)cur * vp, dinv,
+                          (double *)V + (long long)(cur + 1) * vp, S->sigma[cur]};
   const int k0 = blockIdx.z * kz;
   march_tile<double, kTileX, TY>(op, nx, ny, nz, k0, min(k0 + kz, nz), blockIdx.x, blockIdx.y);
 }
 
 // w = sigma A z (z * 1/diag when dinv is given) for the current basis index.
+template <typename DT>
 int launch_arnoldi_matvec(const fpc_StructMatrix_s *A, const double *V, long long vp, const double *zin,
-                          const double *dinv, State *S) {
+                          const DT *dinv, State *S) {
   Dims d = dims_of(A->grid);
   if (tile_worthwhile(d)) {
     TileLaunch tl = tile_launch<8>(d);
     if (dinv)
-      k_arnoldi_matvec_tiled<true, 8><<<tl.grid, tl.block, 0, stream()>>>(A->coef, A->pitch, d.nx, d.ny, d.nz, V, vp,
-                                                                          zin, dinv, S, tl.kz);
+      k_arnoldi_matvec_tiled<true, 8, DT><<<tl.grid, tl.block, 0, stream()>>>(A->coef, A->pitch, d.nx, d.ny, d.nz, V,
+                                                                              vp, zin, dinv, S, tl.kz);
     else
-      k_arnoldi_matvec_tiled<false, 8><<<tl.grid, tl.block, 0, stream()>>>(A->coef, A->pitch, d.nx, d.ny, d.nz, V,
-                                                                           vp, zin, nullptr, S, tl.kz);
+      k_arnoldi_matvec_tiled<false, 8, DT><<<tl.grid, tl.block, 0, stream()>>>(A->coef, A->pitch, d.nx, d.ny, d.nz,
+                                                                               V, vp, zin, nullptr, S, tl.kz);
   } else {
     dim3 g, b;
     launch_shape(d, g, b);
-    k_arnoldi_matvec<<<g, b, 0, stream()>>>(A->coef, A->pitch, d.nx, d.ny, d.nz, V, vp, zin, dinv, S);
+    k_arnoldi_matvec<DT><<<g, b, 0, stream()>>>(A->coef, A->pitch, d.nx, d.ny, d.nz, V, vp, zin, dinv, S);
   }
   FPC_LAUNCH_CHECK();
   return 0;
@@ -123,8 +125,9 @@ __global__ void k_backsolve(State *S) {
 }
 
 // out (+)= [dinv *] sum_j y_j P_j   (x update or the preconditioner input)
+template <typename DT>
 __global__ void k_combine(const double *__restrict__ P, long long vp, long long n, const State *S,
-                          const double *__restrict__ dinv, double *__restrict__ out, int accumulate) {
+                          const DT *__restrict__ dinv, double *__restrict__ out, int accumulate) {
   __shared__ double y[KM + 1];
   int i = S->st.i;
   for (int q = threadIdx.x; q < i; q += blockDim.x) y[q] = S->y[q];
@@ -171,14 +174,16 @@ __global__ void k_set_done(State *S, int v) {
 
 // ---------------------------------------------------------------- Jacobi
 namespace {
-__global__ void k_jacobi_zero(const double *__restrict__ dinv, const double *__restrict__ b, double *__restrict__ x,
+template <typename DT>
+__global__ void k_jacobi_zero(const DT *__restrict__ dinv, const double *__restrict__ b, double *__restrict__ x,
                               long long n, const int *skip) {
   if (skip && *skip) return;
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
-    x[t] = b[t] * dinv[t];
+    x[t] = b[t] * (double)dinv[t];
 }
+template <typename DT>
 __global__ void k_jacobi_sweep(const double *__restrict__ c, long long pitch, int nx, int ny, int nz,
-                               const double *__restrict__ dinv, const double *__restrict__ b,
+                               const DT *__restrict__ dinv, const double *__restrict__ b,
                                const double *__restrict__ x, double *__restrict__ xn, const int *skip) {
   if (skip && *skip) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -193,7 +198,7 @@ __global__ void k_jacobi_sweep(const double *__restrict__ c, long long pitch, in
   if (j < ny - 1) s += c[4 * pitch + r] * x[r + nx];
   if (k > 0) s += c[5 * pitch + r] * x[r - pl];
   if (k < nz - 1) s += c[6 * pitch + r] * x[r + pl];
-  xn[r] = x[r] + (b[r] - s) * dinv[r];
+  xn[r] = x[r] + (b[r] - s) * (double)dinv[r];
 }
 unsigned nblk(long long n) {
   long long b = (n + 255) / 256;
@@ -206,7 +211,12 @@ int jacobi_apply(JacobiImpl *J, const double *b, double *x, const int *skip) {
   int rc = ensure_dinv(A);
   if (rc) return rc;
   long long n = A->grid.N;
-  k_jacobi_zero<<<nblk(n), 256, 0, stream()>>>(A->dinv, b, x, n, skip);
+  if (J->bits == 32) {
+    if ((rc = ensure_fp32(A))) return rc;
+    k_jacobi_zero<float><<<nblk(n), 256, 0, stream()>>>(A->dinv32, b, x, n, skip);
+  } else {
+    k_jacobi_zero<double><<<nblk(n), 256, 0, stream()>>>(A->dinv, b, x, n, skip);
+  }
   FPC_LAUNCH_CHECK();
   if (J->maxit > 1) {
     double *t = scratch_doubles((size_t)n, 3);
@@ -215,7 +225,10 @@ int jacobi_apply(JacobiImpl *J, const double *b, double *x, const int *skip) {
     dim3 g, bl;
     launch_shape(d, g, bl);
     for (int it = 1; it < J->maxit; ++it) {
-      k_jacobi_sweep<<<g, bl, 0, stream()>>>(A->coef, A->pitch, d.nx, d.ny, d.nz, A->dinv, b, x, t, skip);
+      if (J->bits == 32)
+        k_jacobi_sweep<float><<<g, bl, 0, stream()>>>(A->coef, A->pitch, d.nx, d.ny, d.nz, A->dinv32, b, x, t, skip);
+      else
+        k_jacobi_sweep<double><<<g, bl, 0, stream()>>>(A->coef, A->pitch, d.nx, d.ny, d.nz, A->dinv, b, x, t, skip);
       FPC_LAUNCH_CHECK();
       rc = vec_copy(x, t, n);
       if (rc) return rc;
@@ -307,7 +320,7 @@ int gmres_batch_view(fpc_StructSolver s, GmresBatchView *v) {
     return FPC_ERROR_ARG;
   }
   if (G->pc_kind == PC_NONE) v->pc = 0;
-  else if (G->pc_kind == PC_JACOBI1) v->pc = 1;
+  else if (G->pc_kind == PC_JACOBI1 && ((JacobiImpl *)G->pc->impl)->bits == 64) v->pc = 1;
   else if (G->pc_kind == PC_PFMG) v->pc = 3;
   else {
     set_error(FPC_ERROR_ARG, "batched solve: only Jacobi (one sweep) and PFMG preconditioners are supported");
@@ -417,6 +430,7 @@ int fpc_StructGMRESSetup(fpc_StructSolver s, fpc_StructMatrix A, fpc_StructVecto
     JacobiImpl *J = (JacobiImpl *)G->pc->impl;
     J->A = A;
     G->pc_kind = (J->maxit <= 1 && !G->flexible) ? PC_JACOBI1 : PC_JACOBI;
+    if (J->bits == 32 && (rc = ensure_fp32(A))) return rc;
   }
   if (G->pc_kind == PC_PFMG) {
     rc = pfmg_setup(G->pc->impl, A);
@@ -449,11 +463,13 @@ static int true_residual(GmresImpl *G, const double *b, const double *x, double 
 // One Arnoldi matvec with no preconditioner or with the fused one-sweep Jacobi.
 static int arnoldi_step(GmresImpl *G) {
   Dims d = dims_of(G->A->grid);
-  const double *dinv = (G->pc_kind == PC_JACOBI1) ? G->A->dinv : nullptr;
-  // bytes per row: 7 coefficients, the input, 1/diag when Jacobi is fused, the output
-  FPC_PROF_BEGIN(dinv ? "spmv_jacobi" : "spmv");
-  int rc = launch_arnoldi_matvec(G->A, G->V, G->vp, nullptr, dinv, G->S);
-  FPC_PROF_END(dinv ? "spmv_jacobi" : "spmv", (double)d.N * (dinv ? 80.0 : 72.0));
+  const bool jac = G->pc_kind == PC_JACOBI1;
+  const bool j32 = jac && ((JacobiImpl *)G->pc->impl)->bits == 32;
+  // bytes per row: 7 coefficients, the input, 1/diag when Jacobi is fused (4 bytes in FP32), the output
+  FPC_PROF_BEGIN(jac ? "spmv_jacobi" : "spmv");
+  int rc = j32 ? launch_arnoldi_matvec<float>(G->A, G->V, G->vp, nullptr, G->A->dinv32, G->S)
+               : launch_arnoldi_matvec<double>(G->A, G->V, G->vp, nullptr, jac ? G->A->dinv : nullptr, G->S);
+  FPC_PROF_END(jac ? "spmv_jacobi" : "spmv", (double)d.N * (jac ? (j32 ? 76.0 : 80.0) : 72.0));
   return rc;
 }
 
@@ -491,7 +507,7 @@ static int arnoldi_step_pc(GmresImpl *G, int step) {
   int rc = precond_apply(G, G->V + (long long)step * G->vp, z, skip);
   if (rc) return rc;
   FPC_PROF_BEGIN("spmv");
-  rc = launch_arnoldi_matvec(G->A, G->V, G->vp, z, nullptr, G->S);
+  rc = launch_arnoldi_matvec<double>(G->A, G->V, G->vp, z, nullptr, G->S);
   FPC_PROF_END("spmv", (double)d.N * 72.0);
   return rc;
 }
@@ -578,13 +594,17 @@ int fpc_StructGMRESSolve(fpc_StructSolver s, fpc_StructMatrix A, fpc_StructVecto
     k_backsolve<<<1, 1, 0, stream()>>>(G->S);
     FPC_LAUNCH_CHECK();
     if (G->flexible) {
-      k_combine<<<nblk_n, 256, 0, stream()>>>(G->Z, G->vp, N, G->S, nullptr, x, 1);
+      k_combine<double><<<nblk_n, 256, 0, stream()>>>(G->Z, G->vp, N, G->S, nullptr, x, 1);
+      FPC_LAUNCH_CHECK();
+    } else if (G->pc_kind == PC_JACOBI1 && ((JacobiImpl *)G->pc->impl)->bits == 32) {
+      k_combine<float><<<nblk_n, 256, 0, stream()>>>(G->V, G->vp, N, G->S, A->dinv32, x, 1);
       FPC_LAUNCH_CHECK();
     } else if (G->pc_kind == PC_JACOBI1 || G->pc_kind == PC_NONE) {
-      k_combine<<<nblk_n, 256, 0, stream()>>>(G->V, G->vp, N, G->S, G->pc_kind == PC_JACOBI1 ? A->dinv : nullptr, x, 1);
+      k_combine<double><<<nblk_n, 256, 0, stream()>>>(G->V, G->vp, N, G->S, G->pc_kind == PC_JACOBI1 ? A->dinv : nullptr,
+                                                      x, 1);
       FPC_LAUNCH_CHECK();
     } else {
-      k_combine<<<nblk_n, 256, 0, stream()>>>(G->V, G->vp, N, G->S, nullptr, G->W, 0);
+      k_combine<double><<<nblk_n, 256, 0, stream()>>>(G->V, G->vp, N, G->S, nullptr, G->W, 0);
       FPC_LAUNCH_CHECK();
       rc = precond_apply(G, G->W, G->Z, nullptr);
       if (rc) return rc;
@@ -661,6 +681,12 @@ int fpc_StructJacobiCreate(fpc_StructSolver *solver) {
 int fpc_StructJacobiSetMaxIter(fpc_StructSolver s, int maxit) { JIMPL(s); J->maxit = std::max(1, maxit); return 0; }
 int fpc_StructJacobiSetTol(fpc_StructSolver s, double tol) { JIMPL(s); J->tol = tol; return 0; }
 int fpc_StructJacobiSetZeroGuess(fpc_StructSolver s) { JIMPL(s); J->zero_guess = true; return 0; }
+int fpc_StructJacobiSetPrecision(fpc_StructSolver s, int bits) {
+  JIMPL(s);
+  FPC_CHECK_ARG(bits == 32 || bits == 64, "bits must be 32 or 64");
+  J->bits = bits;
+  return 0;
+}
 int fpc_StructJacobiSetNonZeroGuess(fpc_StructSolver s) { JIMPL(s); J->zero_guess = false; return 0; }
 
 int fpc_StructJacobiSetup(fpc_StructSolver s, fpc_StructMatrix A, fpc_StructVector b, fpc_StructVector x) {
@@ -682,6 +708,7 @@ int fpc_StructJacobiSolve(fpc_StructSolver s, fpc_StructMatrix A, fpc_StructVect
   if (J->zero_guess) {
     rc = jacobi_apply(J, b->data, x->data, nullptr);
   } else {
+    if (J->bits == 32 && (rc = ensure_fp32(A))) return rc;
     long long n = A->grid.N;
     double *t = scratch_doubles((size_t)n, 4);
     if (!t) return FPC_ERROR_MEMORY;
@@ -689,7 +716,12 @@ int fpc_StructJacobiSolve(fpc_StructSolver s, fpc_StructMatrix A, fpc_StructVect
     dim3 g, bl;
     launch_shape(d, g, bl);
     for (int it = 0; it < J->maxit; ++it) {
-      k_jacobi_sweep<<<g, bl, 0, stream()>>>(A->coef, A->pitch, d.nx, d.ny, d.nz, A->dinv, b->data, x->data, t, nullptr);
+      if (J->bits == 32)
+        k_jacobi_sweep<float><<<g, bl, 0, stream()>>>(A->coef, A->pitch, d.nx, d.ny, d.nz, A->dinv32, b->data, x->data, t,
+                                                      nullptr);
+      else
+        k_jacobi_sweep<double><<<g, bl, 0, stream()>>>(A->coef, A->pitch, d.nx, d.ny, d.nz, A->dinv, b->data, x->data,
+                                                       t, nullptr);
       FPC_LAUNCH_CHECK();
       rc = vec_copy(x->data, t, n);
       if (rc) return rc;

@@ -43,19 +43,21 @@ struct State {
 __device__ inline double &H(State *S, int row, int col) { return S->hh[col * (KM + 1) + row]; }
 
 // (A z)_r at point (i, j, k) = row r, with z scaled by 1/diag when dinv is given.
+// DT: the stored 1/diag (double, or float for the FP32 Jacobi); the products are FP64.
+template <typename DT = double>
 __device__ inline double arnoldi_row(const double *__restrict__ c, long long pitch, int nx, int ny, int nz,
-                                     const double *__restrict__ z, const double *__restrict__ dinv, int i, int j,
-                                     int k, long long r) {
+                                     const double *__restrict__ z, const DT *__restrict__ dinv, int i, int j, int k,
+                                     long long r) {
   const long long pl = (long long)nx * ny;
   double s;
   if (dinv) {
-    s = c[r] * (z[r] * dinv[r]);
-    if (i > 0) s += c[pitch + r] * (z[r - 1] * dinv[r - 1]);
-    if (i < nx - 1) s += c[2 * pitch + r] * (z[r + 1] * dinv[r + 1]);
-    if (j > 0) s += c[3 * pitch + r] * (z[r - nx] * dinv[r - nx]);
-    if (j < ny - 1) s += c[4 * pitch + r] * (z[r + nx] * dinv[r + nx]);
-    if (k > 0) s += c[5 * pitch + r] * (z[r - pl] * dinv[r - pl]);
-    if (k < nz - 1) s += c[6 * pitch + r] * (z[r + pl] * dinv[r + pl]);
+    s = c[r] * (z[r] * (double)dinv[r]);
+    if (i > 0) s += c[pitch + r] * (z[r - 1] * (double)dinv[r - 1]);
+    if (i < nx - 1) s += c[2 * pitch + r] * (z[r + 1] * (double)dinv[r + 1]);
+    if (j > 0) s += c[3 * pitch + r] * (z[r - nx] * (double)dinv[r - nx]);
+    if (j < ny - 1) s += c[4 * pitch + r] * (z[r + nx] * (double)dinv[r + nx]);
+    if (k > 0) s += c[5 * pitch + r] * (z[r - pl] * (double)dinv[r - pl]);
+    if (k < nz - 1) s += c[6 * pitch + r] * (z[r + pl] * (double)dinv[r + pl]);
   } else {
     s = c[r] * z[r];
     if (i > 0) s += c[pitch + r] * z[r - 1];
@@ -210,11 +212,12 @@ __device__ inline void backsolve_body(State *S) {
 }
 
 // [dinv *] sum_j y_j P_j at point t (y staged in LDS, i terms).
+template <typename DT = double>
 __device__ inline double combine_point(const double *__restrict__ P, long long vp, long long t, const double *y, int i,
-                                       const double *__restrict__ dinv) {
+                                       const DT *__restrict__ dinv) {
   double v = y[i - 1] * P[(long long)(i - 1) * vp + t];
   for (int q = i - 2; q >= 0; --q) v += y[q] * P[(long long)q * vp + t];
-  if (dinv) v *= dinv[t];
+  if (dinv) v *= (double)dinv[t];
   return v;
 }
 
@@ -294,12 +297,13 @@ __device__ inline void reduce_body(const double *__restrict__ part, int n, doubl
 }
 
 // The same matvec on z-marching tiles (each row of A and z read once).
-template <bool kDinv>
+template <bool kDinv, typename DT = double>
 struct ArnoldiOp {
   const double *c;
   long long pitch;
   int nx, ny;
-  const double *z, *dinv;
+  const double *z;
+  const DT *dinv;
   double *w;
   double sg;
   struct Raw {
@@ -311,7 +315,7 @@ struct ArnoldiOp {
   __device__ long long at(int i, int j, int k) const { return ((long long)k * ny + j) * nx + i; }
   __device__ Raw fetch(int i, int j, int k) const {
     long long t = at(i, j, k);
-    return Raw{z[t], kDinv ? dinv[t] : 1.0};
+    return Raw{z[t], kDinv ? (double)dinv[t] : 1.0};
   }
   __device__ double finish(const Raw &r) const { return kDinv ? r.z * r.dinv : r.z; }
   __device__ Row row(int i, int j, int k) const {

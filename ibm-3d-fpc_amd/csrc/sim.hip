@@ -12,7 +12,8 @@
 //  * the non-finite checks run on the device (exit codes 911 / 912 become
 //    error returns);
 //  * with mg_precision = 32 the pressure solve is flexible GMRES with the FP32
-//    PFMG (no reference counterpart).
+//    PFMG, and the momentum Jacobi preconditioners keep 1/diag in FP32 (no
+//    reference counterpart).
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -66,6 +67,8 @@ static int setup_solver_vel(fpc_StructSolver *s, fpc_StructSolver *pc, const fpc
   rc |= fpc_StructJacobiCreate(pc);
   rc |= fpc_StructJacobiSetMaxIter(*pc, par.vel_jacobi_iters);
   rc |= fpc_StructJacobiSetZeroGuess(*pc);
+  // mixed precision: the momentum Jacobi preconditioner on its stored FP32 1/diag
+  rc |= fpc_StructJacobiSetPrecision(*pc, par.mg_precision == 32 ? 32 : 64);
   rc |= fpc_StructGMRESSetPrecond(*s, fpc_StructJacobiSolve, fpc_StructJacobiSetup, *pc);
   if (rc) return rc;
   return fpc_StructGMRESSetup(*s, A, b, x);

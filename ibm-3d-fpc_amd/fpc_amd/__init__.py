@@ -99,6 +99,7 @@ SIGNATURES = {
     "fpc_StructGMRESGetFinalRelativeResidualNorm": (_i, [_vp, _pd]), "fpc_StructGMRESDestroy": (_i, [_vp]),
     "fpc_StructJacobiCreate": (_i, [_pvp]), "fpc_StructJacobiSetMaxIter": (_i, [_vp, _i]),
     "fpc_StructJacobiSetTol": (_i, [_vp, _d]), "fpc_StructJacobiSetZeroGuess": (_i, [_vp]),
+    "fpc_StructJacobiSetPrecision": (_i, [_vp, _i]),
     "fpc_StructJacobiSetNonZeroGuess": (_i, [_vp]), "fpc_StructJacobiSetup": (_i, [_vp, _vp, _vp, _vp]),
     "fpc_StructJacobiSolve": (_i, [_vp, _vp, _vp, _vp]), "fpc_StructJacobiDestroy": (_i, [_vp]),
     "fpc_StructPFMGCreate": (_i, [_pvp]), "fpc_StructPFMGSetMaxIter": (_i, [_vp, _i]),
@@ -541,7 +542,7 @@ class Solver:
     """GMRES (or flexible GMRES) with a Jacobi / PFMG / no preconditioner, set up as main.cu:173-221."""
 
     def __init__(self, box, tol=1e-8, maxit=250, kdim=30, precond="jacobi", pc_iters=1, relax_type=1,
-                 rap_type=1, flexible=False, mg_bits=64):
+                 rap_type=1, flexible=False, mg_bits=64, jacobi_bits=64):
         L = lib()
         self.box = box
         self.s = ctypes.c_void_p()
@@ -555,6 +556,7 @@ class Solver:
             check(L.fpc_StructJacobiCreate(ctypes.byref(self.pc)), "jacobi")
             check(L.fpc_StructJacobiSetMaxIter(self.pc, pc_iters), "jacobi maxit")
             check(L.fpc_StructJacobiSetZeroGuess(self.pc), "zero guess")
+            check(L.fpc_StructJacobiSetPrecision(self.pc, jacobi_bits), "jacobi precision")
             check(L.fpc_StructGMRESSetPrecond(self.s, ctypes.cast(L.fpc_StructJacobiSolve, ctypes.c_void_p),
                                               ctypes.cast(L.fpc_StructJacobiSetup, ctypes.c_void_p), self.pc), "pc")
         elif precond == "pfmg":

@@ -130,6 +130,10 @@ int fpc_StructJacobiCreate(fpc_StructSolver *solver);
 int fpc_StructJacobiSetMaxIter(fpc_StructSolver solver, int maxit);
 int fpc_StructJacobiSetTol(fpc_StructSolver solver, double tol);
 int fpc_StructJacobiSetZeroGuess(fpc_StructSolver solver);
+/* No HYPRE counterpart: bits = 32 keeps the preconditioner's 1/diag as stored
+ * FP32 values (the mixed-precision momentum solves); the products stay FP64,
+ * so the preconditioner is a fixed diagonal and plain GMRES stays exact. */
+int fpc_StructJacobiSetPrecision(fpc_StructSolver solver, int bits);
 int fpc_StructJacobiSetNonZeroGuess(fpc_StructSolver solver);
 int fpc_StructJacobiSetup(fpc_StructSolver solver, fpc_StructMatrix A, fpc_StructVector b, fpc_StructVector x);
 int fpc_StructJacobiSolve(fpc_StructSolver solver, fpc_StructMatrix A, fpc_StructVector b, fpc_StructVector x);

@@ -72,6 +72,24 @@ def test_gmres_jacobi_matches_oracle(torch_cuda, which):
     assert np.linalg.norm(b - A @ x_g) / np.linalg.norm(b) < 1e-11
 
 
+@pytest.mark.parametrize("which", ["u", "w"])
+def test_gmres_fp32_jacobi_reaches_fp64_residual(torch_cuda, which):
+    """Jacobi on the stored FP32 1/diag (the mixed-precision momentum preconditioner):
+    the preconditioner is still a fixed diagonal, so plain GMRES converges to the same
+    FP64 tolerance; solutions agree with the FP64-Jacobi solve and the oracle's to the
+    accuracy the tolerance fixes."""
+    S, (nx, ny, nz), vals = _box(SMALL, which)
+    b = np.random.default_rng(4).standard_normal(nx * ny * nz)
+    x_o, its_o, _, _ = O.gmres(nx, ny, nz, vals, b, tol=1e-12, maxit=200, kdim=30, precond="jacobi")
+    box = F.StructBox(nx, ny, nz, vals)
+    x32, its32, rel32, rc32 = F.Solver(box, tol=1e-12, maxit=200, kdim=30, precond="jacobi", jacobi_bits=32).solve(b)
+    assert rc32 == 0 and rel32 <= 1e-12
+    A = box_csr(nx, ny, nz, vals)
+    assert np.linalg.norm(b - A @ x32) / np.linalg.norm(b) < 1e-11
+    assert rel_l2(x32, x_o) < 1e-10
+    assert abs(its32 - its_o) <= 2
+
+
 @pytest.mark.parametrize("relax", [1, 2])
 def test_gmres_pfmg_matches_oracle(torch_cuda, relax):
     S, (nx, ny, nz), vals = _box(SMALL, "p")
