@@ -198,9 +198,9 @@ int vec_copy(double *dst, const double *src, long long n);
 int vec_zero(double *dst, long long n);
 // x += y
 int axpy_inplace(double *x, const double *y, long long n);
-// r = b - A x (FP64), writes partial sums of r*r per block into part (kRedBlocks)
+// r = b - A x (FP64), writes partial sums of r*r per block into part (red_blocks(N) of them)
 int residual_norm_partials(const fpc_StructMatrix_s *A, const double *b, const double *x, double *r, double *part);
-// sum of kRedBlocks partials into *out (device)
+// sum of nparts partials into *out (device)
 int reduce_partials(const double *part, int nparts, double *out);
 int dot_partials(const double *a, const double *b, long long n, double *part);
 int matvec(double alpha, const fpc_StructMatrix_s *A, const double *x, double beta, double *y);
