@@ -38,12 +38,22 @@
 #include <unordered_map>
 #include <vector>
 
+#include <omp.h>
+
 namespace {
 
 typedef unsigned char u8;
 
 // ---------------------------------------------------------------- indexing
 // Staggered full-array layouts, x fastest (common.hpp:230-238).
+// Host threads of the solver loops (orc_set_threads; 1 by default).  Loops whose
+// iterations write disjoint outputs run as OpenMP static partitions; reductions
+// (dot) sum fixed per-thread partials in thread order, so results depend only on
+// the thread count and with one thread are the sequential sums.
+static int g_threads = 1;
+#define ORC_PAR _Pragma("omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)")
+#define ORC_PAR2 _Pragma("omp parallel for collapse(2) schedule(static) num_threads(g_threads) if (g_threads > 1)")
+
 inline size_t ixP(int i, int j, int k, int nx, int ny) { return ((size_t)k * ny + j) * nx + i; }
 inline size_t ixU(int i, int j, int k, int nx, int ny) { return ((size_t)k * ny + j) * (size_t)(nx + 1) + i; }
 inline size_t ixV(int i, int j, int k, int nx, int ny) { return ((size_t)k * (ny + 1) + j) * nx + i; }
@@ -145,6 +155,7 @@ CSR sp_to_csr(const Sp &A) {  // gpu_dense_ops.hpp:32-48 (columns sorted ascendi
 }
 
 void csr_spmv(const CSR &M, const double *x, double *y) {
+  ORC_PAR
   for (int r = 0; r < M.nrows; ++r) {
     double s = 0.0;
     for (int jj = M.rowptr[r]; jj < M.rowptr[r + 1]; ++jj) s += M.vals[jj] * x[M.colind[jj]];
@@ -657,6 +668,7 @@ inline bool nbr(const Box &B, int i, int j, int k, int s, size_t &r) {
 
 // y = A x; couplings that leave the box contribute nothing (Struct semantics).
 void box_matvec(const Box &B, const double *x, double *y) {
+  ORC_PAR2
   for (int k = 0; k < B.nz; ++k)
     for (int j = 0; j < B.ny; ++j)
       for (int i = 0; i < B.nx; ++i) {
@@ -927,10 +939,12 @@ void mg_relax(MG &M, MGLevel &Lv, bool zero, bool post) {
   if (M.relax_type <= 1) {
     double w = (M.relax_type == 0) ? 1.0 : Lv.omega;
     if (zero) {
+      ORC_PAR
       for (size_t r = 0; r < N; ++r) Lv.x[r] = (B.a[0][r] != 0.0) ? w * Lv.b[r] / B.a[0][r] : 0.0;
       return;
     }
     box_matvec(B, Lv.x.data(), Lv.r.data());
+    ORC_PAR
     for (size_t r = 0; r < N; ++r)
       if (B.a[0][r] != 0.0) Lv.x[r] = Lv.x[r] + w * (Lv.b[r] - Lv.r[r]) / B.a[0][r];
     return;
@@ -940,6 +954,7 @@ void mg_relax(MG &M, MGLevel &Lv, bool zero, bool post) {
   if (post && M.relax_type == 2) { order[0] = 1; order[1] = 0; }
   for (int pass = 0; pass < 2; ++pass) {
     int col = order[pass];
+    ORC_PAR2
     for (int k = 0; k < B.nz; ++k)
       for (int j = 0; j < B.ny; ++j)
         for (int i = 0; i < B.nx; ++i) {
@@ -965,6 +980,7 @@ void mg_vcycle(MG &M, const double *b, double *x) {
     box_matvec(F.A, F.x.data(), F.r.data());
     for (size_t r = 0; r < F.A.n(); ++r) F.r[r] = F.b[r] - F.r[r];
     int d = F.cdir;
+    ORC_PAR2
     for (int k = 0; k < C.A.nz; ++k)
       for (int j = 0; j < C.A.ny; ++j)
         for (int i = 0; i < C.A.nx; ++i) {
@@ -988,6 +1004,7 @@ void mg_vcycle(MG &M, const double *b, double *x) {
     MGLevel &C = M.L[(size_t)l + 1];
     int d = F.cdir;
     int nc[3] = {C.A.nx, C.A.ny, C.A.nz};
+    ORC_PAR2
     for (int k = 0; k < F.A.nz; ++k)
       for (int j = 0; j < F.A.ny; ++j)
         for (int i = 0; i < F.A.nx; ++i) {
@@ -1021,10 +1038,12 @@ struct Precond {
     size_t N = A->n();
     if (kind == 0) { std::copy(b, b + N, x); return; }
     if (kind == 1) {
+      ORC_PAR
       for (size_t r = 0; r < N; ++r) x[r] = b[r] / A->a[0][r];
       std::vector<double> t(N);
       for (int it = 1; it < iters; ++it) {
         box_matvec(*A, x, t.data());
+        ORC_PAR
         for (size_t r = 0; r < N; ++r) x[r] = x[r] + (b[r] - t[r]) / A->a[0][r];
       }
       return;
@@ -1034,8 +1053,10 @@ struct Precond {
       std::vector<double> r(N), e(N);
       for (int it = 1; it < iters; ++it) {
         box_matvec(*A, x, r.data());
+        ORC_PAR
         for (size_t q = 0; q < N; ++q) r[q] = b[q] - r[q];
         mg_vcycle(mg, r.data(), e.data());
+        ORC_PAR
         for (size_t q = 0; q < N; ++q) x[q] += e[q];
       }
     }
@@ -1043,8 +1064,25 @@ struct Precond {
 };
 
 double dot(const std::vector<double> &a, const std::vector<double> &b) {
+  const size_t n = a.size();
+  if (g_threads <= 1) {
+    double s = 0.0;
+    for (size_t i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+  }
+  std::vector<double> part((size_t)g_threads, 0.0);
+  int team = 1;
+#pragma omp parallel num_threads(g_threads)
+  {
+    const int nt = omp_get_num_threads(), t = omp_get_thread_num();
+    const size_t lo = n * (size_t)t / (size_t)nt, hi = n * ((size_t)t + 1) / (size_t)nt;
+    double s = 0.0;
+    for (size_t i = lo; i < hi; ++i) s += a[i] * b[i];
+    part[(size_t)t] = s;
+    if (t == 0) team = nt;
+  }
   double s = 0.0;
-  for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+  for (int t = 0; t < team; ++t) s += part[(size_t)t];
   return s;
 }
 
@@ -1057,6 +1095,7 @@ int gmres_solve(const Box &A, Precond &M, const double *bp, double *xp, double t
   std::vector<double> c((size_t)kdim), s((size_t)kdim), rs((size_t)kdim + 1);
   auto matvec_res = [&](std::vector<double> &out) {  // out = b - A x
     box_matvec(A, x.data(), out.data());
+    ORC_PAR
     for (size_t q = 0; q < N; ++q) out[q] = b[q] - out[q];
   };
   matvec_res(p[0]);
@@ -1077,6 +1116,7 @@ int gmres_solve(const Box &A, Precond &M, const double *bp, double *xp, double t
       if (r_norm <= eps) { conv = true; break; }
     }
     double t = 1.0 / r_norm;
+    ORC_PAR
     for (size_t q = 0; q < N; ++q) p[0][q] *= t;
     i = 0;
     while (i < kdim && iter < maxit) {
@@ -1087,12 +1127,14 @@ int gmres_solve(const Box &A, Precond &M, const double *bp, double *xp, double t
       for (int j = 0; j < i; ++j) {  // modified Gram-Schmidt
         hh[(size_t)j][(size_t)i - 1] = dot(p[(size_t)j], p[(size_t)i]);
         double h = hh[(size_t)j][(size_t)i - 1];
+        ORC_PAR
         for (size_t q = 0; q < N; ++q) p[(size_t)i][q] -= h * p[(size_t)j][q];
       }
       t = std::sqrt(dot(p[(size_t)i], p[(size_t)i]));
       hh[(size_t)i][(size_t)i - 1] = t;
       if (t != 0.0) {
         t = 1.0 / t;
+        ORC_PAR
         for (size_t q = 0; q < N; ++q) p[(size_t)i][q] *= t;
       }
       for (int j = 1; j < i; ++j) {
@@ -1119,10 +1161,13 @@ int gmres_solve(const Box &A, Precond &M, const double *bp, double *xp, double t
       tt += rs[(size_t)k];
       rs[(size_t)k] = tt / hh[(size_t)k][(size_t)k];
     }
+    ORC_PAR
     for (size_t q = 0; q < N; ++q) w[q] = rs[(size_t)i - 1] * p[(size_t)i - 1][q];
     for (int j = i - 2; j >= 0; --j)
+      ORC_PAR
       for (size_t q = 0; q < N; ++q) w[q] += rs[(size_t)j] * p[(size_t)j][q];
     M.apply(w.data(), r.data());
+    ORC_PAR
     for (size_t q = 0; q < N; ++q) x[q] += r[q];
     if (r_norm <= eps) {
       matvec_res(r);
@@ -1140,10 +1185,14 @@ int gmres_solve(const Box &A, Precond &M, const double *bp, double *xp, double t
       rs[(size_t)j] = c[(size_t)j - 1] * rs[(size_t)j];
     }
     if (i) {
+      ORC_PAR
       for (size_t q = 0; q < N; ++q) p[(size_t)i][q] += (rs[(size_t)i] - 1.0) * p[(size_t)i][q];
       for (int j = i - 1; j > 0; --j)
+        ORC_PAR
         for (size_t q = 0; q < N; ++q) p[(size_t)i][q] += rs[(size_t)j] * p[(size_t)j][q];
+      ORC_PAR
       for (size_t q = 0; q < N; ++q) p[0][q] += (rs[0] - 1.0) * p[0][q];
+      ORC_PAR
       for (size_t q = 0; q < N; ++q) p[0][q] += p[(size_t)i][q];
     }
   }
@@ -1231,6 +1280,7 @@ void convection(Sim &M) {
   M.conv_u.assign(u.size(), 0.0);
   M.conv_v.assign(v.size(), 0.0);
   M.conv_w.assign(w.size(), 0.0);
+  ORC_PAR2
   for (int k = 0; k < nz; ++k)
     for (int j = 0; j < ny; ++j)
       for (int i = 1; i <= nx - 1; ++i) {
@@ -1243,6 +1293,7 @@ void convection(Sim &M) {
         double ut = U(i, j, k + 1, uc), ub = U(i, j, k - 1, uc);
         M.conv_u[t] = uc * (ue - uw) / (2 * dx) + va * (un - us) / (2 * dy) + wa * (ut - ub) / (2 * dz);
       }
+  ORC_PAR2
   for (int k = 0; k < nz; ++k)
     for (int j = 1; j <= ny - 1; ++j)
       for (int i = 0; i < nx; ++i) {
@@ -1255,6 +1306,7 @@ void convection(Sim &M) {
         double vt = V(i, j, k + 1, vc), vb = V(i, j, k - 1, vc);
         M.conv_v[t] = ua * (ve - vw) / (2 * dx) + vc * (vn - vs) / (2 * dy) + wa * (vt - vb) / (2 * dz);
       }
+  ORC_PAR2
   for (int k = 1; k <= nz - 1; ++k)
     for (int j = 0; j < ny; ++j)
       for (int i = 0; i < nx; ++i) {
@@ -1306,6 +1358,7 @@ void phase_explicit(Sim &M) {
   M.rhs_u.assign((size_t)(nx - 1) * ny * nz, 0.0); M.x_u.assign(M.rhs_u.size(), 0.0);
   M.rhs_v.assign((size_t)nx * (ny - 1) * nz, 0.0); M.x_v.assign(M.rhs_v.size(), 0.0);
   M.rhs_w.assign((size_t)nx * ny * (nz - 1), 0.0); M.x_w.assign(M.rhs_w.size(), 0.0);
+  ORC_PAR2
   for (int k = 0; k < nz; ++k)
     for (int j = 0; j < ny; ++j)
       for (int i = 1; i <= nx - 1; ++i) {
@@ -1316,6 +1369,7 @@ void phase_explicit(Sim &M) {
         M.rhs_u[row] = r;
         M.x_u[row] = zero ? 0.0 : M.u[t];
       }
+  ORC_PAR2
   for (int k = 0; k < nz; ++k)
     for (int j = 1; j <= ny - 1; ++j)
       for (int i = 0; i < nx; ++i) {
@@ -1326,6 +1380,7 @@ void phase_explicit(Sim &M) {
         M.rhs_v[row] = r;
         M.x_v[row] = zero ? 0.0 : M.v[t];
       }
+  ORC_PAR2
   for (int k = 1; k <= nz - 1; ++k)
     for (int j = 0; j < ny; ++j)
       for (int i = 0; i < nx; ++i) {
@@ -1665,6 +1720,11 @@ int copy_out(const View &v, void *dst, long long nbytes) {
 
 // ---------------------------------------------------------------- C API
 extern "C" {
+
+void orc_set_threads(int n) {
+  g_threads = n < 1 ? 1 : n;
+  omp_set_dynamic(0);
+}
 
 void orc_default_params(orc_params *p) {
   // common.hpp:45-85

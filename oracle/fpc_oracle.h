@@ -31,6 +31,8 @@ typedef struct {
   unsigned long long seed;
 } orc_params;
 
+/* Host threads of the solver and stage loops (default 1: the sequential sums). */
+void orc_set_threads(int n);
 void orc_default_params(orc_params *par);
 
 /* One-time setup (geometry, masks, ids, assembly, scaling, embedding). */

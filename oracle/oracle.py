@@ -53,6 +53,7 @@ def lib():
         dp = ctypes.POINTER(ctypes.c_double)
         ip = ctypes.POINTER(ctypes.c_int)
         L.orc_default_params.argtypes = [ctypes.POINTER(OrcParams)]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_setup_create.argtypes = [ctypes.POINTER(OrcParams)]
         L.orc_setup_create.restype = vp
         L.orc_setup_destroy.argtypes = [vp]
@@ -81,6 +82,11 @@ def lib():
         L.orc_sim_scalar.restype = ctypes.c_double
         _lib = L
     return _lib
+
+
+def set_threads(n):
+    """Host threads of the oracle's solver and stage loops (1 = the sequential sums)."""
+    lib().orc_set_threads(int(n))
 
 
 def default_params(**kw):
