@@ -1,6 +1,8 @@
 // fpc_oracle.cpp -- CPU oracle for the per-timestep path (TEST INFRASTRUCTURE ONLY).
 //
-// A plain, sequential C++ restatement of the reference algorithm.  Tests,
+// A plain C++ restatement of the reference algorithm, sequential unless
+// orc_set_threads(n > 1) spreads its solver and stage loops over OpenMP
+// threads (bench.py's CPU baseline does; the tests run it sequentially).  Tests,
 // __graft_entry__.smoke() and bench.py's cpu_baseline leg use it as the checker;
 // nothing in the product library links, loads or calls it.
 //

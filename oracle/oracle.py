@@ -3,7 +3,7 @@
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
 module, always as the checker.  The oracle restates the reference algorithm
 (JD63021/ibm-3d-fpc main.cu:20-376, gpu_dense_ops.hpp, immersed_*.hpp,
-solver_helpers.hpp) in plain sequential C++; see fpc_oracle.cpp for the line
+solver_helpers.hpp) in plain C++ (sequential unless set_threads(n > 1)); see fpc_oracle.cpp for the line
 citations and for which parts are pinned by what.
 """
 import ctypes
