@@ -49,6 +49,8 @@ namespace {
 
 // Largest coarsest level for a hierarchy whose finest level has n0 points.
 inline long long coarse_max(long long n0) { return std::min(1152LL, std::max(64LL, n0 / 2048)); }
+// Upper bound of a coarsest size set by the caller (dense inverse: n^2 doubles).
+constexpr long long kCoarseSizeLimit = 4096;
 
 inline int em(int d) { return 1 + 2 * d; }
 inline int ep(int d) { return 2 + 2 * d; }
@@ -493,6 +495,7 @@ struct PfmgImpl {
   int maxit = 1;
   double tol = 0.0;
   int relax = 1, rap = 1, npre = 1, npost = 1, maxlev = 0, print = 0, bits = 64;
+  long long coarse = 0, built_coarse = -1;  // coarsest-size cap; 0 = coarse_max(N0)
   bool zero_guess = false;
   int its = 0;
   fpc_StructMatrix_s *A = nullptr;
@@ -601,7 +604,7 @@ int build(PfmgImpl *P, fpc_StructMatrix_s *A) {
     FPC_HIP(hipMemcpyAsync(h0.a.data() + (size_t)s * (size_t)h0.N, A->coef + (long long)s * A->pitch,
                            (size_t)h0.N * sizeof(double), hipMemcpyDeviceToHost, stream()));
   FPC_HIP(hipStreamSynchronize(stream()));
-  const long long cmax = coarse_max(h0.N);
+  const long long cmax = P->coarse > 0 ? P->coarse : coarse_max(h0.N);
   for (;;) {
     HLevel &h = H.back();
     choose_direction(h);
@@ -623,6 +626,7 @@ int build(PfmgImpl *P, fpc_StructMatrix_s *A) {
   if (rc) return rc;
   P->version = A->version;
   P->built_bits = P->bits;
+  P->built_coarse = P->coarse;
   return 0;
 }
 
@@ -799,7 +803,9 @@ PfmgImpl *pimpl(fpc_StructSolver s) {
 
 int pfmg_setup(void *pf, fpc_StructMatrix_s *A) {
   PfmgImpl *P = (PfmgImpl *)pf;
-  if (P->A == A && P->version == A->version && P->built_bits == P->bits && !P->L.empty()) return 0;
+  if (P->A == A && P->version == A->version && P->built_bits == P->bits && P->built_coarse == P->coarse &&
+      !P->L.empty())
+    return 0;
   return build(P, A);
 }
 
@@ -904,6 +910,13 @@ int fpc_StructPFMGSetPrecision(fpc_StructSolver s, int bits) {
   PIMPL(s);
   FPC_CHECK_ARG(bits == 32 || bits == 64, "precision must be 32 or 64");
   P->bits = bits;
+  return 0;
+}
+
+int fpc_StructPFMGSetCoarseSize(fpc_StructSolver s, int n) {
+  PIMPL(s);
+  FPC_CHECK_ARG(n >= 0 && n <= kCoarseSizeLimit, "coarse size must be 0..4096");
+  P->coarse = n;
   return 0;
 }
 

@@ -108,6 +108,7 @@ SIGNATURES = {
     "fpc_StructPFMGSetNumPostRelax": (_i, [_vp, _i]), "fpc_StructPFMGSetMaxLevels": (_i, [_vp, _i]),
     "fpc_StructPFMGSetPrintLevel": (_i, [_vp, _i]), "fpc_StructPFMGSetZeroGuess": (_i, [_vp]),
     "fpc_StructPFMGSetNonZeroGuess": (_i, [_vp]), "fpc_StructPFMGSetPrecision": (_i, [_vp, _i]),
+    "fpc_StructPFMGSetCoarseSize": (_i, [_vp, _i]),
     "fpc_StructPFMGSetup": (_i, [_vp, _vp, _vp, _vp]), "fpc_StructPFMGSolve": (_i, [_vp, _vp, _vp, _vp]),
     "fpc_StructPFMGGetNumIterations": (_i, [_vp, _pi]), "fpc_StructPFMGGetNumLevels": (_i, [_vp, _pi]),
     "fpc_StructPFMGGetLevelInfo": (_i, [_vp, _i, _pi, _pi, _pd]), "fpc_StructPFMGDestroy": (_i, [_vp]),
@@ -542,7 +543,7 @@ class Solver:
     """GMRES (or flexible GMRES) with a Jacobi / PFMG / no preconditioner, set up as main.cu:173-221."""
 
     def __init__(self, box, tol=1e-8, maxit=250, kdim=30, precond="jacobi", pc_iters=1, relax_type=1,
-                 rap_type=1, flexible=False, mg_bits=64, jacobi_bits=64):
+                 rap_type=1, flexible=False, mg_bits=64, jacobi_bits=64, coarse_size=0):
         L = lib()
         self.box = box
         self.s = ctypes.c_void_p()
@@ -567,6 +568,7 @@ class Solver:
             check(L.fpc_StructPFMGSetRAPType(self.pc, rap_type), "rap")
             check(L.fpc_StructPFMGSetZeroGuess(self.pc), "zero guess")
             check(L.fpc_StructPFMGSetPrecision(self.pc, mg_bits), "precision")
+            check(L.fpc_StructPFMGSetCoarseSize(self.pc, coarse_size), "coarse size")
             check(L.fpc_StructGMRESSetPrecond(self.s, ctypes.cast(L.fpc_StructPFMGSolve, ctypes.c_void_p),
                                               ctypes.cast(L.fpc_StructPFMGSetup, ctypes.c_void_p), self.pc), "pc")
         self.b = box.vector(np.zeros(box.N))
@@ -622,7 +624,7 @@ class Solver:
             self.pc = None
 
 
-def pfmg_apply(box, b, relax_type=1, mg_bits=64):
+def pfmg_apply(box, b, relax_type=1, mg_bits=64, coarse_size=0):
     """One V-cycle from zero (HYPRE_StructPFMGSolve with MaxIter 1, zero guess)."""
     L = lib()
     pc = ctypes.c_void_p()
@@ -632,6 +634,7 @@ def pfmg_apply(box, b, relax_type=1, mg_bits=64):
         check(L.fpc_StructPFMGSetRelaxType(pc, relax_type), "relax")
         check(L.fpc_StructPFMGSetZeroGuess(pc), "zero")
         check(L.fpc_StructPFMGSetPrecision(pc, mg_bits), "bits")
+        check(L.fpc_StructPFMGSetCoarseSize(pc, coarse_size), "coarse size")
         bv = box.vector(b)
         xv = box.vector(np.zeros(box.N))
         check(L.fpc_StructPFMGSetup(pc, box.A, bv.h, xv.h), "setup")

@@ -158,6 +158,11 @@ int fpc_StructPFMGSetNonZeroGuess(fpc_StructSolver solver);
  * smoothing, restriction and prolongation.  The coarsest solve stays FP64.
  * No reference counterpart. */
 int fpc_StructPFMGSetPrecision(fpc_StructSolver solver, int bits);
+/* Coarsening stops at the first level with at most n points (1..4096), solved
+ * by a dense inverse; 0 (default) = min(1152, max(64, N0/2048)) for a finest
+ * level of N0 points.  No reference counterpart (HYPRE's PFMG has no such
+ * setting); the adjoint's transposed pressure solves set 64. */
+int fpc_StructPFMGSetCoarseSize(fpc_StructSolver solver, int n);
 int fpc_StructPFMGSetup(fpc_StructSolver solver, fpc_StructMatrix A, fpc_StructVector b, fpc_StructVector x);
 int fpc_StructPFMGSolve(fpc_StructSolver solver, fpc_StructMatrix A, fpc_StructVector b, fpc_StructVector x);
 int fpc_StructPFMGGetNumIterations(fpc_StructSolver solver, int *num_iterations);

@@ -715,6 +715,8 @@ void box_matvec(const Box &B, const double *x, double *y) {
 //    would put its own rounding (~cond * eps of the coarsest matrix) at the
 //    level of a 1e-12 GMRES tolerance.
 inline size_t mg_coarse_max(size_t n0) { return std::min<size_t>(1152, std::max<size_t>(64, n0 / 2048)); }
+// A coarsest size set by orc_set_coarse_size (0: the rule above).
+static size_t g_coarse = 0;
 
 struct MGLevel {
   Box A;
@@ -904,7 +906,7 @@ void mg_setup(MG &M, const Box &A0, int relax_type, int max_levels) {
   M.L.clear();
   M.L.emplace_back();
   M.L.back().A = A0;
-  const size_t cmax = mg_coarse_max(A0.n());
+  const size_t cmax = g_coarse > 0 ? g_coarse : mg_coarse_max(A0.n());
   for (;;) {
     MGLevel &Lv = M.L.back();
     mg_choose(Lv.A, Lv.cdir, Lv.omega);
@@ -1722,6 +1724,8 @@ int copy_out(const View &v, void *dst, long long nbytes) {
 
 // ---------------------------------------------------------------- C API
 extern "C" {
+
+void orc_set_coarse_size(int n) { g_coarse = n > 0 ? (size_t)n : 0; }
 
 void orc_set_threads(int n) {
   g_threads = n < 1 ? 1 : n;

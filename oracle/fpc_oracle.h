@@ -33,6 +33,8 @@ typedef struct {
 
 /* Host threads of the solver and stage loops (default 1: the sequential sums). */
 void orc_set_threads(int n);
+/* Largest coarsest MG level (0, the default: min(1152, max(64, N0/2048))). */
+void orc_set_coarse_size(int n);
 void orc_default_params(orc_params *par);
 
 /* One-time setup (geometry, masks, ids, assembly, scaling, embedding). */

@@ -54,6 +54,7 @@ def lib():
         ip = ctypes.POINTER(ctypes.c_int)
         L.orc_default_params.argtypes = [ctypes.POINTER(OrcParams)]
         L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_set_coarse_size.argtypes = [ctypes.c_int]
         L.orc_setup_create.argtypes = [ctypes.POINTER(OrcParams)]
         L.orc_setup_create.restype = vp
         L.orc_setup_destroy.argtypes = [vp]
@@ -87,6 +88,11 @@ def lib():
 def set_threads(n):
     """Host threads of the oracle's solver and stage loops (1 = the sequential sums)."""
     lib().orc_set_threads(int(n))
+
+
+def set_coarse_size(n):
+    """Largest coarsest MG level (0 = the size-scaled default rule)."""
+    lib().orc_set_coarse_size(int(n))
 
 
 def default_params(**kw):

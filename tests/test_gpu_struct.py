@@ -149,6 +149,24 @@ def test_pfmg_vcycle_matches_oracle(torch_cuda, relax):
     assert rel_l2(got, want) < 1e-12
 
 
+@pytest.mark.parametrize("coarse", [200, 1000])
+def test_pfmg_coarse_size_matches_oracle(torch_cuda, coarse):
+    S, (nx, ny, nz), vals = _box(SMALL, "p")
+    b = np.random.default_rng(13).standard_normal(nx * ny * nz)
+    try:
+        O.set_coarse_size(coarse)
+        want = O.precond_apply(nx, ny, nz, vals, b, precond="pfmg")
+        lv_o = O.mg_levels(nx, ny, nz, vals)
+    finally:
+        O.set_coarse_size(0)
+    box = F.StructBox(nx, ny, nz, vals)
+    got = F.pfmg_apply(box, b, coarse_size=coarse)
+    assert rel_l2(got, want) < 1e-12
+    sol = F.Solver(box, tol=1e-8, maxit=5, kdim=5, precond="pfmg", coarse_size=coarse)
+    assert [(tuple(int(t) for t in a[0]), a[1]) for a in lv_o] == [(a[0], a[1]) for a in sol.levels()]
+    assert int(np.prod(lv_o[-1][0])) <= coarse
+
+
 def test_pfmg_fp32_vcycle_close_to_fp64(torch_cuda):
     S, (nx, ny, nz), vals = _box(SMALL, "p")
     b = np.random.default_rng(2).standard_normal(nx * ny * nz)

@@ -383,6 +383,27 @@ def test_vcycle_converges_on_poisson(dims):
     assert np.all(rates < 0.5), res
 
 
+def test_coarse_size_stops_coarsening_and_solves_exactly():
+    # a coarsest size at or above N keeps one level: the V-cycle is the exact solve
+    nx, ny, nz = 10, 8, 6
+    vals = _poisson(nx, ny, nz, 1.0, 1.2, 0.8)
+    b = np.random.default_rng(9).standard_normal(nx * ny * nz)
+    try:
+        O.set_coarse_size(nx * ny * nz)
+        assert len(O.mg_levels(nx, ny, nz, vals)) == 1
+        got = O.precond_apply(nx, ny, nz, vals, b, precond="pfmg")
+        O.set_coarse_size(200)
+        lv = O.mg_levels(nx, ny, nz, vals)
+    finally:
+        O.set_coarse_size(0)
+    want = spl.spsolve(box_csr(nx, ny, nz, vals).tocsc(), b)
+    assert rel_l2(got, want) < 1e-13
+    # otherwise coarsening stops at the first level with <= n points
+    sizes = [int(np.prod(l[0])) for l in lv]
+    assert sizes[-1] <= 200 < sizes[-2]
+    assert len(O.mg_levels(nx, ny, nz, vals)) > len(lv)
+
+
 # ---------------------------------------------------------------- dense ops and a whole step
 def test_convection_exact_on_linear_and_uniform_fields():
     kw = dict(nx=20, ny=10, nz=8, Lx=1.0, xcirc=5.0, ycirc=5.0, Rcirc=0.01)  # cylinder outside: open channel
