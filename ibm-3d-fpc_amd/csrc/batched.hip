@@ -44,11 +44,15 @@ using kry::Status;
 constexpr int kBMax = 768;  // systems per group
 
 struct BSys {
-  const double *c;
+  const double *c;  // the operator's planes as the z-marching kernels read them (rows by zm)
   long long pitch;
+  ZMap zm;
+  const double *fc;  // the full planes (one-point kernels)
+  long long fpitch;
   int nx, ny, nz, nred;
   long long N, vp;
-  const double *dinv;  // fused Jacobi, or null
+  const double *dinv;       // fused Jacobi, stored like c, or null
+  const double *dinv_full;  // the same 1/diag at every point (one-point matvec, solution update), or null
   double *V, *Z, *W, *part;
   State *S;
   const double *b;
@@ -56,8 +60,11 @@ struct BSys {
 };
 
 struct BLev {
-  const double *c, *dinv, *wm, *wp;
+  const double *c, *dinv, *wm, *wp;  // z-marching kernels: rows by zm
   long long pitch;
+  ZMap zm;
+  const double *fc, *fdinv, *fwm, *fwp;  // one-point kernels: the full arrays
+  long long fpitch;
   double *b, *r, *y;  // level >= 1 (level 0 reads the call's input, writes Z)
   double *bc;         // coarse b (restriction target)
   const double *yc;   // coarse y (prolongation source)
@@ -118,7 +125,7 @@ __global__ void __launch_bounds__(256) k_b_residual(const BSys *sys, Sub sub, in
   if ((int)blockIdx.x >= s.nred) return;
   if (blockIdx.z == 0) {
     double *r = (out_mode == 0) ? s.V : (out_mode == 1 ? s.W : nullptr);
-    kry::residual_body(s.c, s.pitch, s.nx, s.ny, s.nz, s.b, s.x, r, s.part, blockIdx.x, s.nred);
+    kry::residual_body(s.fc, s.fpitch, ZMap(), s.nx, s.ny, s.nz, s.b, s.x, r, s.part, blockIdx.x, s.nred);
   } else {
     kry::dot_body(s.b, s.b, s.N, s.part + NB, blockIdx.x, s.nred);
   }
@@ -161,7 +168,7 @@ __global__ void __launch_bounds__(256) k_b_matvec(const BSys *sys, const int *it
   ijk_of(t, s.nx, s.ny, i, j, k);
   const double *z = use_z ? s.Z : s.V + (long long)cur * s.vp;
   const double sg = S->sigma[cur];
-  const double v = kry::arnoldi_row(s.c, s.pitch, s.nx, s.ny, s.nz, z, s.dinv, i, j, k, t);
+  const double v = kry::arnoldi_row(s.fc, s.fpitch, ZMap(), s.nx, s.ny, s.nz, z, s.dinv_full, i, j, k, t);
   s.V[(long long)(cur + 1) * s.vp + t] = sg * v;
 }
 
@@ -208,7 +215,7 @@ __global__ void __launch_bounds__(256) k_b_combine(const BSys *sys, Sub sub, int
   __syncthreads();
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < s.N; t += (long long)gridDim.x * blockDim.x) {
     if (mode == 0) {
-      const double v = kry::combine_point(s.V, s.vp, t, y, i, s.dinv);
+      const double v = kry::combine_point(s.V, s.vp, t, y, i, s.dinv_full);
       s.x[t] = s.x[t] + v;
     } else {
       s.W[t] = kry::combine_point<double>(s.V, s.vp, t, y, i, nullptr);
@@ -264,7 +271,8 @@ __global__ void __launch_bounds__(256) k_b_down(const BLev *lev, const BSys *sys
   if (t >= L.N) return;
   int i, j, k;
   ijk_of(t, L.nx, L.ny, i, j, k);
-  L.r[t] = mgd::down_point<double>(L.c, L.pitch, L.nx, L.ny, L.nz, L.dinv, lev_b(L, s, in_mode, step), L.w, i, j, k, t);
+  L.r[t] = mgd::down_point<double>(L.fc, L.fpitch, ZMap(), L.nx, L.ny, L.nz, L.fdinv, lev_b(L, s, in_mode, step), L.w,
+                                   i, j, k, t);
 }
 
 __global__ void __launch_bounds__(256) k_b_restrict(const BLev *lev, const BSys *sys, const int *item, const int *first,
@@ -276,7 +284,7 @@ __global__ void __launch_bounds__(256) k_b_restrict(const BLev *lev, const BSys 
   if (t >= L.Nc) return;
   int i, j, k;
   ijk_of(t, L.cnx, L.cny, i, j, k);
-  L.bc[t] = mgd::restrict_point<double>(L.nx, L.ny, L.nz, L.cdir, L.r, L.wm, L.wp, i, j, k);
+  L.bc[t] = mgd::restrict_point<double>(L.nx, L.ny, L.nz, L.cdir, L.r, L.fwm, L.fwp, ZMap(), i, j, k);
 }
 
 // coarsest levels: four rows per block, one wavefront per row
@@ -306,14 +314,14 @@ __global__ void __launch_bounds__(256) k_b_up(const BLev *lev, const BSys *sys, 
   const double *b = lev_b(L, s, in_mode, step);
   double v;
   if (L.cdir == 0)
-    v = mgd::up_point<double, 0>(L.c, L.pitch, L.nx, L.ny, L.nz, L.dinv, b, L.wm, L.wp, L.yc, L.cnx, L.cny, L.ncd, L.w,
-                                 i, j, k, t);
+    v = mgd::up_point<double, 0>(L.fc, L.fpitch, ZMap(), L.nx, L.ny, L.nz, L.fdinv, b, L.fwm, L.fwp, L.yc, L.cnx,
+                                 L.cny, L.ncd, L.w, i, j, k, t);
   else if (L.cdir == 1)
-    v = mgd::up_point<double, 1>(L.c, L.pitch, L.nx, L.ny, L.nz, L.dinv, b, L.wm, L.wp, L.yc, L.cnx, L.cny, L.ncd, L.w,
-                                 i, j, k, t);
+    v = mgd::up_point<double, 1>(L.fc, L.fpitch, ZMap(), L.nx, L.ny, L.nz, L.fdinv, b, L.fwm, L.fwp, L.yc, L.cnx,
+                                 L.cny, L.ncd, L.w, i, j, k, t);
   else
-    v = mgd::up_point<double, 2>(L.c, L.pitch, L.nx, L.ny, L.nz, L.dinv, b, L.wm, L.wp, L.yc, L.cnx, L.cny, L.ncd, L.w,
-                                 i, j, k, t);
+    v = mgd::up_point<double, 2>(L.fc, L.fpitch, ZMap(), L.nx, L.ny, L.nz, L.fdinv, b, L.fwm, L.fwp, L.yc, L.cnx,
+                                 L.cny, L.ncd, L.w, i, j, k, t);
   (L.lvl == 0 ? s.Z : L.y)[t] = v;
 }
 
@@ -326,7 +334,7 @@ __global__ void __launch_bounds__(kTileX *kTY) k_bt_matvec(const BSys *sys, cons
   if (S->st.done) return;
   const int cur = S->st.i;
   kry::ArnoldiOp<kDinv> op{s.c, s.pitch, s.nx, s.ny, use_z ? s.Z : s.V + (long long)cur * s.vp, s.dinv,
-                           s.V + (long long)(cur + 1) * s.vp, S->sigma[cur]};
+                           s.V + (long long)(cur + 1) * s.vp, S->sigma[cur], s.zm};
   march_tile<double, kTileX, kTY>(op, s.nx, s.ny, s.nz, e.k0, e.k1, e.bxy & 0xffff, e.bxy >> 16);
 }
 
@@ -343,9 +351,10 @@ __global__ void __launch_bounds__(kTileX *kTY) k_bt_down(const BLev *lev, const 
     mgd::DownRestrictZOp<double> op;
     op.c = L.c; op.pitch = L.pitch; op.nx = L.nx; op.ny = L.ny; op.dinv = L.dinv; op.b = b; op.r = nullptr;
     op.w = L.w; op.wm = L.wm; op.wp = L.wp; op.bc = L.bc; op.nz = L.nz; op.ncz = L.cnz; op.k0 = e.k0;
+    op.zm = L.zm;
     march_tile<double, kTileX, kTY>(op, L.nx, L.ny, L.nz, e.k0, e.k1 < L.nz ? e.k1 + 1 : e.k1, bx, by);
   } else {
-    mgd::DownOp<double> op{L.c, L.pitch, L.nx, L.ny, L.dinv, b, L.r, L.w};
+    mgd::DownOp<double> op{L.c, L.pitch, L.nx, L.ny, L.dinv, b, L.r, L.w, L.zm};
     march_tile<double, kTileX, kTY>(op, L.nx, L.ny, L.nz, e.k0, e.k1, bx, by);
   }
 }
@@ -360,13 +369,13 @@ __global__ void __launch_bounds__(kTileX *kTY) k_bt_up(const BLev *lev, const BS
   double *y = L.lvl == 0 ? s.Z : L.y;
   const int bx = e.bxy & 0xffff, by = e.bxy >> 16;
   if (L.cdir == 0) {
-    mgd::UpOp<double, 0> op{L.c, L.pitch, L.nx, L.ny, L.dinv, b, L.wm, L.wp, L.yc, L.cnx, L.cny, L.ncd, y, L.w};
+    mgd::UpOp<double, 0> op{L.c, L.pitch, L.nx, L.ny, L.dinv, b, L.wm, L.wp, L.yc, L.cnx, L.cny, L.ncd, y, L.w, L.zm};
     march_tile<double, kTileX, kTY>(op, L.nx, L.ny, L.nz, e.k0, e.k1, bx, by);
   } else if (L.cdir == 1) {
-    mgd::UpOp<double, 1> op{L.c, L.pitch, L.nx, L.ny, L.dinv, b, L.wm, L.wp, L.yc, L.cnx, L.cny, L.ncd, y, L.w};
+    mgd::UpOp<double, 1> op{L.c, L.pitch, L.nx, L.ny, L.dinv, b, L.wm, L.wp, L.yc, L.cnx, L.cny, L.ncd, y, L.w, L.zm};
     march_tile<double, kTileX, kTY>(op, L.nx, L.ny, L.nz, e.k0, e.k1, bx, by);
   } else {
-    mgd::UpOp<double, 2> op{L.c, L.pitch, L.nx, L.ny, L.dinv, b, L.wm, L.wp, L.yc, L.cnx, L.cny, L.ncd, y, L.w};
+    mgd::UpOp<double, 2> op{L.c, L.pitch, L.nx, L.ny, L.dinv, b, L.wm, L.wp, L.yc, L.cnx, L.cny, L.ncd, y, L.w, L.zm};
     march_tile<double, kTileX, kTY>(op, L.nx, L.ny, L.nz, e.k0, e.k1, bx, by);
   }
 }
@@ -704,15 +713,20 @@ int batch_group_create(const std::vector<fpc_StructSolver> &solvers, const std::
     }
     Dims d = dims_of(v.A->grid);
     BSys &s = hs[(size_t)q];
-    s.c = v.A->coef;
-    s.pitch = v.A->pitch;
+    const Planes<double> P = planes64(v.A);
+    s.c = P.c;
+    s.pitch = P.pitch;
+    s.zm = P.zm;
+    s.fc = v.A->coef;
+    s.fpitch = v.A->pitch;
     s.nx = d.nx;
     s.ny = d.ny;
     s.nz = d.nz;
     s.nred = v.nred;
     s.N = d.N;
     s.vp = v.vp;
-    s.dinv = v.dinv;
+    s.dinv = v.dinv ? jacobi_dinv<double>(v.A) : nullptr;
+    s.dinv_full = v.dinv;
     s.V = v.V;
     s.Z = v.Z;
     s.W = v.W;
@@ -747,6 +761,12 @@ int batch_group_create(const std::vector<fpc_StructSolver> &solvers, const std::
         e.wm = L[(size_t)l].wm;
         e.wp = L[(size_t)l].wp;
         e.pitch = L[(size_t)l].pitch;
+        e.zm = L[(size_t)l].zm;
+        e.fc = L[(size_t)l].fcoef;
+        e.fdinv = L[(size_t)l].fdinv;
+        e.fwm = L[(size_t)l].fwm;
+        e.fwp = L[(size_t)l].fwp;
+        e.fpitch = L[(size_t)l].fpitch;
         e.b = L[(size_t)l].b;
         e.r = L[(size_t)l].r;
         e.y = L[(size_t)l].y;

@@ -23,8 +23,13 @@ struct MgLevelView {
   long long N = 0;
   int cdir = -1;
   double w = 1.0;  // relaxation weight of the level
+  // what the z-marching kernels read: the hot arrays of an extruded level (rows by zm)
   long long pitch = 0;
+  ZMap zm;
   const double *coef = nullptr, *dinv = nullptr, *wm = nullptr, *wp = nullptr;
+  // the full arrays (one-point kernels)
+  long long fpitch = 0;
+  const double *fcoef = nullptr, *fdinv = nullptr, *fwm = nullptr, *fwp = nullptr;
   double *b = nullptr, *r = nullptr, *y = nullptr;
 };
 
@@ -36,7 +41,7 @@ struct GmresBatchView {
   long long N = 0, vp = 0;
   int kd = 0, nred = 1, maxit = 0;
   double tol = 0.0, atol = 0.0;
-  const double *dinv = nullptr;
+  const double *dinv = nullptr;  // fused Jacobi: 1/diag at every point, or null
   fpc_StructMatrix_s *A = nullptr;
   void *pfmg = nullptr;
   int *pred = nullptr, *its = nullptr;

@@ -5,6 +5,15 @@
 // stored as seven x-contiguous planes ("SoA"), canonical slot order
 // C, -x, +x, -y, +y, -z, +z (the reference stencil order, hypre_struct_utils.hpp:9),
 // each plane padded to a 256-byte multiple so every plane starts aligned.
+//
+// An operator extruded along z (a full-height obstacle between z walls: the
+// planes away from the walls repeat) also keeps a "hot" copy for the stencil
+// kernels of the solve loops: m planes at each wall, one interior plane, then
+// all rows again.  Points whose row differs from the interior plane (rounding
+// in the assembly, a boundary layer deeper than m) are flagged and read their
+// own row from the full part, so results are those of the full planes bit for
+// bit while the coefficient traffic is about 2m+1 planes instead of nz.  Every
+// other reader keeps the full planes.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -107,6 +116,23 @@ inline void launch_shape(const Dims &d, dim3 &grid, dim3 &block) {
   grid = dim3((d.nx + tx - 1) / tx, (d.ny + ty - 1) / ty, d.nz);
 }
 
+// Row of the stored planes that holds point t (on z-plane k).  pl == 0: full
+// storage, row t.  Otherwise the hot copy: a flagged point reads row xoff + t;
+// any other reads compact plane kc -- k for the m bottom planes, m for the
+// interior, m+1 .. 2m for the m top planes -- at row t + (kc - k) * pl.
+struct ZMap {
+  long long pl = 0;    // nx * ny of the box, or 0
+  int nz = 0, m = 0;   // box depth, wall planes kept per side
+  long long xoff = 0;  // first row of the full part
+  const unsigned char *xf = nullptr;  // per point: 1 = read the full part (null: none)
+  __host__ __device__ inline long long operator()(long long t, int k) const {
+    if (!pl) return t;
+    if (xf && xf[t]) return xoff + t;
+    const int kc = (k < m) ? k : ((k >= nz - m) ? k - nz + 2 * m + 1 : m);
+    return t + (long long)(kc - k) * pl;
+  }
+};
+
 template <typename T>
 struct Stencil {
   const T *c;        // 7 planes
@@ -155,6 +181,12 @@ struct fpc_StructMatrix_s {
   double *dinv = nullptr;    // device, 1 / diagonal (Jacobi)
   float *coef32 = nullptr;   // device, FP32 copy (mixed-precision paths)
   float *dinv32 = nullptr;
+  // hot copies of an extruded operator (zpitch apart, rows by zmap; zmap.pl != 0 when present)
+  double *zcoef = nullptr, *zdinv = nullptr;
+  float *zcoef32 = nullptr, *zdinv32 = nullptr;
+  unsigned char *zxf = nullptr;  // zmap.xf
+  long long zpitch = 0, zexc = 0;  // zexc: flagged points
+  fpc::ZMap zmap;
   bool initialized = false, assembled = false;
   unsigned long long version = 0;  // bumped whenever values change
 };
@@ -206,6 +238,52 @@ int dot_partials(const double *a, const double *b, long long n, double *part);
 int matvec(double alpha, const fpc_StructMatrix_s *A, const double *x, double beta, double *y);
 int ensure_dinv(fpc_StructMatrix_s *A);
 int ensure_fp32(fpc_StructMatrix_s *A);
+
+// The planes the solve loops' stencil kernels read: the extruded copy when the
+// operator has one, else the full planes (ensure_dinv / ensure_fp32 first).
+template <typename T>
+struct Planes {
+  const T *c = nullptr, *dinv = nullptr;
+  long long pitch = 0;
+  ZMap zm;
+};
+inline Planes<double> planes64(const fpc_StructMatrix_s *A) {
+  Planes<double> p;
+  if (A->zmap.pl) {
+    p.c = A->zcoef; p.dinv = A->zdinv; p.pitch = A->zpitch; p.zm = A->zmap;
+  } else {
+    p.c = A->coef; p.dinv = A->dinv; p.pitch = A->pitch;
+  }
+  return p;
+}
+inline Planes<float> planes32(const fpc_StructMatrix_s *A) {
+  Planes<float> p;
+  if (A->zmap.pl) {
+    p.c = A->zcoef32; p.dinv = A->zdinv32; p.pitch = A->zpitch; p.zm = A->zmap;
+  } else {
+    p.c = A->coef32; p.dinv = A->dinv32; p.pitch = A->pitch;
+  }
+  return p;
+}
+// The 1/diag the fused Jacobi matvec reads: in the storage planes64 / planes32
+// use (hot), or the full plane.
+template <typename DT>
+inline const DT *jacobi_dinv(const fpc_StructMatrix_s *A, bool hot = true);
+template <>
+inline const double *jacobi_dinv<double>(const fpc_StructMatrix_s *A, bool hot) {
+  return (hot && A->zmap.pl) ? A->zdinv : A->dinv;
+}
+template <>
+inline const float *jacobi_dinv<float>(const fpc_StructMatrix_s *A, bool hot) {
+  return (hot && A->zmap.pl) ? A->zdinv32 : A->dinv32;
+}
+// Coefficient rows a stencil pass reads: the compact planes and the flagged
+// points' own rows of a hot copy, or every row.
+inline double stored_rows(const ZMap &zm, long long N, long long nexc) {
+  return zm.pl ? (double)(2 * zm.m + 1) * (double)zm.pl + (double)nexc : (double)N;
+}
+// Whether new operators and MG levels get extruded copies ("z_extrude" tuning key).
+bool z_extrude_enabled();
 // Device scratch owned by the library (grows on demand, freed at Finalize).
 double *scratch_doubles(size_t n, int which);
 

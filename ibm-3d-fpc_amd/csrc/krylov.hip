@@ -41,7 +41,7 @@ using namespace kry;
 // w = A z with z = sigma * p (* dinv when Jacobi is fused), one thread per point.
 // The current basis index is read from the device status.
 template <typename DT>
-__global__ void k_arnoldi_matvec(const double *__restrict__ c, long long pitch, int nx, int ny, int nz,
+__global__ void k_arnoldi_matvec(const double *__restrict__ c, long long pitch, ZMap zm, int nx, int ny, int nz,
                                  const double *__restrict__ V, long long vp, const double *__restrict__ zin,
                                  const DT *__restrict__ dinv, State *S) {
   if (S->st.done) return;
@@ -54,40 +54,45 @@ __global__ void k_arnoldi_matvec(const double *__restrict__ c, long long pitch, 
   double sg = S->sigma[cur];
   double *w = (double *)V + (long long)(cur + 1) * vp;
   long long r = ((long long)k * ny + j) * nx + i;
-  double s = arnoldi_row(c, pitch, nx, ny, nz, z, dinv, i, j, k, r);
+  double s = arnoldi_row(c, pitch, zm, nx, ny, nz, z, dinv, i, j, k, r);
   w[r] = sg * s;
 }
 
 template <bool kDinv, int TY, typename DT>
 __global__ void __launch_bounds__(kTileX *TY)
-    k_arnoldi_matvec_tiled(const double *__restrict__ c, long long pitch, int nx, int ny, int nz,
+    k_arnoldi_matvec_tiled(const double *__restrict__ c, long long pitch, ZMap zm, int nx, int ny, int nz,
                            const double *__restrict__ V, long long vp, const double *__restrict__ zin,
                            const DT *__restrict__ dinv, State *S, int kz) {
   if (S->st.done) return;
   const int cur = S->st.i;
   ArnoldiOp<kDinv, DT> op{c, pitch, nx, ny, zin ? zin : V + (long long)cur * vp, dinv,
-                          (double *)V + (long long)(cur + 1) * vp, S->sigma[cur]};
+                          (double *)V + (long long)(cur + 1) * vp, S->sigma[cur], zm};
   const int k0 = blockIdx.z * kz;
   march_tile<double, kTileX, TY>(op, nx, ny, nz, k0, min(k0 + kz, nz), blockIdx.x, blockIdx.y);
 }
 
-// w = sigma A z (z * 1/diag when dinv is given) for the current basis index.
+// w = sigma A z (z * 1/diag when jac: the FP64 or, DT = float, the FP32 1/diag)
+// for the current basis index.  The z-marching kernels read the operator's hot
+// copy when it has one; the one-point kernels (latency-bound boxes) the full planes.
 template <typename DT>
-int launch_arnoldi_matvec(const fpc_StructMatrix_s *A, const double *V, long long vp, const double *zin,
-                          const DT *dinv, State *S) {
+int launch_arnoldi_matvec(const fpc_StructMatrix_s *A, const double *V, long long vp, const double *zin, bool jac,
+                          State *S) {
   Dims d = dims_of(A->grid);
   if (tile_worthwhile(d)) {
+    const Planes<double> P = planes64(A);
+    const DT *dinv = jac ? jacobi_dinv<DT>(A) : nullptr;
     TileLaunch tl = tile_launch<8>(d);
     if (dinv)
-      k_arnoldi_matvec_tiled<true, 8, DT><<<tl.grid, tl.block, 0, stream()>>>(A->coef, A->pitch, d.nx, d.ny, d.nz, V,
+      k_arnoldi_matvec_tiled<true, 8, DT><<<tl.grid, tl.block, 0, stream()>>>(P.c, P.pitch, P.zm, d.nx, d.ny, d.nz, V,
                                                                               vp, zin, dinv, S, tl.kz);
     else
-      k_arnoldi_matvec_tiled<false, 8, DT><<<tl.grid, tl.block, 0, stream()>>>(A->coef, A->pitch, d.nx, d.ny, d.nz,
+      k_arnoldi_matvec_tiled<false, 8, DT><<<tl.grid, tl.block, 0, stream()>>>(P.c, P.pitch, P.zm, d.nx, d.ny, d.nz,
                                                                                V, vp, zin, nullptr, S, tl.kz);
   } else {
+    const DT *dinv = jac ? jacobi_dinv<DT>(A, false) : nullptr;
     dim3 g, b;
     launch_shape(d, g, b);
-    k_arnoldi_matvec<DT><<<g, b, 0, stream()>>>(A->coef, A->pitch, d.nx, d.ny, d.nz, V, vp, zin, dinv, S);
+    k_arnoldi_matvec<DT><<<g, b, 0, stream()>>>(A->coef, A->pitch, ZMap(), d.nx, d.ny, d.nz, V, vp, zin, dinv, S);
   }
   FPC_LAUNCH_CHECK();
   return 0;
@@ -463,14 +468,13 @@ static int true_residual(GmresImpl *G, const double *b, const double *x, double 
 
 // One Arnoldi matvec with no preconditioner or with the fused one-sweep Jacobi.
 static int arnoldi_step(GmresImpl *G) {
-  Dims d = dims_of(G->A->grid);
   const bool jac = G->pc_kind == PC_JACOBI1;
   const bool j32 = jac && ((JacobiImpl *)G->pc->impl)->bits == 32;
   // bytes per row: 7 coefficients, the input, 1/diag when Jacobi is fused (4 bytes in FP32), the output
   FPC_PROF_BEGIN(jac ? "spmv_jacobi" : "spmv");
-  int rc = j32 ? launch_arnoldi_matvec<float>(G->A, G->V, G->vp, nullptr, G->A->dinv32, G->S)
-               : launch_arnoldi_matvec<double>(G->A, G->V, G->vp, nullptr, jac ? G->A->dinv : nullptr, G->S);
-  FPC_PROF_END(jac ? "spmv_jacobi" : "spmv", (double)d.N * (jac ? (j32 ? 76.0 : 80.0) : 72.0));
+  int rc = j32 ? launch_arnoldi_matvec<float>(G->A, G->V, G->vp, nullptr, true, G->S)
+               : launch_arnoldi_matvec<double>(G->A, G->V, G->vp, nullptr, jac, G->S);
+  FPC_PROF_END(jac ? "spmv_jacobi" : "spmv", stencil_bytes(G->A, jac ? (j32 ? 4 : 8) : 0, 16.0, true));
   return rc;
 }
 
@@ -501,15 +505,14 @@ static int arnoldi_tail(GmresImpl *G, int step) {
 // One Arnoldi step when the preconditioner runs separately; step = host-side index
 // of the current basis vector (used for the flexible Z slot).
 static int arnoldi_step_pc(GmresImpl *G, int step) {
-  Dims d = dims_of(G->A->grid);
   int *skip = &G->S->st.done;
   double *z = G->flexible ? G->Z + (long long)step * G->vp : G->Z;
   // the preconditioner input is the stored (unnormalised) p_step
   int rc = precond_apply(G, G->V + (long long)step * G->vp, z, skip);
   if (rc) return rc;
   FPC_PROF_BEGIN("spmv");
-  rc = launch_arnoldi_matvec<double>(G->A, G->V, G->vp, z, nullptr, G->S);
-  FPC_PROF_END("spmv", (double)d.N * 72.0);
+  rc = launch_arnoldi_matvec<double>(G->A, G->V, G->vp, z, false, G->S);
+  FPC_PROF_END("spmv", stencil_bytes(G->A, 0, 16.0, true));
   return rc;
 }
 

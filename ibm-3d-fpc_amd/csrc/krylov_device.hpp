@@ -44,28 +44,30 @@ __device__ inline double &H(State *S, int row, int col) { return S->hh[col * (KM
 
 // (A z)_r at point (i, j, k) = row r, with z scaled by 1/diag when dinv is given.
 // DT: the stored 1/diag (double, or float for the FP32 Jacobi); the products are FP64.
+// c and dinv are read at the rows zm gives (full or z-extruded storage).
 template <typename DT = double>
-__device__ inline double arnoldi_row(const double *__restrict__ c, long long pitch, int nx, int ny, int nz,
+__device__ inline double arnoldi_row(const double *__restrict__ c, long long pitch, ZMap zm, int nx, int ny, int nz,
                                      const double *__restrict__ z, const DT *__restrict__ dinv, int i, int j, int k,
                                      long long r) {
   const long long pl = (long long)nx * ny;
+  const long long q = zm(r, k);
   double s;
   if (dinv) {
-    s = c[r] * (z[r] * (double)dinv[r]);
-    if (i > 0) s += c[pitch + r] * (z[r - 1] * (double)dinv[r - 1]);
-    if (i < nx - 1) s += c[2 * pitch + r] * (z[r + 1] * (double)dinv[r + 1]);
-    if (j > 0) s += c[3 * pitch + r] * (z[r - nx] * (double)dinv[r - nx]);
-    if (j < ny - 1) s += c[4 * pitch + r] * (z[r + nx] * (double)dinv[r + nx]);
-    if (k > 0) s += c[5 * pitch + r] * (z[r - pl] * (double)dinv[r - pl]);
-    if (k < nz - 1) s += c[6 * pitch + r] * (z[r + pl] * (double)dinv[r + pl]);
+    s = c[q] * (z[r] * (double)dinv[q]);
+    if (i > 0) s += c[pitch + q] * (z[r - 1] * (double)dinv[zm(r - 1, k)]);
+    if (i < nx - 1) s += c[2 * pitch + q] * (z[r + 1] * (double)dinv[zm(r + 1, k)]);
+    if (j > 0) s += c[3 * pitch + q] * (z[r - nx] * (double)dinv[zm(r - nx, k)]);
+    if (j < ny - 1) s += c[4 * pitch + q] * (z[r + nx] * (double)dinv[zm(r + nx, k)]);
+    if (k > 0) s += c[5 * pitch + q] * (z[r - pl] * (double)dinv[zm(r - pl, k - 1)]);
+    if (k < nz - 1) s += c[6 * pitch + q] * (z[r + pl] * (double)dinv[zm(r + pl, k + 1)]);
   } else {
-    s = c[r] * z[r];
-    if (i > 0) s += c[pitch + r] * z[r - 1];
-    if (i < nx - 1) s += c[2 * pitch + r] * z[r + 1];
-    if (j > 0) s += c[3 * pitch + r] * z[r - nx];
-    if (j < ny - 1) s += c[4 * pitch + r] * z[r + nx];
-    if (k > 0) s += c[5 * pitch + r] * z[r - pl];
-    if (k < nz - 1) s += c[6 * pitch + r] * z[r + pl];
+    s = c[q] * z[r];
+    if (i > 0) s += c[pitch + q] * z[r - 1];
+    if (i < nx - 1) s += c[2 * pitch + q] * z[r + 1];
+    if (j > 0) s += c[3 * pitch + q] * z[r - nx];
+    if (j < ny - 1) s += c[4 * pitch + q] * z[r + nx];
+    if (k > 0) s += c[5 * pitch + q] * z[r - pl];
+    if (k < nz - 1) s += c[6 * pitch + q] * z[r + pl];
   }
   return s;
 }
@@ -251,7 +253,7 @@ __device__ inline void cycle_start_body(State *S, double r_norm, double eps, int
 }
 
 // r = b - A x (r optional) with the block's partial sum of r^2 into part[bid]; block bid of nblk.
-__device__ inline void residual_body(const double *__restrict__ c, long long pitch, int nx, int ny, int nz,
+__device__ inline void residual_body(const double *__restrict__ c, long long pitch, ZMap zm, int nx, int ny, int nz,
                                      const double *__restrict__ b, const double *__restrict__ x, double *__restrict__ r,
                                      double *__restrict__ part, int bid, int nblk) {
   __shared__ double lds[8];
@@ -262,13 +264,14 @@ __device__ inline void residual_body(const double *__restrict__ c, long long pit
     long long q = t / nx;
     int j = (int)(q % ny);
     int k = (int)(q / ny);
-    double s = c[t] * x[t];
-    if (i > 0) s += c[pitch + t] * x[t - 1];
-    if (i < nx - 1) s += c[2 * pitch + t] * x[t + 1];
-    if (j > 0) s += c[3 * pitch + t] * x[t - nx];
-    if (j < ny - 1) s += c[4 * pitch + t] * x[t + nx];
-    if (k > 0) s += c[5 * pitch + t] * x[t - pl];
-    if (k < nz - 1) s += c[6 * pitch + t] * x[t + pl];
+    const long long u = zm(t, k);
+    double s = c[u] * x[t];
+    if (i > 0) s += c[pitch + u] * x[t - 1];
+    if (i < nx - 1) s += c[2 * pitch + u] * x[t + 1];
+    if (j > 0) s += c[3 * pitch + u] * x[t - nx];
+    if (j < ny - 1) s += c[4 * pitch + u] * x[t + nx];
+    if (k > 0) s += c[5 * pitch + u] * x[t - pl];
+    if (k < nz - 1) s += c[6 * pitch + u] * x[t + pl];
     double rv = b[t] - s;
     if (r) r[t] = rv;
     acc += rv * rv;
@@ -306,6 +309,7 @@ struct ArnoldiOp {
   const DT *dinv;
   double *w;
   double sg;
+  ZMap zm;  // rows of c and dinv
   struct Raw {
     double z, dinv;
   };
@@ -315,12 +319,12 @@ struct ArnoldiOp {
   __device__ long long at(int i, int j, int k) const { return ((long long)k * ny + j) * nx + i; }
   __device__ Raw fetch(int i, int j, int k) const {
     long long t = at(i, j, k);
-    return Raw{z[t], kDinv ? (double)dinv[t] : 1.0};
+    return Raw{z[t], kDinv ? (double)dinv[zm(t, k)] : 1.0};
   }
   __device__ double finish(const Raw &r) const { return kDinv ? r.z * r.dinv : r.z; }
   __device__ Row row(int i, int j, int k) const {
     Row q;
-    q.a.load(c, pitch, at(i, j, k));
+    q.a.load(c, pitch, zm(at(i, j, k), k));
     return q;
   }
   __device__ void out(const Row &q, int i, int j, int k, const Nbr<double> &nb) { w[at(i, j, k)] = sg * q.a.apply(nb); }

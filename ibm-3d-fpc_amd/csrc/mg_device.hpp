@@ -1,6 +1,10 @@
 // mg_device.hpp -- point bodies of the one-point-per-thread multigrid kernels
 // and the z-marching tile operations of the same stages (libfpc).
 //
+// Coefficients, 1/diag and the interpolation weights of a point are read at the
+// row a ZMap gives for that point (full storage, or the hot copy of a z-extruded
+// level); vectors are always full.
+//
 // The level kernels of one hierarchy (pfmg.hip) and the batched kernels that run
 // a level of many hierarchies in one launch (batched.hip) evaluate these same
 // functions, so a V-cycle in a batch is bit for bit the V-cycle alone, whichever
@@ -16,17 +20,18 @@ namespace mgd {
 
 // Residual of one weighted-Jacobi sweep from zero: b - A (w D^-1 b) at (i, j, k) = row t.
 template <typename T>
-__device__ inline T down_point(const T *__restrict__ c, long long pitch, int nx, int ny, int nz,
+__device__ inline T down_point(const T *__restrict__ c, long long pitch, ZMap zm, int nx, int ny, int nz,
                                const T *__restrict__ dinv, const T *__restrict__ b, T w, int i, int j, int k,
                                long long t) {
   const long long pl = (long long)nx * ny;
-  T s = c[t] * (w * b[t] * dinv[t]);
-  if (i > 0) s += c[pitch + t] * (w * b[t - 1] * dinv[t - 1]);
-  if (i < nx - 1) s += c[2 * pitch + t] * (w * b[t + 1] * dinv[t + 1]);
-  if (j > 0) s += c[3 * pitch + t] * (w * b[t - nx] * dinv[t - nx]);
-  if (j < ny - 1) s += c[4 * pitch + t] * (w * b[t + nx] * dinv[t + nx]);
-  if (k > 0) s += c[5 * pitch + t] * (w * b[t - pl] * dinv[t - pl]);
-  if (k < nz - 1) s += c[6 * pitch + t] * (w * b[t + pl] * dinv[t + pl]);
+  const long long u = zm(t, k);
+  T s = c[u] * (w * b[t] * dinv[u]);
+  if (i > 0) s += c[pitch + u] * (w * b[t - 1] * dinv[zm(t - 1, k)]);
+  if (i < nx - 1) s += c[2 * pitch + u] * (w * b[t + 1] * dinv[zm(t + 1, k)]);
+  if (j > 0) s += c[3 * pitch + u] * (w * b[t - nx] * dinv[zm(t - nx, k)]);
+  if (j < ny - 1) s += c[4 * pitch + u] * (w * b[t + nx] * dinv[zm(t + nx, k)]);
+  if (k > 0) s += c[5 * pitch + u] * (w * b[t - pl] * dinv[zm(t - pl, k - 1)]);
+  if (k < nz - 1) s += c[6 * pitch + u] * (w * b[t + pl] * dinv[zm(t + pl, k + 1)]);
   return b[t] - s;
 }
 
@@ -34,14 +39,15 @@ __device__ inline T down_point(const T *__restrict__ c, long long pitch, int nx,
 // r(C) + w+(F-) r(F-) + w-(F+) r(F+), C = fine 2c+1.
 template <typename T>
 __device__ inline T restrict_point(int fnx, int fny, int fnz, int d, const T *__restrict__ r, const T *__restrict__ wm,
-                                   const T *__restrict__ wp, int i, int j, int k) {
+                                   const T *__restrict__ wp, ZMap zm, int i, int j, int k) {
   const int fi = (d == 0) ? 2 * i + 1 : i, fj = (d == 1) ? 2 * j + 1 : j, fk = (d == 2) ? 2 * k + 1 : k;
   const int pd = (d == 0) ? fi : (d == 1 ? fj : fk);
   const int fnd = (d == 0) ? fnx : (d == 1 ? fny : fnz);
   const long long stride = (d == 0) ? 1 : (d == 1 ? (long long)fnx : (long long)fnx * fny);
   const long long f0 = ((long long)fk * fny + fj) * fnx + fi;
-  T v = r[f0] + wp[f0 - stride] * r[f0 - stride];
-  if (pd + 1 < fnd) v += wm[f0 + stride] * r[f0 + stride];
+  const int dk = (d == 2) ? 1 : 0;
+  T v = r[f0] + wp[zm(f0 - stride, fk - dk)] * r[f0 - stride];
+  if (pd + 1 < fnd) v += wm[zm(f0 + stride, fk + dk)] * r[f0 + stride];
   return v;
 }
 
@@ -49,10 +55,10 @@ __device__ inline T restrict_point(int fnx, int fny, int fnz, int d, const T *__
 // the level coarsened in direction D (C points inject, F points take the
 // weighted neighbours).
 template <typename T, int D>
-__device__ inline T corrected(const T *b, const T *dinv, const T *wm, const T *wp, const T *yc, T w, int i, int j,
-                              int k, int fnx, int fny, int cnx, int cny, int ncd) {
-  const long long t = ((long long)k * fny + j) * fnx + i;
-  T x = w * b[t] * dinv[t];
+__device__ inline T corrected(const T *b, const T *dinv, const T *wm, const T *wp, const T *yc, ZMap zm, T w, int i,
+                              int j, int k, int fnx, int fny, int cnx, int cny, int ncd) {
+  const long long t = ((long long)k * fny + j) * fnx + i, u = zm(t, k);
+  T x = w * b[t] * dinv[u];
   const int pd = (D == 0) ? i : (D == 1 ? j : k);
   const long long cstride = (D == 0) ? 1 : (D == 1 ? (long long)cnx : (long long)cnx * cny);
   const int q = (pd & 1) ? ((pd - 1) >> 1) : (pd >> 1);
@@ -62,8 +68,8 @@ __device__ inline T corrected(const T *b, const T *dinv, const T *wm, const T *w
     x += yc[cr];
   } else {
     T s = (T)0;
-    if (q - 1 >= 0) s += wm[t] * yc[cr - cstride];
-    if (q < ncd) s += wp[t] * yc[cr];
+    if (q - 1 >= 0) s += wm[u] * yc[cr - cstride];
+    if (q < ncd) s += wp[u] * yc[cr];
     x += s;
   }
   return x;
@@ -72,21 +78,22 @@ __device__ inline T corrected(const T *b, const T *dinv, const T *wm, const T *w
 // Prolongation + one weighted-Jacobi sweep at (i, j, k) = row t:
 // y = x + w D^-1 (b - A x) with x = corrected() at the point and its neighbours.
 template <typename T, int D>
-__device__ inline T up_point(const T *__restrict__ c, long long pitch, int nx, int ny, int nz,
+__device__ inline T up_point(const T *__restrict__ c, long long pitch, ZMap zm, int nx, int ny, int nz,
                              const T *__restrict__ dinv, const T *__restrict__ b, const T *__restrict__ wm,
                              const T *__restrict__ wp, const T *__restrict__ yc, int cnx, int cny, int ncd, T w, int i,
                              int j, int k, long long t) {
-#define FPC_CORR(ii, jj, kk) corrected<T, D>(b, dinv, wm, wp, yc, w, ii, jj, kk, nx, ny, cnx, cny, ncd)
+#define FPC_CORR(ii, jj, kk) corrected<T, D>(b, dinv, wm, wp, yc, zm, w, ii, jj, kk, nx, ny, cnx, cny, ncd)
+  const long long u = zm(t, k);
   T xc = FPC_CORR(i, j, k);
-  T s = c[t] * xc;
-  if (i > 0) s += c[pitch + t] * FPC_CORR(i - 1, j, k);
-  if (i < nx - 1) s += c[2 * pitch + t] * FPC_CORR(i + 1, j, k);
-  if (j > 0) s += c[3 * pitch + t] * FPC_CORR(i, j - 1, k);
-  if (j < ny - 1) s += c[4 * pitch + t] * FPC_CORR(i, j + 1, k);
-  if (k > 0) s += c[5 * pitch + t] * FPC_CORR(i, j, k - 1);
-  if (k < nz - 1) s += c[6 * pitch + t] * FPC_CORR(i, j, k + 1);
+  T s = c[u] * xc;
+  if (i > 0) s += c[pitch + u] * FPC_CORR(i - 1, j, k);
+  if (i < nx - 1) s += c[2 * pitch + u] * FPC_CORR(i + 1, j, k);
+  if (j > 0) s += c[3 * pitch + u] * FPC_CORR(i, j - 1, k);
+  if (j < ny - 1) s += c[4 * pitch + u] * FPC_CORR(i, j + 1, k);
+  if (k > 0) s += c[5 * pitch + u] * FPC_CORR(i, j, k - 1);
+  if (k < nz - 1) s += c[6 * pitch + u] * FPC_CORR(i, j, k + 1);
 #undef FPC_CORR
-  return xc + w * (b[t] - s) * dinv[t];
+  return xc + w * (b[t] - s) * dinv[u];
 }
 
 // Row r of the coarsest solve x = inv b (FP64 sums), one wavefront; valid in lane 0.
@@ -121,6 +128,7 @@ struct DownOp {
   const T *dinv, *b;
   T *r;
   T w;
+  ZMap zm;  // rows of c and dinv
   struct Raw {
     T b, dinv;
   };
@@ -131,13 +139,13 @@ struct DownOp {
   __device__ long long at(int i, int j, int k) const { return ((long long)k * ny + j) * nx + i; }
   __device__ Raw fetch(int i, int j, int k) const {
     long long t = at(i, j, k);
-    return Raw{b[t], dinv[t]};
+    return Raw{b[t], dinv[zm(t, k)]};
   }
   __device__ T finish(const Raw &q) const { return w * q.b * q.dinv; }
   __device__ Row row(int i, int j, int k) const {
     long long t = at(i, j, k);
     Row q;
-    q.a.load(c, pitch, t);
+    q.a.load(c, pitch, zm(t, k));
     q.b = b[t];
     return q;
   }
@@ -158,13 +166,13 @@ struct DownRestrictZOp : DownOp<T> {
     const long long t = this->at(i, j, k), pl = (long long)this->nx * this->ny;
     const T rk = q.b - q.a.apply(nb);
     if (k & 1) {
-      const T v = rk + wp[t - pl] * r_prev;
+      const T v = rk + wp[this->zm(t - pl, k - 1)] * r_prev;
       const int kc = (k - 1) >> 1;
       if (k + 1 < nz) pend = v;
       else if (kc < ncz) bc[((long long)kc * this->ny + j) * this->nx + i] = v;
     } else if (k > k0) {
       const int kc = (k >> 1) - 1;
-      if (kc < ncz) bc[((long long)kc * this->ny + j) * this->nx + i] = pend + wm[t] * rk;
+      if (kc < ncz) bc[((long long)kc * this->ny + j) * this->nx + i] = pend + wm[this->zm(t, k)] * rk;
     }
     r_prev = rk;
   }
@@ -181,6 +189,7 @@ struct UpOp {
   int cnx, cny, ncd;
   T *y;
   T w;
+  ZMap zm;  // rows of c, dinv, wm and wp
   // Loads of corrected<T, D>: C points use e1 only; at F points an absent coarse
   // neighbour reads as a zero weight and value (0 + wp e1 == wp e1 exactly).
   struct Raw {
@@ -199,18 +208,19 @@ struct UpOp {
     const int q = (pd & 1) ? ((pd - 1) >> 1) : (pd >> 1);
     const int ci = (D == 0) ? q : i, cj = (D == 1) ? q : j, ck = (D == 2) ? q : k;
     const long long cr = ((long long)ck * cny + cj) * cnx + ci;
+    const long long u = zm(t, k);
     Raw r;
     r.b = b[t];
-    r.dinv = dinv[t];
+    r.dinv = dinv[u];
     r.f = !(pd & 1);
     if (pd & 1) {
       r.wm = r.wp = r.e0 = T(0);
       r.e1 = yc[cr];
     } else {
       const bool lo = q - 1 >= 0, hi = q < ncd;
-      r.wm = lo ? wm[t] : T(0);
+      r.wm = lo ? wm[u] : T(0);
       r.e0 = lo ? yc[cr - cstride] : T(0);
-      r.wp = hi ? wp[t] : T(0);
+      r.wp = hi ? wp[u] : T(0);
       r.e1 = hi ? yc[cr] : T(0);
     }
     return r;
@@ -228,11 +238,11 @@ struct UpOp {
     return x;
   }
   __device__ Row row(int i, int j, int k) const {
-    long long t = at(i, j, k);
+    long long t = at(i, j, k), u = zm(t, k);
     Row q;
-    q.a.load(c, pitch, t);
+    q.a.load(c, pitch, u);
     q.b = b[t];
-    q.dinv = dinv[t];
+    q.dinv = dinv[u];
     return q;
   }
   __device__ void out(const Row &q, int i, int j, int k, const Nbr<T> &nb) {

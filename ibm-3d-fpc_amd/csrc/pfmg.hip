@@ -39,6 +39,7 @@
 #include <thread>
 #include <vector>
 
+#include "extrude.hpp"
 #include "fpc_internal.hpp"
 #include "batched.hpp"
 #include "mg_device.hpp"
@@ -341,13 +342,14 @@ __global__ void k_relax_rb(const T *__restrict__ c, long long pitch, int nx, int
 // b_c = r(F0) + w+(Fm) r(Fm) + w-(Fp) r(Fp), one thread per coarse point.
 template <typename T>
 __global__ void k_restrict(int fnx, int fny, int fnz, int d, const T *__restrict__ r, const T *__restrict__ wm,
-                           const T *__restrict__ wp, int cnx, int cny, int cnz, T *__restrict__ bc, const int *skip) {
+                           const T *__restrict__ wp, ZMap zm, int cnx, int cny, int cnz, T *__restrict__ bc,
+                           const int *skip) {
   if (skip && *skip) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y;
   int k = blockIdx.z;
   if (i >= cnx || j >= cny) return;
-  bc[((long long)k * cny + j) * cnx + i] = mgd::restrict_point<T>(fnx, fny, fnz, d, r, wm, wp, i, j, k);
+  bc[((long long)k * cny + j) * cnx + i] = mgd::restrict_point<T>(fnx, fny, fnz, d, r, wm, wp, zm, i, j, k);
 }
 
 // x_f += P x_c, one thread per fine point.
@@ -382,7 +384,7 @@ __global__ void k_prolong_add(int fnx, int fny, int fnz, int d, const T *__restr
 // x = w b dinv is recomputed on the fly at every stencil point, so only the
 // residual r = b - A x is written (x is recomputed by the up-sweep).
 template <typename T>
-__global__ void k_down_jacobi(const T *__restrict__ c, long long pitch, int nx, int ny, int nz,
+__global__ void k_down_jacobi(const T *__restrict__ c, long long pitch, ZMap zm, int nx, int ny, int nz,
                               const T *__restrict__ dinv, const T *__restrict__ b, T *__restrict__ r, T w,
                               const int *skip) {
   if (skip && *skip) return;
@@ -391,11 +393,11 @@ __global__ void k_down_jacobi(const T *__restrict__ c, long long pitch, int nx, 
   int k = blockIdx.z;
   if (i >= nx || j >= ny) return;
   long long t = ((long long)k * ny + j) * nx + i;
-  r[t] = mgd::down_point<T>(c, pitch, nx, ny, nz, dinv, b, w, i, j, k, t);
+  r[t] = mgd::down_point<T>(c, pitch, zm, nx, ny, nz, dinv, b, w, i, j, k, t);
 }
 
 template <typename T, int D>
-__global__ void k_up_jacobi(const T *__restrict__ c, long long pitch, int nx, int ny, int nz,
+__global__ void k_up_jacobi(const T *__restrict__ c, long long pitch, ZMap zm, int nx, int ny, int nz,
                             const T *__restrict__ dinv, const T *__restrict__ b, const T *__restrict__ wm,
                             const T *__restrict__ wp, const T *__restrict__ yc, int cnx, int cny, int ncd,
                             T *__restrict__ y, T w, const int *skip) {
@@ -405,7 +407,7 @@ __global__ void k_up_jacobi(const T *__restrict__ c, long long pitch, int nx, in
   int k = blockIdx.z;
   if (i >= nx || j >= ny) return;
   long long t = ((long long)k * ny + j) * nx + i;
-  y[t] = mgd::up_point<T, D>(c, pitch, nx, ny, nz, dinv, b, wm, wp, yc, cnx, cny, ncd, w, i, j, k, t);
+  y[t] = mgd::up_point<T, D>(c, pitch, zm, nx, ny, nz, dinv, b, wm, wp, yc, cnx, cny, ncd, w, i, j, k, t);
 }
 
 // x = Ainv b on the coarsest level (FP64 inverse), one block.
@@ -415,22 +417,23 @@ __global__ void k_up_jacobi(const T *__restrict__ c, long long pitch, int nx, in
 
 template <typename T, int TY>
 __global__ void __launch_bounds__(kTileX *TY)
-    k_down_tiled(const T *__restrict__ c, long long pitch, int nx, int ny, int nz, const T *__restrict__ dinv,
+    k_down_tiled(const T *__restrict__ c, long long pitch, ZMap zm, int nx, int ny, int nz, const T *__restrict__ dinv,
                  const T *__restrict__ b, T *__restrict__ r, T w, int kz, const int *skip) {
   if (skip && *skip) return;
-  mgd::DownOp<T> op{c, pitch, nx, ny, dinv, b, r, w};
+  mgd::DownOp<T> op{c, pitch, nx, ny, dinv, b, r, w, zm};
   const int k0 = blockIdx.z * kz;
   march_tile<T, kTileX, TY>(op, nx, ny, nz, k0, min(k0 + kz, nz), blockIdx.x, blockIdx.y);
 }
 
 template <typename T, int TY>
 __global__ void __launch_bounds__(kTileX *TY)
-    k_down_restrict_z(const T *__restrict__ c, long long pitch, int nx, int ny, int nz, const T *__restrict__ dinv,
-                      const T *__restrict__ b, const T *__restrict__ wm, const T *__restrict__ wp,
-                      T *__restrict__ bc, int ncz, T w, int kz, const int *skip) {
+    k_down_restrict_z(const T *__restrict__ c, long long pitch, ZMap zm, int nx, int ny, int nz,
+                      const T *__restrict__ dinv, const T *__restrict__ b, const T *__restrict__ wm,
+                      const T *__restrict__ wp, T *__restrict__ bc, int ncz, T w, int kz, const int *skip) {
   if (skip && *skip) return;
   mgd::DownRestrictZOp<T> op;
   op.c = c; op.pitch = pitch; op.nx = nx; op.ny = ny; op.dinv = dinv; op.b = b; op.r = nullptr; op.w = w;
+  op.zm = zm;
   op.wm = wm; op.wp = wp; op.bc = bc; op.nz = nz; op.ncz = ncz;
   const int k0 = blockIdx.z * kz, k1 = min(k0 + kz, nz);
   op.k0 = k0;
@@ -439,12 +442,12 @@ __global__ void __launch_bounds__(kTileX *TY)
 
 template <typename T, int D, int TY>
 __global__ void __launch_bounds__(kTileX *TY)
-    k_up_tiled(const T *__restrict__ c, long long pitch, int nx, int ny, int nz, const T *__restrict__ dinv,
+    k_up_tiled(const T *__restrict__ c, long long pitch, ZMap zm, int nx, int ny, int nz, const T *__restrict__ dinv,
                const T *__restrict__ b, const T *__restrict__ wm, const T *__restrict__ wp,
                const T *__restrict__ yc, int cnx, int cny, int ncd, T *__restrict__ y, T w, int kz,
                const int *skip) {
   if (skip && *skip) return;
-  mgd::UpOp<T, D> op{c, pitch, nx, ny, dinv, b, wm, wp, yc, cnx, cny, ncd, y, w};
+  mgd::UpOp<T, D> op{c, pitch, nx, ny, dinv, b, wm, wp, yc, cnx, cny, ncd, y, w, zm};
   const int k0 = blockIdx.z * kz;
   march_tile<T, kTileX, TY>(op, nx, ny, nz, k0, min(k0 + kz, nz), blockIdx.x, blockIdx.y);
 }
@@ -489,6 +492,15 @@ struct DLevel {
   bool own_dinv = true;
   void *wm = nullptr, *wp = nullptr;
   void *x = nullptr, *y = nullptr, *b = nullptr, *r = nullptr, *tmp = nullptr;
+  // What the fused down/up kernels read: hot copies of the coefficients, 1/diag
+  // and weights when the level is z-extruded (zm.pl != 0; xf flags, nexc of
+  // them), else aliases of the full arrays.  own_hc / own_hw: hc, hd / hwm, hwp
+  // allocated for the level.
+  void *hc = nullptr, *hd = nullptr, *hwm = nullptr, *hwp = nullptr;
+  unsigned char *xf = nullptr;
+  long long hpitch = 0, nexc = 0;
+  ZMap zm;
+  bool own_hc = false, own_hw = false;
 };
 
 struct PfmgImpl {
@@ -514,6 +526,15 @@ void free_levels(PfmgImpl *P) {
     if (l.own_dinv && l.dinv) (void)hipFree(l.dinv);
     for (void *q : {l.wm, l.wp, l.x, l.y, l.b, l.r, l.tmp})
       if (q) (void)hipFree(q);
+    if (l.own_hc) {
+      if (l.hc) (void)hipFree(l.hc);
+      if (l.hd) (void)hipFree(l.hd);
+    }
+    if (l.own_hw) {
+      if (l.hwm) (void)hipFree(l.hwm);
+      if (l.hwp) (void)hipFree(l.hwp);
+    }
+    if (l.xf) (void)hipFree(l.xf);
   }
   P->L.clear();
   if (P->inv) (void)hipFree(P->inv);
@@ -537,6 +558,87 @@ template <typename T>
 int alloc_t(void **dst, long long n) {
   FPC_HIP(hipMalloc(dst, (size_t)std::max(1LL, n) * sizeof(T)));
   FPC_HIP(hipMemsetAsync(*dst, 0, (size_t)std::max(1LL, n) * sizeof(T), stream()));
+  return 0;
+}
+
+template <typename T>
+__global__ void k_dinv_t(const T *__restrict__ c, T *__restrict__ d, long long n) {
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+    d[t] = (c[t] != (T)0) ? (T)1 / c[t] : (T)0;
+}
+
+// The arrays the fused kernels of level l read (DLevel::hc .. zm): a hot copy
+// (extrude.hpp) when the level's coefficients and weights repeat along z, else
+// aliases of the full arrays.  1/diag is derived as the full one is: on the
+// device from the coefficients at level 0 (as the matrix's), on the host at
+// the levels built here.
+template <typename T>
+int build_hot(const HLevel &h, size_t l, DLevel &dl) {
+  ExtrudePlan plan;
+  // The weights as the hot copy holds them.  Only F points read weights; when the
+  // level coarsens z its C planes (odd k) take the next F plane's weights so that
+  // the interior planes repeat (their own values are never read).
+  std::vector<double> wm, wp;
+  if (h.cdir >= 0) {
+    wm = h.wm;
+    wp = h.wp;
+    if (h.cdir == 2) {
+      const size_t pl = (size_t)h.n[0] * (size_t)h.n[1];
+      for (int k = 1; k < h.n[2]; k += 2) {
+        const int src = (k + 1 < h.n[2]) ? k + 1 : k - 1;
+        std::memcpy(wm.data() + (size_t)k * pl, wm.data() + (size_t)src * pl, pl * sizeof(double));
+        std::memcpy(wp.data() + (size_t)k * pl, wp.data() + (size_t)src * pl, pl * sizeof(double));
+      }
+    }
+  }
+  if (z_extrude_enabled() && h.n[2] >= 5) {
+    std::vector<const double *> arrays;
+    for (int q = 0; q < kSlots; ++q) arrays.push_back(h.a.data() + (size_t)q * (size_t)h.N);
+    if (h.cdir >= 0) {
+      arrays.push_back(wm.data());
+      arrays.push_back(wp.data());
+    }
+    plan = plan_extrusion(h.n[0], h.n[1], h.n[2], arrays);
+  }
+  if (!plan.m) {
+    dl.hc = dl.coef; dl.hd = dl.dinv; dl.hwm = dl.wm; dl.hwp = dl.wp;
+    dl.hpitch = dl.pitch;
+    return 0;
+  }
+  int rc;
+  const long long rows = plan.xoff + h.N;
+  dl.hpitch = pad256(rows, (int)sizeof(T));
+  std::vector<double> hot = hot_planes(h.a.data(), h.N, kSlots, h.n[0], h.n[1], h.n[2], plan, dl.hpitch);
+  dl.own_hc = true;
+  if ((rc = upload<T>(&dl.hc, hot, hot.size(), hot.size()))) return rc;
+  if (l == 0) {
+    FPC_HIP(hipMalloc(&dl.hd, (size_t)dl.hpitch * sizeof(T)));
+    k_dinv_t<T><<<(unsigned)std::min((dl.hpitch + 255) / 256, 65536LL), 256, 0, stream()>>>((const T *)dl.hc,
+                                                                                           (T *)dl.hd, dl.hpitch);
+    FPC_LAUNCH_CHECK();
+  } else {
+    std::vector<double> dv((size_t)dl.hpitch, 0.0);
+    for (size_t q = 0; q < dv.size(); ++q) dv[q] = (hot[q] != 0.0) ? 1.0 / hot[q] : 0.0;
+    if ((rc = upload<T>(&dl.hd, dv, dv.size(), dv.size()))) return rc;
+  }
+  if (h.cdir >= 0) {
+    dl.own_hw = true;
+    std::vector<double> hwm = hot_planes(wm.data(), h.N, 1, h.n[0], h.n[1], h.n[2], plan, rows);
+    std::vector<double> hwp = hot_planes(wp.data(), h.N, 1, h.n[0], h.n[1], h.n[2], plan, rows);
+    if ((rc = upload<T>(&dl.hwm, hwm, hwm.size(), hwm.size()))) return rc;
+    if ((rc = upload<T>(&dl.hwp, hwp, hwp.size(), hwp.size()))) return rc;
+  }
+  if (plan.nexc) {
+    FPC_HIP(hipMalloc(&dl.xf, (size_t)h.N));
+    FPC_HIP(hipMemcpyAsync(dl.xf, plan.xf.data(), (size_t)h.N, hipMemcpyHostToDevice, stream()));
+    FPC_HIP(hipStreamSynchronize(stream()));  // plan.xf is released on return
+  }
+  dl.nexc = plan.nexc;
+  dl.zm.pl = (long long)h.n[0] * h.n[1];
+  dl.zm.nz = h.n[2];
+  dl.zm.m = plan.m;
+  dl.zm.xoff = plan.xoff;
+  dl.zm.xf = dl.xf;
   return 0;
 }
 
@@ -585,7 +687,9 @@ int build_device(PfmgImpl *P, std::vector<HLevel> &H) {
     if ((rc = alloc_t<T>(&dl.tmp, h.N))) return rc;
     if (l > 0 || sizeof(T) != 8)
       if ((rc = alloc_t<T>(&dl.b, h.N))) return rc;
-    P->L.push_back(dl);
+    rc = build_hot<T>(h, l, dl);
+    P->L.push_back(dl);  // owned allocations are freed with the level, also on error
+    if (rc) return rc;
   }
   return 0;
 }
@@ -630,6 +734,25 @@ int build(PfmgImpl *P, fpc_StructMatrix_s *A) {
   return 0;
 }
 
+// What a fused kernel of level F reads: the hot arrays on z-marching launches of
+// an extruded level, else the full arrays (one-point launches are latency-bound,
+// and the flag a hot row may need would add a dependent load).
+template <typename T>
+struct HotView {
+  const T *c, *d, *wm, *wp;
+  long long pitch;
+  ZMap zm;
+  double rows;  // coefficient rows read
+};
+template <typename T>
+HotView<T> hot_view(const DLevel &F) {
+  if (F.zm.pl && tile_worthwhile(F.d))
+    return HotView<T>{(const T *)F.hc, (const T *)F.hd, (const T *)F.hwm, (const T *)F.hwp, F.hpitch, F.zm,
+                      stored_rows(F.zm, F.d.N, F.nexc)};
+  return HotView<T>{(const T *)F.coef, (const T *)F.dinv, (const T *)F.wm, (const T *)F.wp, F.pitch, ZMap(),
+                    (double)F.d.N};
+}
+
 // One V-cycle with zero initial guess: out = M^-1 in.
 template <typename T>
 int vcycle(PfmgImpl *P, const double *in, double *out, const int *skip) {
@@ -665,37 +788,42 @@ int vcycle(PfmgImpl *P, const double *in, double *out, const int *skip) {
     T *x = xof(l);
     const double sz = (double)sizeof(T);
     if (fused) {
-      // bytes per fine point: 7 coefficients, b, 1/diag, and r written -- or, when z is
-      // restricted on the fly, one weight at half the points and the coarse b instead of r
+      // The z-marching kernels read the level's hot arrays (when it is extruded); the
+      // one-point kernels of the latency-bound small levels the full ones.
+      const HotView<T> H = hot_view<T>(F);
+      const T *hc = H.c, *hd = H.d, *hwm = H.wm, *hwp = H.wp;
+      const double rows = H.rows;  // rows of the coefficient, 1/diag and weight planes read
+      // bytes: 7 coefficients and 1/diag as stored, b at every point, and r written -- or,
+      // when z is restricted on the fly, one weight plane and the coarse b instead of r
       const char *cls = (l == 0) ? "mg_down0" : "mg_down";
       if (tile_worthwhile(F.d) && F.cdir == 2) {
         TileLaunch tl = tile_launch<8>(F.d);
         FPC_PROF_BEGIN(cls);
-        k_down_restrict_z<T, 8><<<tl.grid, tl.block, 0, st>>>(c, F.pitch, F.d.nx, F.d.ny, F.d.nz, dv, bof(l),
-                                                               (const T *)F.wm, (const T *)F.wp, (T *)C.b, C.d.nz,
-                                                               w_of(l), tl.kz, skip);
-        FPC_PROF_END(cls, ((double)F.d.N * 10.0 + (double)C.d.N) * sz);
+        k_down_restrict_z<T, 8><<<tl.grid, tl.block, 0, st>>>(hc, H.pitch, H.zm, F.d.nx, F.d.ny, F.d.nz, hd, bof(l),
+                                                               hwm, hwp, (T *)C.b, C.d.nz, w_of(l), tl.kz, skip);
+        FPC_PROF_END(cls, (rows * 9.0 + (double)F.d.N + (double)C.d.N) * sz);
         FPC_LAUNCH_CHECK();
         continue;
       }
       FPC_PROF_BEGIN(cls);
       if (tile_worthwhile(F.d)) {
         TileLaunch tl = tile_launch<8>(F.d);
-        k_down_tiled<T, 8><<<tl.grid, tl.block, 0, st>>>(c, F.pitch, F.d.nx, F.d.ny, F.d.nz, dv, bof(l), (T *)F.r,
-                                                          w_of(l), tl.kz, skip);
+        k_down_tiled<T, 8><<<tl.grid, tl.block, 0, st>>>(hc, H.pitch, H.zm, F.d.nx, F.d.ny, F.d.nz, hd, bof(l),
+                                                          (T *)F.r, w_of(l), tl.kz, skip);
       } else {
-        k_down_jacobi<T><<<g, bl, 0, st>>>(c, F.pitch, F.d.nx, F.d.ny, F.d.nz, dv, bof(l), (T *)F.r, w_of(l), skip);
+        k_down_jacobi<T><<<g, bl, 0, st>>>(hc, H.pitch, H.zm, F.d.nx, F.d.ny, F.d.nz, hd, bof(l), (T *)F.r, w_of(l),
+                                           skip);
       }
-      FPC_PROF_END(cls, (double)F.d.N * 10.0 * sz);
+      FPC_PROF_END(cls, (rows * 8.0 + (double)F.d.N * 2.0) * sz);
       FPC_LAUNCH_CHECK();
       dim3 gc, bc;
       launch_shape(C.d, gc, bc);
       {
         // bytes: r at every fine point, the weights at the F points, b of each coarse point
         FPC_PROF_BEGIN("mg_restrict");
-        k_restrict<T><<<gc, bc, 0, st>>>(F.d.nx, F.d.ny, F.d.nz, F.cdir, (const T *)F.r, (const T *)F.wm,
-                                        (const T *)F.wp, C.d.nx, C.d.ny, C.d.nz, (T *)C.b, skip);
-        FPC_PROF_END("mg_restrict", ((double)F.d.N * 2.0 + (double)C.d.N) * sz);
+        k_restrict<T><<<gc, bc, 0, st>>>(F.d.nx, F.d.ny, F.d.nz, F.cdir, (const T *)F.r, hwm, hwp, H.zm, C.d.nx,
+                                        C.d.ny, C.d.nz, (T *)C.b, skip);
+        FPC_PROF_END("mg_restrict", ((double)F.d.N + rows + (double)C.d.N) * sz);
       }
       FPC_LAUNCH_CHECK();
       continue;
@@ -718,7 +846,7 @@ int vcycle(PfmgImpl *P, const double *in, double *out, const int *skip) {
     dim3 gc, bc;
     launch_shape(C.d, gc, bc);
     k_restrict<T><<<gc, bc, 0, st>>>(F.d.nx, F.d.ny, F.d.nz, F.cdir, (const T *)F.r, (const T *)F.wm, (const T *)F.wp,
-                                    C.d.nx, C.d.ny, C.d.nz, (T *)C.b, skip);
+                                    ZMap(), C.d.nx, C.d.ny, C.d.nz, (T *)C.b, skip);
     FPC_LAUNCH_CHECK();
   }
   {
@@ -735,26 +863,28 @@ int vcycle(PfmgImpl *P, const double *in, double *out, const int *skip) {
     const T *dv = (const T *)F.dinv;
     T *x = xof(l);
     if (fused) {
-      // bytes per fine point: 7 coefficients, b, 1/diag, y written, a weight pair at
-      // the F points (half of them); plus the coarse correction once
+      // bytes: 7 coefficients, 1/diag and a weight pair as stored (the pair is read at
+      // the F points, half of them), b at every point, y written; the coarse correction once
       const char *cls = (l == 0) ? "mg_up0" : "mg_up";
-      const double bytes = ((double)F.d.N * 11.0 + (double)C.d.N) * (double)sizeof(T);
+      const HotView<T> H = hot_view<T>(F);
+      const double bytes = (H.rows * 9.0 + (double)F.d.N * 2.0 + (double)C.d.N) * (double)sizeof(T);
       FPC_PROF_BEGIN(cls);
       const int ncd = (F.cdir == 0) ? C.d.nx : (F.cdir == 1 ? C.d.ny : C.d.nz);
-      const T *wm = (const T *)F.wm, *wp = (const T *)F.wp, *yc = (const T *)yof(l + 1);
+      const T *hc = H.c, *hd = H.d;
+      const T *wm = H.wm, *wp = H.wp, *yc = (const T *)yof(l + 1);
       if (tile_worthwhile(F.d)) {
         TileLaunch tl = tile_launch<8>(F.d);
-#define FPC_UP_TILED(D)                                                                                     \
-  k_up_tiled<T, D, 8><<<tl.grid, tl.block, 0, st>>>(c, F.pitch, F.d.nx, F.d.ny, F.d.nz, dv, bof(l), wm, wp, yc, \
-                                                    C.d.nx, C.d.ny, ncd, yof(l), w_of(l), tl.kz, skip)
+#define FPC_UP_TILED(D)                                                                                          \
+  k_up_tiled<T, D, 8><<<tl.grid, tl.block, 0, st>>>(hc, H.pitch, H.zm, F.d.nx, F.d.ny, F.d.nz, hd, bof(l), wm, wp, \
+                                                    yc, C.d.nx, C.d.ny, ncd, yof(l), w_of(l), tl.kz, skip)
         if (F.cdir == 0) FPC_UP_TILED(0);
         else if (F.cdir == 1) FPC_UP_TILED(1);
         else FPC_UP_TILED(2);
 #undef FPC_UP_TILED
       } else {
-#define FPC_UP_SIMPLE(D)                                                                                       \
-  k_up_jacobi<T, D><<<g, bl, 0, st>>>(c, F.pitch, F.d.nx, F.d.ny, F.d.nz, dv, bof(l), wm, wp, yc, C.d.nx, C.d.ny, \
-                                      ncd, yof(l), w_of(l), skip)
+#define FPC_UP_SIMPLE(D)                                                                                        \
+  k_up_jacobi<T, D><<<g, bl, 0, st>>>(hc, H.pitch, H.zm, F.d.nx, F.d.ny, F.d.nz, hd, bof(l), wm, wp, yc, C.d.nx, \
+                                      C.d.ny, ncd, yof(l), w_of(l), skip)
         if (F.cdir == 0) FPC_UP_SIMPLE(0);
         else if (F.cdir == 1) FPC_UP_SIMPLE(1);
         else FPC_UP_SIMPLE(2);
@@ -830,11 +960,17 @@ int pfmg_batch_view(void *pf, std::vector<MgLevelView> &levels, const double **i
     v.N = L.d.N;
     v.cdir = L.cdir;
     v.w = (P->relax == 0) ? 1.0 : L.omega;
-    v.pitch = L.pitch;
-    v.coef = (const double *)L.coef;
-    v.dinv = (const double *)L.dinv;
-    v.wm = (const double *)L.wm;
-    v.wp = (const double *)L.wp;
+    v.pitch = L.hpitch;
+    v.zm = L.zm;
+    v.coef = (const double *)L.hc;
+    v.dinv = (const double *)L.hd;
+    v.wm = (const double *)L.hwm;
+    v.wp = (const double *)L.hwp;
+    v.fpitch = L.pitch;
+    v.fcoef = (const double *)L.coef;
+    v.fdinv = (const double *)L.dinv;
+    v.fwm = (const double *)L.wm;
+    v.fwp = (const double *)L.wp;
     v.b = (double *)L.b;
     v.r = (double *)L.r;
     v.y = (double *)L.y;
@@ -987,6 +1123,15 @@ int fpc_StructPFMGGetNumLevels(fpc_StructSolver s, int *n) {
   PIMPL(s);
   FPC_CHECK_ARG(n, "null pointer");
   *n = (int)P->L.size();
+  return 0;
+}
+
+int fpc_StructPFMGGetLevelExtruded(fpc_StructSolver s, int level, int *wall_planes, long long *flagged) {
+  PIMPL(s);
+  FPC_CHECK_ARG(wall_planes && flagged && level >= 0 && level < (int)P->L.size(), "bad level");
+  const DLevel &L = P->L[(size_t)level];
+  *wall_planes = L.zm.pl ? L.zm.m : 0;
+  *flagged = L.nexc;
   return 0;
 }
 

@@ -139,6 +139,7 @@ struct TileLaunch {
 struct TileTuning {
   long long min_points = -1;
   int kz = -1;
+  int z_extrude = 1;  // three-plane copies of z-extruded operators
 };
 inline TileTuning &tile_tuning() {
   static TileTuning t = [] {
@@ -147,6 +148,8 @@ inline TileTuning &tile_tuning() {
     x.min_points = e ? std::max(0LL, std::atoll(e)) : (1LL << 20);
     e = std::getenv("FPC_STENCIL_KZ");
     x.kz = e ? std::max(0, std::atoi(e)) : 0;
+    e = std::getenv("FPC_Z_EXTRUDE");
+    x.z_extrude = e ? (std::atoi(e) != 0) : 1;
     return x;
   }();
   return t;
@@ -155,6 +158,16 @@ inline TileTuning &tile_tuning() {
 // Smaller boxes run the one-point-per-thread kernels.
 inline bool tile_worthwhile(const Dims &d) {
   return d.nx >= 32 && d.ny >= 8 && d.N >= std::max(tile_tuning().min_points, 1LL << 15);
+}
+
+// Algorithmic bytes of one stencil pass over A: per_point bytes of vectors at
+// every point (plus the flag byte of a hot copy), and the 7 coefficient planes
+// and a dinv_bytes-wide 1/diag plane (0: none) over the rows read.
+// hot: the pass reads the hot copy when A has one (tiled launches), else the full planes.
+inline double stencil_bytes(const fpc_StructMatrix_s *A, int dinv_bytes, double per_point, bool hot) {
+  const bool z = hot && A->zmap.pl && tile_worthwhile(dims_of(A->grid));
+  const double rows = z ? stored_rows(A->zmap, A->grid.N, A->zexc) : (double)A->grid.N;
+  return (double)A->grid.N * (per_point + (z ? 1.0 : 0.0)) + rows * (8.0 * kSlots + dinv_bytes);
 }
 
 // z-chunk length: about 2048 workgroups in flight, chunks of at least 4 planes,

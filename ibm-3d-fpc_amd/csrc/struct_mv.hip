@@ -13,6 +13,7 @@
 #include <atomic>
 #include <mutex>
 
+#include "extrude.hpp"
 #include "fpc_internal.hpp"
 #include "krylov_device.hpp"
 #include "stencil_tile.hpp"
@@ -240,10 +241,10 @@ __global__ void k_matvec(const double *__restrict__ c, long long pitch, int nx, 
 }
 
 // r = b - A x with per-block partial sums of r^2.
-__global__ void __launch_bounds__(256) k_residual(const double *__restrict__ c, long long pitch, int nx, int ny, int nz,
-                                                  const double *__restrict__ b, const double *__restrict__ x,
+__global__ void __launch_bounds__(256) k_residual(const double *__restrict__ c, long long pitch, ZMap zm, int nx, int ny,
+                                                  int nz, const double *__restrict__ b, const double *__restrict__ x,
                                                   double *__restrict__ r, double *__restrict__ part) {
-  kry::residual_body(c, pitch, nx, ny, nz, b, x, r, part, blockIdx.x, gridDim.x);
+  kry::residual_body(c, pitch, zm, nx, ny, nz, b, x, r, part, blockIdx.x, gridDim.x);
 }
 
 __global__ void __launch_bounds__(256) k_dot(const double *__restrict__ a, const double *__restrict__ b, long long n,
@@ -297,7 +298,9 @@ int matvec(double alpha, const fpc_StructMatrix_s *A, const double *x, double be
 
 int residual_norm_partials(const fpc_StructMatrix_s *A, const double *b, const double *x, double *r, double *part) {
   Dims d = dims_of(A->grid);
-  k_residual<<<red_blocks(d.N), 256, 0, stream()>>>(A->coef, A->pitch, d.nx, d.ny, d.nz, b, x, r, part);
+  // the hot copy on boxes the z-marching kernels take; latency-bound small boxes the full planes
+  const Planes<double> P = tile_worthwhile(d) ? planes64(A) : Planes<double>{A->coef, A->dinv, A->pitch, ZMap()};
+  k_residual<<<red_blocks(d.N), 256, 0, stream()>>>(P.c, P.pitch, P.zm, d.nx, d.ny, d.nz, b, x, r, part);
   FPC_LAUNCH_CHECK();
   return 0;
 }
@@ -314,12 +317,64 @@ int reduce_partials(const double *part, int nparts, double *out) {
   return 0;
 }
 
+bool z_extrude_enabled() { return tile_tuning().z_extrude != 0; }
+
+namespace {
+
+// FP32 hot copies of an extruded operator, once it has FP32 planes: the same
+// conversions as coef32 / dinv32, over the hot layout.
+int pack_extruded32(fpc_StructMatrix_s *A) {
+  if (!A->zmap.pl || !A->coef32 || A->zcoef32) return 0;
+  FPC_HIP(hipMalloc(&A->zcoef32, (size_t)kSlots * A->zpitch * sizeof(float)));
+  FPC_HIP(hipMalloc(&A->zdinv32, (size_t)A->zpitch * sizeof(float)));
+  const long long n = (long long)kSlots * A->zpitch;
+  k_to_fp32<<<blocks_for(n), 256, 0, stream()>>>(A->zcoef, A->zcoef32, n);
+  FPC_LAUNCH_CHECK();
+  k_dinv<float><<<blocks_for(A->zpitch), 256, 0, stream()>>>(A->zcoef32, A->zdinv32, A->zpitch);
+  FPC_LAUNCH_CHECK();
+  return 0;
+}
+
+// Plans the hot copy of a z-extruded operator on a host copy of its planes
+// (extrude.hpp) and uploads it; 1/diag is derived on the device as dinv is.
+int make_extruded(fpc_StructMatrix_s *A) {
+  const int nx = A->grid.n[0], ny = A->grid.n[1], nz = A->grid.n[2];
+  const long long N = A->grid.N;
+  if (!z_extrude_enabled() || nz < 5 || N <= 0) return 0;
+  std::vector<double> h((size_t)kSlots * (size_t)A->pitch);
+  FPC_HIP(hipMemcpyAsync(h.data(), A->coef, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream()));
+  FPC_HIP(hipStreamSynchronize(stream()));
+  std::vector<const double *> slots;
+  for (int q = 0; q < kSlots; ++q) slots.push_back(h.data() + (size_t)q * (size_t)A->pitch);
+  ExtrudePlan plan = plan_extrusion(nx, ny, nz, slots);
+  if (!plan.m) return 0;
+  A->zpitch = pad256(plan.xoff + N, (int)sizeof(double));
+  std::vector<double> hot = hot_planes(h.data(), A->pitch, kSlots, nx, ny, nz, plan, A->zpitch);
+  FPC_HIP(hipMalloc(&A->zcoef, hot.size() * sizeof(double)));
+  FPC_HIP(hipMalloc(&A->zdinv, (size_t)A->zpitch * sizeof(double)));
+  FPC_HIP(hipMalloc(&A->zxf, (size_t)N));
+  FPC_HIP(hipMemcpyAsync(A->zcoef, hot.data(), hot.size() * sizeof(double), hipMemcpyHostToDevice, stream()));
+  FPC_HIP(hipMemcpyAsync(A->zxf, plan.xf.data(), (size_t)N, hipMemcpyHostToDevice, stream()));
+  k_dinv<double><<<blocks_for(A->zpitch), 256, 0, stream()>>>(A->zcoef, A->zdinv, A->zpitch);
+  FPC_LAUNCH_CHECK();
+  FPC_HIP(hipStreamSynchronize(stream()));  // hot and plan.xf are released on return
+  A->zexc = plan.nexc;
+  A->zmap.pl = (long long)nx * ny;
+  A->zmap.nz = nz;
+  A->zmap.m = plan.m;
+  A->zmap.xoff = plan.xoff;
+  A->zmap.xf = plan.nexc ? A->zxf : nullptr;
+  return pack_extruded32(A);
+}
+
+}  // namespace
+
 int ensure_dinv(fpc_StructMatrix_s *A) {
   if (A->dinv) return 0;
   FPC_HIP(hipMalloc(&A->dinv, (size_t)A->pitch * sizeof(double)));
   k_dinv<double><<<blocks_for(A->grid.N), 256, 0, stream()>>>(A->coef, A->dinv, A->grid.N);
   FPC_LAUNCH_CHECK();
-  return 0;
+  return make_extruded(A);
 }
 
 int ensure_fp32(fpc_StructMatrix_s *A) {
@@ -331,16 +386,21 @@ int ensure_fp32(fpc_StructMatrix_s *A) {
   FPC_LAUNCH_CHECK();
   k_dinv<float><<<blocks_for(A->grid.N), 256, 0, stream()>>>(A->coef32, A->dinv32, A->grid.N);
   FPC_LAUNCH_CHECK();
-  return 0;
+  return pack_extruded32(A);
 }
 
 static void drop_derived(fpc_StructMatrix_s *A) {
-  if (A->dinv) (void)hipFree(A->dinv);
-  if (A->coef32) (void)hipFree(A->coef32);
-  if (A->dinv32) (void)hipFree(A->dinv32);
+  for (void *p : {(void *)A->dinv, (void *)A->coef32, (void *)A->dinv32, (void *)A->zcoef, (void *)A->zdinv,
+                  (void *)A->zcoef32, (void *)A->zdinv32, (void *)A->zxf})
+    if (p) (void)hipFree(p);
   A->dinv = nullptr;
   A->coef32 = nullptr;
   A->dinv32 = nullptr;
+  A->zcoef = A->zdinv = nullptr;
+  A->zcoef32 = A->zdinv32 = nullptr;
+  A->zxf = nullptr;
+  A->zpitch = A->zexc = 0;
+  A->zmap = ZMap();
 }
 
 }  // namespace fpc
@@ -553,6 +613,13 @@ int fpc_StructMatrixMatvec(double alpha, fpc_StructMatrix A, fpc_StructVector x,
   int rc = matvec(alpha, A, x->data, beta, y->data);
   if (rc) return rc;
   FPC_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+int fpc_StructMatrixGetExtruded(fpc_StructMatrix A, int *wall_planes, long long *flagged) {
+  FPC_CHECK_ARG(A && wall_planes && flagged, "null argument");
+  *wall_planes = A->zmap.pl ? A->zmap.m : 0;
+  *flagged = A->zexc;
   return 0;
 }
 
@@ -771,6 +838,9 @@ int fpc_SetTuning(const char *key, long long value) {
   } else if (k == "stencil_kz") {
     FPC_CHECK_ARG(value >= 0 && value <= 1024, "stencil_kz must be in [0, 1024]");
     t.kz = (int)value;
+  } else if (k == "z_extrude") {
+    FPC_CHECK_ARG(value == 0 || value == 1, "z_extrude must be 0 or 1");
+    t.z_extrude = (int)value;
   } else {
     FPC_CHECK_ARG(false, "unknown tuning key");
   }

@@ -109,6 +109,8 @@ SIGNATURES = {
     "fpc_StructPFMGSetPrintLevel": (_i, [_vp, _i]), "fpc_StructPFMGSetZeroGuess": (_i, [_vp]),
     "fpc_StructPFMGSetNonZeroGuess": (_i, [_vp]), "fpc_StructPFMGSetPrecision": (_i, [_vp, _i]),
     "fpc_StructPFMGSetCoarseSize": (_i, [_vp, _i]),
+    "fpc_StructMatrixGetExtruded": (_i, [_vp, _pi, ctypes.POINTER(ctypes.c_longlong)]),
+    "fpc_StructPFMGGetLevelExtruded": (_i, [_vp, _i, _pi, ctypes.POINTER(ctypes.c_longlong)]),
     "fpc_StructPFMGSetup": (_i, [_vp, _vp, _vp, _vp]), "fpc_StructPFMGSolve": (_i, [_vp, _vp, _vp, _vp]),
     "fpc_StructPFMGGetNumIterations": (_i, [_vp, _pi]), "fpc_StructPFMGGetNumLevels": (_i, [_vp, _pi]),
     "fpc_StructPFMGGetLevelInfo": (_i, [_vp, _i, _pi, _pi, _pd]), "fpc_StructPFMGDestroy": (_i, [_vp]),
@@ -490,6 +492,12 @@ class StructBox:
             v.set(values)
         return v
 
+    def extruded(self):
+        """(wall planes m, flagged points) of the copy the solve loops read; m = 0: full planes."""
+        m, f = ctypes.c_int(), ctypes.c_longlong()
+        check(lib().fpc_StructMatrixGetExtruded(self.A, ctypes.byref(m), ctypes.byref(f)), "GetExtruded")
+        return m.value, f.value
+
     def matvec(self, x, alpha=1.0, beta=0.0, y=None):
         xv = self.vector(x)
         yv = self.vector(y if y is not None else np.zeros(self.N))
@@ -601,6 +609,15 @@ class Solver:
             w = ctypes.c_double()
             check(L.fpc_StructPFMGGetLevelInfo(self.pc, l, dims, ctypes.byref(cd), ctypes.byref(w)), "level info")
             out.append((tuple(dims), cd.value, w.value))
+        return out
+
+    def levels_extruded(self):
+        """Per MG level: (wall planes m, flagged points) of what its fused kernels read."""
+        out = []
+        for l in range(len(self.levels())):
+            m, f = ctypes.c_int(), ctypes.c_longlong()
+            check(lib().fpc_StructPFMGGetLevelExtruded(self.pc, l, ctypes.byref(m), ctypes.byref(f)), "level extruded")
+            out.append((m.value, f.value))
         return out
 
     def level_matrix(self, level):

@@ -87,6 +87,12 @@ int fpc_StructMatrixGetBoxValues(fpc_StructMatrix A, const int *ilower, const in
 int fpc_StructMatrixAssemble(fpc_StructMatrix A);
 /* y = alpha A x + beta y (HYPRE_StructMatrixMatvec). */
 int fpc_StructMatrixMatvec(double alpha, fpc_StructMatrix A, fpc_StructVector x, double beta, fpc_StructVector y);
+/* Storage the solve loops read for an assembled operator: *wall_planes = m > 0
+ * when its z-planes away from the walls repeat and they read a copy of m planes
+ * per wall plus one interior plane, *flagged = points whose row differs from
+ * that plane and is read in full; *wall_planes = 0: the full planes.
+ * No reference counterpart (storage inspection). */
+int fpc_StructMatrixGetExtruded(fpc_StructMatrix A, int *wall_planes, long long *flagged);
 int fpc_StructMatrixDestroy(fpc_StructMatrix A);
 
 /* ---- vector (HYPRE_StructVector*, hypre_struct_utils.hpp:21-26) ---- */
@@ -172,6 +178,9 @@ int fpc_StructPFMGGetLevelInfo(fpc_StructSolver solver, int level, int *dims, in
 /* Level l's 7-point operator as stored on the device, AoS (7 per point, entry order
  * C,-x,+x,-y,+y,-z,+z), FP32 levels widened; values holds 7 * points of the level. */
 int fpc_StructPFMGGetLevelMatrix(fpc_StructSolver solver, int level, double *values);
+/* The same for level l's fused smoothing kernels (coefficients, 1/diag and
+ * interpolation weights).  No reference counterpart. */
+int fpc_StructPFMGGetLevelExtruded(fpc_StructSolver solver, int level, int *wall_planes, long long *flagged);
 int fpc_StructPFMGDestroy(fpc_StructSolver solver);
 
 /* ---- helpers with the reference signatures (hypre_struct_utils.hpp:1-26) ---- */
@@ -205,7 +214,10 @@ int fpc_DeviceSynchronize(void);
  * "tile_min_points": boxes with at least this many points (and nx >= 32,
  * ny >= 8) use the z-marching stencil kernels (default 1048576, or the
  * FPC_TILE_MIN_POINTS environment variable); "stencil_kz": z-planes per
- * workgroup of those kernels, 0 = automatic (FPC_STENCIL_KZ). */
+ * workgroup of those kernels, 0 = automatic (FPC_STENCIL_KZ); "z_extrude":
+ * 1 (default) = operators and MG levels whose z-planes 1..nz-2 are equal keep a
+ * three-plane copy that the solve loops read, 0 = off (FPC_Z_EXTRUDE); applies
+ * to operators assembled and solvers set up afterwards, with identical results. */
 int fpc_SetTuning(const char *key, long long value);
 
 /* ---- kernel timing (no reference counterpart) ----
