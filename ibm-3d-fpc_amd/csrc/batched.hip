@@ -349,7 +349,7 @@ __global__ void __launch_bounds__(kTileX *kTY) k_bt_down(const BLev *lev, const 
   if (L.cdir == 2) {
     // residual restricted in z on the fly (no separate restriction)
     mgd::DownRestrictZOp<double> op;
-    op.c = L.c; op.pitch = L.pitch; op.nx = L.nx; op.ny = L.ny; op.dinv = L.dinv; op.b = b; op.r = nullptr;
+    op.c = L.c; op.pitch = L.pitch; op.nx = L.nx; op.ny = L.ny; op.dinv = L.dinv; op.b = b;
     op.w = L.w; op.wm = L.wm; op.wp = L.wp; op.bc = L.bc; op.nz = L.nz; op.ncz = L.cnz; op.k0 = e.k0;
     op.zm = L.zm;
     march_tile<double, kTileX, kTY>(op, L.nx, L.ny, L.nz, e.k0, e.k1 < L.nz ? e.k1 + 1 : e.k1, bx, by);

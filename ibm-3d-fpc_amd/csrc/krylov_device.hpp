@@ -312,22 +312,17 @@ struct ArnoldiOp {
   ZMap zm;  // rows of c and dinv
   struct Raw {
     double z, dinv;
-  };
-  struct Row {
-    Coef7<double> a;
+    long long u;  // row of the point's coefficients and 1/diag
   };
   __device__ long long at(int i, int j, int k) const { return ((long long)k * ny + j) * nx + i; }
   __device__ Raw fetch(int i, int j, int k) const {
-    long long t = at(i, j, k);
-    return Raw{z[t], kDinv ? (double)dinv[zm(t, k)] : 1.0};
+    const long long t = at(i, j, k), u = zm(t, k);
+    return Raw{z[t], kDinv ? (double)dinv[u] : 1.0, u};
   }
   __device__ double finish(const Raw &r) const { return kDinv ? r.z * r.dinv : r.z; }
-  __device__ Row row(int i, int j, int k) const {
-    Row q;
-    q.a.load(c, pitch, zm(at(i, j, k), k));
-    return q;
+  __device__ void out(const Coef7<double> &a, const Raw &, int i, int j, int k, const Nbr<double> &nb) {
+    w[at(i, j, k)] = sg * a.apply(nb);
   }
-  __device__ void out(const Row &q, int i, int j, int k, const Nbr<double> &nb) { w[at(i, j, k)] = sg * q.a.apply(nb); }
 };
 
 }  // namespace kry

@@ -131,50 +131,59 @@ struct DownOp {
   ZMap zm;  // rows of c and dinv
   struct Raw {
     T b, dinv;
-  };
-  struct Row {
-    Coef7<T> a;
-    T b;
+    long long u;
   };
   __device__ long long at(int i, int j, int k) const { return ((long long)k * ny + j) * nx + i; }
   __device__ Raw fetch(int i, int j, int k) const {
-    long long t = at(i, j, k);
-    return Raw{b[t], dinv[zm(t, k)]};
+    const long long t = at(i, j, k), u = zm(t, k);
+    return Raw{b[t], dinv[u], u};
   }
   __device__ T finish(const Raw &q) const { return w * q.b * q.dinv; }
-  __device__ Row row(int i, int j, int k) const {
-    long long t = at(i, j, k);
-    Row q;
-    q.a.load(c, pitch, zm(t, k));
-    q.b = b[t];
-    return q;
+  __device__ void out(const Coef7<T> &a, const Raw &own, int i, int j, int k, const Nbr<T> &nb) {
+    r[at(i, j, k)] = own.b - a.apply(nb);
   }
-  __device__ void out(const Row &q, int i, int j, int k, const Nbr<T> &nb) { r[at(i, j, k)] = q.b - q.a.apply(nb); }
 };
 
 // The same, restricted on the fly when the level coarsens z: coarse plane kc
 // (fine C plane 2kc+1) = r(C) + wp(F-) r(F-) + wm(F+) r(F+), in that order; the
 // fine residual is never stored.  Chunks start on even planes and run one plane
-// past their end so the F+ plane of their last coarse plane is seen.
+// past their end so the F+ plane of their last coarse plane is seen.  The weights
+// of a point come with its fetch, ahead of use.
 template <typename T>
-struct DownRestrictZOp : DownOp<T> {
+struct DownRestrictZOp {
+  const T *c;
+  long long pitch;
+  int nx, ny;
+  const T *dinv, *b;
+  T w;
+  ZMap zm;  // rows of c, dinv, wm and wp
   const T *wm, *wp;
   T *bc;
   int nz, ncz, k0;
-  T r_prev = T(0), pend = T(0);
-  __device__ void out(const typename DownOp<T>::Row &q, int i, int j, int k, const Nbr<T> &nb) {
-    const long long t = this->at(i, j, k), pl = (long long)this->nx * this->ny;
-    const T rk = q.b - q.a.apply(nb);
+  T r_prev = T(0), wp_prev = T(0), pend = T(0);  // the plane below: residual, its wp
+  struct Raw {
+    T b, dinv, wm, wp;
+    long long u;
+  };
+  __device__ long long at(int i, int j, int k) const { return ((long long)k * ny + j) * nx + i; }
+  __device__ Raw fetch(int i, int j, int k) const {
+    const long long t = at(i, j, k), u = zm(t, k);
+    return Raw{b[t], dinv[u], wm[u], wp[u], u};
+  }
+  __device__ T finish(const Raw &q) const { return w * q.b * q.dinv; }
+  __device__ void out(const Coef7<T> &a, const Raw &own, int i, int j, int k, const Nbr<T> &nb) {
+    const T rk = own.b - a.apply(nb);
     if (k & 1) {
-      const T v = rk + wp[this->zm(t - pl, k - 1)] * r_prev;
+      const T v = rk + wp_prev * r_prev;
       const int kc = (k - 1) >> 1;
       if (k + 1 < nz) pend = v;
-      else if (kc < ncz) bc[((long long)kc * this->ny + j) * this->nx + i] = v;
+      else if (kc < ncz) bc[((long long)kc * ny + j) * nx + i] = v;
     } else if (k > k0) {
       const int kc = (k >> 1) - 1;
-      if (kc < ncz) bc[((long long)kc * this->ny + j) * this->nx + i] = pend + wm[this->zm(t, k)] * rk;
+      if (kc < ncz) bc[((long long)kc * ny + j) * nx + i] = pend + own.wm * rk;
     }
     r_prev = rk;
+    wp_prev = own.wp;
   }
 };
 
@@ -195,10 +204,7 @@ struct UpOp {
   struct Raw {
     T b, dinv, wm, wp, e0, e1;
     bool f;
-  };
-  struct Row {
-    Coef7<T> a;
-    T b, dinv;
+    long long u;
   };
   __device__ long long at(int i, int j, int k) const { return ((long long)k * ny + j) * nx + i; }
   __device__ Raw fetch(int i, int j, int k) const {
@@ -210,6 +216,7 @@ struct UpOp {
     const long long cr = ((long long)ck * cny + cj) * cnx + ci;
     const long long u = zm(t, k);
     Raw r;
+    r.u = u;
     r.b = b[t];
     r.dinv = dinv[u];
     r.f = !(pd & 1);
@@ -237,16 +244,8 @@ struct UpOp {
     }
     return x;
   }
-  __device__ Row row(int i, int j, int k) const {
-    long long t = at(i, j, k), u = zm(t, k);
-    Row q;
-    q.a.load(c, pitch, u);
-    q.b = b[t];
-    q.dinv = dinv[u];
-    return q;
-  }
-  __device__ void out(const Row &q, int i, int j, int k, const Nbr<T> &nb) {
-    y[at(i, j, k)] = nb.c + w * (q.b - q.a.apply(nb)) * q.dinv;
+  __device__ void out(const Coef7<T> &a, const Raw &own, int i, int j, int k, const Nbr<T> &nb) {
+    y[at(i, j, k)] = nb.c + w * (own.b - a.apply(nb)) * own.dinv;
   }
 };
 

@@ -432,7 +432,7 @@ __global__ void __launch_bounds__(kTileX *TY)
                       const T *__restrict__ wp, T *__restrict__ bc, int ncz, T w, int kz, const int *skip) {
   if (skip && *skip) return;
   mgd::DownRestrictZOp<T> op;
-  op.c = c; op.pitch = pitch; op.nx = nx; op.ny = ny; op.dinv = dinv; op.b = b; op.r = nullptr; op.w = w;
+  op.c = c; op.pitch = pitch; op.nx = nx; op.ny = ny; op.dinv = dinv; op.b = b; op.w = w;
   op.zm = zm;
   op.wm = wm; op.wp = wp; op.bc = bc; op.nz = nz; op.ncz = ncz;
   const int k0 = blockIdx.z * kz, k1 = min(k0 + kz, nz);

@@ -54,61 +54,10 @@ __device__ inline unsigned nbr_in_box(int i, int j, int k, int nx, int ny, int n
          ((unsigned)(k > 0) << 4) | ((unsigned)(k < nz - 1) << 5);
 }
 
-// Walk planes k0 .. kend-1 of tile (bx, by), owned by this workgroup.  The op splits
-// its point function in two so loads run ahead of use: op.fetch(i, j, k) issues
-// the loads (an Op::Raw) and op.finish(raw) does the arithmetic; op.row(i, j, k)
-// loads the point's own data (an Op::Row) and op.out(row, i, j, k, nb) consumes
-// the neighbour values.  The own row of plane k is loaded at the top of its
-// iteration, and the halo ring of plane k+1 and the own column at k+2 are
-// issued then too; a barrier waits only for LDS (s_waitcnt lgkmcnt), so all of
-// them stay in flight across it.  (Loading the row a plane ahead as well costs
-// 16-22 more VGPRs and one wave per SIMD of occupancy, and measured slower.)
-// Points outside the box read as zero.
-template <typename T, int TX, int TY, class Op>
-__device__ inline void march_tile(Op &op, int nx, int ny, int nz, int k0, int kend, int bx, int by) {
-  using Raw = typename Op::Raw;
-  using Row = typename Op::Row;
-  __shared__ T plane[2][TY + 2][TX + 2];
-  const int tx = threadIdx.x, ty = threadIdx.y;
-  const int i = bx * TX + tx, j = by * TY + ty;
-  const bool in = (i < nx) && (j < ny);
-  int hx = 0, hy = 0;
-  const bool ring = tile_ring<TX, TY>(ty * TX + tx, hx, hy);
-  const int hi = bx * TX + hx, hj = by * TY + hy;
-  const bool ring_in = ring && hi >= 0 && hi < nx && hj >= 0 && hj < ny;
-  T zm = (in && k0 > 0) ? op.finish(op.fetch(i, j, k0 - 1)) : T(0);
-  T zc = (in && k0 < nz) ? op.finish(op.fetch(i, j, k0)) : T(0);
-  Raw own1{}, ring0{};
-  if (in && k0 + 1 < nz) own1 = op.fetch(i, j, k0 + 1);
-  if (ring_in && k0 < kend) ring0 = op.fetch(hi, hj, k0);
-  for (int k = k0; k < kend; ++k) {
-    Raw own2{}, ring1{};
-    Row row{};
-    if (in) row = op.row(i, j, k);
-    if (in && k + 2 < nz && k + 1 < kend) own2 = op.fetch(i, j, k + 2);
-    if (ring_in && k + 1 < kend) ring1 = op.fetch(hi, hj, k + 1);
-    const T zp = (in && k + 1 < nz) ? op.finish(own1) : T(0);
-    T(*P)[TX + 2] = plane[(k - k0) & 1];
-    P[ty + 1][tx + 1] = zc;
-    if (ring) P[hy + 1][hx + 1] = ring_in ? op.finish(ring0) : T(0);
-    __syncthreads();
-    if (in) {
-      Nbr<T> nb{zc, P[ty + 1][tx], P[ty + 1][tx + 2], P[ty][tx + 1], P[ty + 2][tx + 1], zm, zp,
-                nbr_in_box(i, j, k, nx, ny, nz)};
-      op.out(row, i, j, k, nb);
-    }
-    zm = zc;
-    zc = zp;
-    own1 = own2;
-    ring0 = ring1;
-  }
-}
-
-// The seven coefficients of a row, loaded ahead of the plane barrier; apply()
-// sums in slot order C, -x, +x, -y, +y, -z, +z over the neighbours in the box.
-// Terms of absent neighbours are skipped rather than added as a*0, so the sum
-// is bit for bit that of the one-point-per-thread kernels (a*0 would turn a -0
-// partial sum into +0).
+// The seven coefficients of a row; apply() sums in slot order C, -x, +x, -y,
+// +y, -z, +z over the neighbours in the box.  Terms of absent neighbours are
+// skipped rather than added as a*0, so the sum is bit for bit that of the
+// one-point-per-thread kernels (a*0 would turn a -0 partial sum into +0).
 template <typename T>
 struct Coef7 {
   T a[7];
@@ -127,6 +76,63 @@ struct Coef7 {
     return s;
   }
 };
+
+// Walk planes k0 .. kend-1 of tile (bx, by), owned by this workgroup.  The op
+// splits its point function so loads run ahead of use: op.fetch(i, j, k) issues
+// a point's loads (an Op::Raw, with Raw::u the row of its coefficients) and
+// op.finish(raw) gives the value its neighbours see; op.out(a, own, i, j, k, nb)
+// writes the point from its coefficients a, its own raw and the neighbour values.
+// The own column is fetched two planes ahead and the halo ring one plane ahead;
+// a barrier waits only for LDS (s_waitcnt lgkmcnt), so those loads stay in flight
+// across it.  The coefficients stay in registers while the row does not change
+// -- the interior planes of a z-extruded operator's hot copy share one row -- so
+// in steady state no coefficient is loaded at all; a point whose row changes
+// (wall planes, flagged points, full storage) loads them at the top of the plane.
+// Points outside the box read as zero.
+template <typename T, int TX, int TY, class Op>
+__device__ inline void march_tile(Op &op, int nx, int ny, int nz, int k0, int kend, int bx, int by) {
+  using Raw = typename Op::Raw;
+  __shared__ T plane[2][TY + 2][TX + 2];
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i = bx * TX + tx, j = by * TY + ty;
+  const bool in = (i < nx) && (j < ny);
+  int hx = 0, hy = 0;
+  const bool ring = tile_ring<TX, TY>(ty * TX + tx, hx, hy);
+  const int hi = bx * TX + hx, hj = by * TY + hy;
+  const bool ring_in = ring && hi >= 0 && hi < nx && hj >= 0 && hj < ny;
+  T zm = (in && k0 > 0) ? op.finish(op.fetch(i, j, k0 - 1)) : T(0);
+  Raw own0{}, own1{}, ring0{};
+  if (in && k0 < nz) own0 = op.fetch(i, j, k0);
+  if (in && k0 + 1 < nz) own1 = op.fetch(i, j, k0 + 1);
+  if (ring_in && k0 < kend) ring0 = op.fetch(hi, hj, k0);
+  T zc = (in && k0 < nz) ? op.finish(own0) : T(0);
+  Coef7<T> a;
+  long long arow = -1;
+  for (int k = k0; k < kend; ++k) {
+    Raw own2{}, ring1{};
+    if (in && own0.u != arow) {
+      a.load(op.c, op.pitch, own0.u);
+      arow = own0.u;
+    }
+    if (in && k + 2 < nz && k + 1 < kend) own2 = op.fetch(i, j, k + 2);
+    if (ring_in && k + 1 < kend) ring1 = op.fetch(hi, hj, k + 1);
+    const T zp = (in && k + 1 < nz) ? op.finish(own1) : T(0);
+    T(*P)[TX + 2] = plane[(k - k0) & 1];
+    P[ty + 1][tx + 1] = zc;
+    if (ring) P[hy + 1][hx + 1] = ring_in ? op.finish(ring0) : T(0);
+    __syncthreads();
+    if (in) {
+      Nbr<T> nb{zc, P[ty + 1][tx], P[ty + 1][tx + 2], P[ty][tx + 1], P[ty + 2][tx + 1], zm, zp,
+                nbr_in_box(i, j, k, nx, ny, nz)};
+      op.out(a, own0, i, j, k, nb);
+    }
+    zm = zc;
+    zc = zp;
+    own0 = own1;
+    own1 = own2;
+    ring0 = ring1;
+  }
+}
 
 // Launch geometry of a z-marching kernel: tile rows TY and z-chunk length.
 struct TileLaunch {
