@@ -125,7 +125,7 @@ __global__ void __launch_bounds__(256) k_b_residual(const BSys *sys, Sub sub, in
   if ((int)blockIdx.x >= s.nred) return;
   if (blockIdx.z == 0) {
     double *r = (out_mode == 0) ? s.V : (out_mode == 1 ? s.W : nullptr);
-    kry::residual_body(s.fc, s.fpitch, ZMap(), s.nx, s.ny, s.nz, s.b, s.x, r, s.part, blockIdx.x, s.nred);
+    kry::residual_body(s.fc, s.fpitch, FullMap(), s.nx, s.ny, s.nz, s.b, s.x, r, s.part, blockIdx.x, s.nred);
   } else {
     kry::dot_body(s.b, s.b, s.N, s.part + NB, blockIdx.x, s.nred);
   }
@@ -168,7 +168,7 @@ __global__ void __launch_bounds__(256) k_b_matvec(const BSys *sys, const int *it
   ijk_of(t, s.nx, s.ny, i, j, k);
   const double *z = use_z ? s.Z : s.V + (long long)cur * s.vp;
   const double sg = S->sigma[cur];
-  const double v = kry::arnoldi_row(s.fc, s.fpitch, ZMap(), s.nx, s.ny, s.nz, z, s.dinv_full, i, j, k, t);
+  const double v = kry::arnoldi_row(s.fc, s.fpitch, FullMap(), s.nx, s.ny, s.nz, z, s.dinv_full, i, j, k, t);
   s.V[(long long)(cur + 1) * s.vp + t] = sg * v;
 }
 
@@ -271,8 +271,8 @@ __global__ void __launch_bounds__(256) k_b_down(const BLev *lev, const BSys *sys
   if (t >= L.N) return;
   int i, j, k;
   ijk_of(t, L.nx, L.ny, i, j, k);
-  L.r[t] = mgd::down_point<double>(L.fc, L.fpitch, ZMap(), L.nx, L.ny, L.nz, L.fdinv, lev_b(L, s, in_mode, step), L.w,
-                                   i, j, k, t);
+  L.r[t] = mgd::down_point<double>(L.fc, L.fpitch, FullMap(), L.nx, L.ny, L.nz, L.fdinv, lev_b(L, s, in_mode, step),
+                                   L.w, i, j, k, t);
 }
 
 __global__ void __launch_bounds__(256) k_b_restrict(const BLev *lev, const BSys *sys, const int *item, const int *first,
@@ -284,7 +284,7 @@ __global__ void __launch_bounds__(256) k_b_restrict(const BLev *lev, const BSys 
   if (t >= L.Nc) return;
   int i, j, k;
   ijk_of(t, L.cnx, L.cny, i, j, k);
-  L.bc[t] = mgd::restrict_point<double>(L.nx, L.ny, L.nz, L.cdir, L.r, L.fwm, L.fwp, ZMap(), i, j, k);
+  L.bc[t] = mgd::restrict_point<double>(L.nx, L.ny, L.nz, L.cdir, L.r, L.fwm, L.fwp, FullMap(), i, j, k);
 }
 
 // coarsest levels: four rows per block, one wavefront per row
@@ -314,13 +314,13 @@ __global__ void __launch_bounds__(256) k_b_up(const BLev *lev, const BSys *sys, 
   const double *b = lev_b(L, s, in_mode, step);
   double v;
   if (L.cdir == 0)
-    v = mgd::up_point<double, 0>(L.fc, L.fpitch, ZMap(), L.nx, L.ny, L.nz, L.fdinv, b, L.fwm, L.fwp, L.yc, L.cnx,
+    v = mgd::up_point<double, 0>(L.fc, L.fpitch, FullMap(), L.nx, L.ny, L.nz, L.fdinv, b, L.fwm, L.fwp, L.yc, L.cnx,
                                  L.cny, L.ncd, L.w, i, j, k, t);
   else if (L.cdir == 1)
-    v = mgd::up_point<double, 1>(L.fc, L.fpitch, ZMap(), L.nx, L.ny, L.nz, L.fdinv, b, L.fwm, L.fwp, L.yc, L.cnx,
+    v = mgd::up_point<double, 1>(L.fc, L.fpitch, FullMap(), L.nx, L.ny, L.nz, L.fdinv, b, L.fwm, L.fwp, L.yc, L.cnx,
                                  L.cny, L.ncd, L.w, i, j, k, t);
   else
-    v = mgd::up_point<double, 2>(L.fc, L.fpitch, ZMap(), L.nx, L.ny, L.nz, L.fdinv, b, L.fwm, L.fwp, L.yc, L.cnx,
+    v = mgd::up_point<double, 2>(L.fc, L.fpitch, FullMap(), L.nx, L.ny, L.nz, L.fdinv, b, L.fwm, L.fwp, L.yc, L.cnx,
                                  L.cny, L.ncd, L.w, i, j, k, t);
   (L.lvl == 0 ? s.Z : L.y)[t] = v;
 }

@@ -133,6 +133,12 @@ struct ZMap {
   }
 };
 
+// The full-storage map as a type: row t, no flags, no branches (the one-point
+// kernels, whose neighbour loads must issue together).
+struct FullMap {
+  __host__ __device__ inline long long operator()(long long t, int) const { return t; }
+};
+
 template <typename T>
 struct Stencil {
   const T *c;        // 7 planes

@@ -41,7 +41,7 @@ using namespace kry;
 // w = A z with z = sigma * p (* dinv when Jacobi is fused), one thread per point.
 // The current basis index is read from the device status.
 template <typename DT>
-__global__ void k_arnoldi_matvec(const double *__restrict__ c, long long pitch, ZMap zm, int nx, int ny, int nz,
+__global__ void k_arnoldi_matvec(const double *__restrict__ c, long long pitch, int nx, int ny, int nz,
                                  const double *__restrict__ V, long long vp, const double *__restrict__ zin,
                                  const DT *__restrict__ dinv, State *S) {
   if (S->st.done) return;
@@ -54,7 +54,7 @@ __global__ void k_arnoldi_matvec(const double *__restrict__ c, long long pitch, 
   double sg = S->sigma[cur];
   double *w = (double *)V + (long long)(cur + 1) * vp;
   long long r = ((long long)k * ny + j) * nx + i;
-  double s = arnoldi_row(c, pitch, zm, nx, ny, nz, z, dinv, i, j, k, r);
+  double s = arnoldi_row(c, pitch, FullMap(), nx, ny, nz, z, dinv, i, j, k, r);
   w[r] = sg * s;
 }
 
@@ -92,7 +92,7 @@ int launch_arnoldi_matvec(const fpc_StructMatrix_s *A, const double *V, long lon
     const DT *dinv = jac ? jacobi_dinv<DT>(A, false) : nullptr;
     dim3 g, b;
     launch_shape(d, g, b);
-    k_arnoldi_matvec<DT><<<g, b, 0, stream()>>>(A->coef, A->pitch, ZMap(), d.nx, d.ny, d.nz, V, vp, zin, dinv, S);
+    k_arnoldi_matvec<DT><<<g, b, 0, stream()>>>(A->coef, A->pitch, d.nx, d.ny, d.nz, V, vp, zin, dinv, S);
   }
   FPC_LAUNCH_CHECK();
   return 0;

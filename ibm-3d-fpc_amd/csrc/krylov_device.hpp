@@ -44,31 +44,35 @@ __device__ inline double &H(State *S, int row, int col) { return S->hh[col * (KM
 
 // (A z)_r at point (i, j, k) = row r, with z scaled by 1/diag when dinv is given.
 // DT: the stored 1/diag (double, or float for the FP32 Jacobi); the products are FP64.
-// c and dinv are read at the rows zm gives (full or z-extruded storage).
-template <typename DT = double>
-__device__ inline double arnoldi_row(const double *__restrict__ c, long long pitch, ZMap zm, int nx, int ny, int nz,
+// c and dinv are read at the rows zm gives (M: ZMap or FullMap).  Every load is
+// issued at once: a neighbour outside the box is read at the point itself and its
+// term skipped by a select (skipped, not added as a*0: the tile forms' bits).
+template <typename DT = double, class M>
+__device__ inline double arnoldi_row(const double *__restrict__ c, long long pitch, M zm, int nx, int ny, int nz,
                                      const double *__restrict__ z, const DT *__restrict__ dinv, int i, int j, int k,
                                      long long r) {
   const long long pl = (long long)nx * ny;
+  const bool xm = i > 0, xp = i < nx - 1, ym = j > 0, yp = j < ny - 1, zl = k > 0, zh = k < nz - 1;
   const long long q = zm(r, k);
-  double s;
+  const long long r1 = xm ? r - 1 : r, r2 = xp ? r + 1 : r, r3 = ym ? r - nx : r, r4 = yp ? r + nx : r;
+  const long long r5 = zl ? r - pl : r, r6 = zh ? r + pl : r;
+  double v0 = z[r], v1 = z[r1], v2 = z[r2], v3 = z[r3], v4 = z[r4], v5 = z[r5], v6 = z[r6];
   if (dinv) {
-    s = c[q] * (z[r] * (double)dinv[q]);
-    if (i > 0) s += c[pitch + q] * (z[r - 1] * (double)dinv[zm(r - 1, k)]);
-    if (i < nx - 1) s += c[2 * pitch + q] * (z[r + 1] * (double)dinv[zm(r + 1, k)]);
-    if (j > 0) s += c[3 * pitch + q] * (z[r - nx] * (double)dinv[zm(r - nx, k)]);
-    if (j < ny - 1) s += c[4 * pitch + q] * (z[r + nx] * (double)dinv[zm(r + nx, k)]);
-    if (k > 0) s += c[5 * pitch + q] * (z[r - pl] * (double)dinv[zm(r - pl, k - 1)]);
-    if (k < nz - 1) s += c[6 * pitch + q] * (z[r + pl] * (double)dinv[zm(r + pl, k + 1)]);
-  } else {
-    s = c[q] * z[r];
-    if (i > 0) s += c[pitch + q] * z[r - 1];
-    if (i < nx - 1) s += c[2 * pitch + q] * z[r + 1];
-    if (j > 0) s += c[3 * pitch + q] * z[r - nx];
-    if (j < ny - 1) s += c[4 * pitch + q] * z[r + nx];
-    if (k > 0) s += c[5 * pitch + q] * z[r - pl];
-    if (k < nz - 1) s += c[6 * pitch + q] * z[r + pl];
+    v0 = v0 * (double)dinv[q];
+    v1 = v1 * (double)dinv[zm(r1, k)];
+    v2 = v2 * (double)dinv[zm(r2, k)];
+    v3 = v3 * (double)dinv[zm(r3, k)];
+    v4 = v4 * (double)dinv[zm(r4, k)];
+    v5 = v5 * (double)dinv[zm(r5, zl ? k - 1 : k)];
+    v6 = v6 * (double)dinv[zm(r6, zh ? k + 1 : k)];
   }
+  double s = c[q] * v0;
+  s = xm ? s + c[pitch + q] * v1 : s;
+  s = xp ? s + c[2 * pitch + q] * v2 : s;
+  s = ym ? s + c[3 * pitch + q] * v3 : s;
+  s = yp ? s + c[4 * pitch + q] * v4 : s;
+  s = zl ? s + c[5 * pitch + q] * v5 : s;
+  s = zh ? s + c[6 * pitch + q] * v6 : s;
   return s;
 }
 
@@ -253,7 +257,8 @@ __device__ inline void cycle_start_body(State *S, double r_norm, double eps, int
 }
 
 // r = b - A x (r optional) with the block's partial sum of r^2 into part[bid]; block bid of nblk.
-__device__ inline void residual_body(const double *__restrict__ c, long long pitch, ZMap zm, int nx, int ny, int nz,
+template <class M>
+__device__ inline void residual_body(const double *__restrict__ c, long long pitch, M zm, int nx, int ny, int nz,
                                      const double *__restrict__ b, const double *__restrict__ x, double *__restrict__ r,
                                      double *__restrict__ part, int bid, int nblk) {
   __shared__ double lds[8];
@@ -265,13 +270,16 @@ __device__ inline void residual_body(const double *__restrict__ c, long long pit
     int j = (int)(q % ny);
     int k = (int)(q / ny);
     const long long u = zm(t, k);
+    const bool xm = i > 0, xp = i < nx - 1, ym = j > 0, yp = j < ny - 1, zl = k > 0, zh = k < nz - 1;
+    const double x1 = x[xm ? t - 1 : t], x2 = x[xp ? t + 1 : t], x3 = x[ym ? t - nx : t], x4 = x[yp ? t + nx : t];
+    const double x5 = x[zl ? t - pl : t], x6 = x[zh ? t + pl : t];
     double s = c[u] * x[t];
-    if (i > 0) s += c[pitch + u] * x[t - 1];
-    if (i < nx - 1) s += c[2 * pitch + u] * x[t + 1];
-    if (j > 0) s += c[3 * pitch + u] * x[t - nx];
-    if (j < ny - 1) s += c[4 * pitch + u] * x[t + nx];
-    if (k > 0) s += c[5 * pitch + u] * x[t - pl];
-    if (k < nz - 1) s += c[6 * pitch + u] * x[t + pl];
+    s = xm ? s + c[pitch + u] * x1 : s;
+    s = xp ? s + c[2 * pitch + u] * x2 : s;
+    s = ym ? s + c[3 * pitch + u] * x3 : s;
+    s = yp ? s + c[4 * pitch + u] * x4 : s;
+    s = zl ? s + c[5 * pitch + u] * x5 : s;
+    s = zh ? s + c[6 * pitch + u] * x6 : s;
     double rv = b[t] - s;
     if (r) r[t] = rv;
     acc += rv * rv;

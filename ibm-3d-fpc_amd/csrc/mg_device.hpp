@@ -18,81 +18,143 @@
 namespace fpc {
 namespace mgd {
 
-// Residual of one weighted-Jacobi sweep from zero: b - A (w D^-1 b) at (i, j, k) = row t.
+// The one-point bodies below issue every load of a point at once: a neighbour
+// outside the box is read at the point itself (a valid address) and its term is
+// then skipped by a select, so no load waits behind a bounds branch.  Skipping
+// (rather than adding a*0) keeps the sums bit for bit those of the tile forms.
+
+// s + a v when the neighbour is in the box, else s.
 template <typename T>
-__device__ inline T down_point(const T *__restrict__ c, long long pitch, ZMap zm, int nx, int ny, int nz,
+__device__ inline T add_if(bool in, T s, T a, T v) {
+  return in ? s + a * v : s;
+}
+
+// Residual of one weighted-Jacobi sweep from zero: b - A (w D^-1 b) at (i, j, k) = row t.
+template <typename T, class M>
+__device__ inline T down_point(const T *__restrict__ c, long long pitch, M zm, int nx, int ny, int nz,
                                const T *__restrict__ dinv, const T *__restrict__ b, T w, int i, int j, int k,
                                long long t) {
   const long long pl = (long long)nx * ny;
+  const bool xm = i > 0, xp = i < nx - 1, ym = j > 0, yp = j < ny - 1, zl = k > 0, zh = k < nz - 1;
   const long long u = zm(t, k);
-  T s = c[u] * (w * b[t] * dinv[u]);
-  if (i > 0) s += c[pitch + u] * (w * b[t - 1] * dinv[zm(t - 1, k)]);
-  if (i < nx - 1) s += c[2 * pitch + u] * (w * b[t + 1] * dinv[zm(t + 1, k)]);
-  if (j > 0) s += c[3 * pitch + u] * (w * b[t - nx] * dinv[zm(t - nx, k)]);
-  if (j < ny - 1) s += c[4 * pitch + u] * (w * b[t + nx] * dinv[zm(t + nx, k)]);
-  if (k > 0) s += c[5 * pitch + u] * (w * b[t - pl] * dinv[zm(t - pl, k - 1)]);
-  if (k < nz - 1) s += c[6 * pitch + u] * (w * b[t + pl] * dinv[zm(t + pl, k + 1)]);
+  const long long t1 = xm ? t - 1 : t, t2 = xp ? t + 1 : t, t3 = ym ? t - nx : t, t4 = yp ? t + nx : t;
+  const long long t5 = zl ? t - pl : t, t6 = zh ? t + pl : t;
+  const T x0 = w * b[t] * dinv[u];
+  const T x1 = w * b[t1] * dinv[zm(t1, k)], x2 = w * b[t2] * dinv[zm(t2, k)];
+  const T x3 = w * b[t3] * dinv[zm(t3, k)], x4 = w * b[t4] * dinv[zm(t4, k)];
+  const T x5 = w * b[t5] * dinv[zm(t5, zl ? k - 1 : k)], x6 = w * b[t6] * dinv[zm(t6, zh ? k + 1 : k)];
+  T s = c[u] * x0;
+  s = add_if(xm, s, c[pitch + u], x1);
+  s = add_if(xp, s, c[2 * pitch + u], x2);
+  s = add_if(ym, s, c[3 * pitch + u], x3);
+  s = add_if(yp, s, c[4 * pitch + u], x4);
+  s = add_if(zl, s, c[5 * pitch + u], x5);
+  s = add_if(zh, s, c[6 * pitch + u], x6);
   return b[t] - s;
 }
 
 // Restriction to coarse point (i, j, k) of a level coarsened in direction d:
 // r(C) + w+(F-) r(F-) + w-(F+) r(F+), C = fine 2c+1.
-template <typename T>
+template <typename T, class M>
 __device__ inline T restrict_point(int fnx, int fny, int fnz, int d, const T *__restrict__ r, const T *__restrict__ wm,
-                                   const T *__restrict__ wp, ZMap zm, int i, int j, int k) {
+                                   const T *__restrict__ wp, M zm, int i, int j, int k) {
   const int fi = (d == 0) ? 2 * i + 1 : i, fj = (d == 1) ? 2 * j + 1 : j, fk = (d == 2) ? 2 * k + 1 : k;
   const int pd = (d == 0) ? fi : (d == 1 ? fj : fk);
   const int fnd = (d == 0) ? fnx : (d == 1 ? fny : fnz);
   const long long stride = (d == 0) ? 1 : (d == 1 ? (long long)fnx : (long long)fnx * fny);
   const long long f0 = ((long long)fk * fny + fj) * fnx + fi;
   const int dk = (d == 2) ? 1 : 0;
-  T v = r[f0] + wp[zm(f0 - stride, fk - dk)] * r[f0 - stride];
-  if (pd + 1 < fnd) v += wm[zm(f0 + stride, fk + dk)] * r[f0 + stride];
-  return v;
+  const bool hp = pd + 1 < fnd;
+  const long long fp = hp ? f0 + stride : f0;
+  const T v = r[f0] + wp[zm(f0 - stride, fk - dk)] * r[f0 - stride];
+  return add_if(hp, v, wm[zm(fp, hp ? fk + dk : fk)], r[fp]);
 }
 
-// x = w D^-1 b plus the interpolated coarse correction at fine point (i, j, k),
-// the level coarsened in direction D (C points inject, F points take the
-// weighted neighbours).
-template <typename T, int D>
-__device__ inline T corrected(const T *b, const T *dinv, const T *wm, const T *wp, const T *yc, ZMap zm, T w, int i,
-                              int j, int k, int fnx, int fny, int cnx, int cny, int ncd) {
+// What the interpolated value x of fine point (i, j, k) needs, the level coarsened
+// in direction D: C points (odd) inject e1; F points take wm e0 + wp e1, an
+// absent coarse neighbour reading as a zero weight and value.
+template <typename T>
+struct UpLoads {
+  T b, dinv, wm, wp, e0, e1;
+  bool f;
+  long long u;  // row of the point's coefficients
+};
+// kSpec: load every operand unconditionally at valid addresses and select (one
+// load round trip, for the latency-bound one-point kernels), or load only what
+// the point uses (the tile forms, which prefetch ahead); the values are the same.
+template <typename T, int D, bool kSpec, class M>
+__device__ inline UpLoads<T> up_loads(const T *b, const T *dinv, const T *wm, const T *wp, const T *yc, M zm, int i,
+                                      int j, int k, int fnx, int fny, int cnx, int cny, int ncd) {
   const long long t = ((long long)k * fny + j) * fnx + i, u = zm(t, k);
-  T x = w * b[t] * dinv[u];
   const int pd = (D == 0) ? i : (D == 1 ? j : k);
   const long long cstride = (D == 0) ? 1 : (D == 1 ? (long long)cnx : (long long)cnx * cny);
   const int q = (pd & 1) ? ((pd - 1) >> 1) : (pd >> 1);
   const int ci = (D == 0) ? q : i, cj = (D == 1) ? q : j, ck = (D == 2) ? q : k;
   const long long cr = ((long long)ck * cny + cj) * cnx + ci;
-  if (pd & 1) {
-    x += yc[cr];
+  const bool odd = pd & 1, lo = !odd && q >= 1, hi = !odd && q < ncd;
+  UpLoads<T> r;
+  r.u = u;
+  r.b = b[t];
+  r.dinv = dinv[u];
+  r.f = !odd;
+  if (kSpec) {
+    // valid coarse rows: cr exists for C points and for q < ncd; q = ncd implies q >= 1
+    const long long c1 = (odd || hi) ? cr : cr - cstride;
+    const long long c0 = lo ? cr - cstride : c1;
+    const T wml = wm[u], wpl = wp[u], y0 = yc[c0], y1 = yc[c1];
+    r.wm = lo ? wml : T(0);
+    r.e0 = lo ? y0 : T(0);
+    r.wp = hi ? wpl : T(0);
+    r.e1 = (odd || hi) ? y1 : T(0);
+  } else if (odd) {
+    r.wm = r.wp = r.e0 = T(0);
+    r.e1 = yc[cr];
   } else {
+    r.wm = lo ? wm[u] : T(0);
+    r.e0 = lo ? yc[cr - cstride] : T(0);
+    r.wp = hi ? wp[u] : T(0);
+    r.e1 = hi ? yc[cr] : T(0);
+  }
+  return r;
+}
+// x = w D^-1 b plus the interpolated coarse correction.
+template <typename T>
+__device__ inline T up_value(const UpLoads<T> &r, T w) {
+  T x = w * r.b * r.dinv;
+  if (r.f) {
     T s = (T)0;
-    if (q - 1 >= 0) s += wm[u] * yc[cr - cstride];
-    if (q < ncd) s += wp[u] * yc[cr];
+    s += r.wm * r.e0;
+    s += r.wp * r.e1;
     x += s;
+  } else {
+    x += r.e1;
   }
   return x;
 }
 
 // Prolongation + one weighted-Jacobi sweep at (i, j, k) = row t:
-// y = x + w D^-1 (b - A x) with x = corrected() at the point and its neighbours.
-template <typename T, int D>
-__device__ inline T up_point(const T *__restrict__ c, long long pitch, ZMap zm, int nx, int ny, int nz,
+// y = x + w D^-1 (b - A x) with x = up_value() at the point and its neighbours.
+template <typename T, int D, class M>
+__device__ inline T up_point(const T *__restrict__ c, long long pitch, M zm, int nx, int ny, int nz,
                              const T *__restrict__ dinv, const T *__restrict__ b, const T *__restrict__ wm,
                              const T *__restrict__ wp, const T *__restrict__ yc, int cnx, int cny, int ncd, T w, int i,
                              int j, int k, long long t) {
-#define FPC_CORR(ii, jj, kk) corrected<T, D>(b, dinv, wm, wp, yc, zm, w, ii, jj, kk, nx, ny, cnx, cny, ncd)
+#define FPC_X(ii, jj, kk) \
+  up_value<T>(up_loads<T, D, true>(b, dinv, wm, wp, yc, zm, ii, jj, kk, nx, ny, cnx, cny, ncd), w)
+  const bool xm = i > 0, xp = i < nx - 1, ym = j > 0, yp = j < ny - 1, zl = k > 0, zh = k < nz - 1;
   const long long u = zm(t, k);
-  T xc = FPC_CORR(i, j, k);
+  const T xc = FPC_X(i, j, k);
+  const T x1 = FPC_X(xm ? i - 1 : i, j, k), x2 = FPC_X(xp ? i + 1 : i, j, k);
+  const T x3 = FPC_X(i, ym ? j - 1 : j, k), x4 = FPC_X(i, yp ? j + 1 : j, k);
+  const T x5 = FPC_X(i, j, zl ? k - 1 : k), x6 = FPC_X(i, j, zh ? k + 1 : k);
+#undef FPC_X
   T s = c[u] * xc;
-  if (i > 0) s += c[pitch + u] * FPC_CORR(i - 1, j, k);
-  if (i < nx - 1) s += c[2 * pitch + u] * FPC_CORR(i + 1, j, k);
-  if (j > 0) s += c[3 * pitch + u] * FPC_CORR(i, j - 1, k);
-  if (j < ny - 1) s += c[4 * pitch + u] * FPC_CORR(i, j + 1, k);
-  if (k > 0) s += c[5 * pitch + u] * FPC_CORR(i, j, k - 1);
-  if (k < nz - 1) s += c[6 * pitch + u] * FPC_CORR(i, j, k + 1);
-#undef FPC_CORR
+  s = add_if(xm, s, c[pitch + u], x1);
+  s = add_if(xp, s, c[2 * pitch + u], x2);
+  s = add_if(ym, s, c[3 * pitch + u], x3);
+  s = add_if(yp, s, c[4 * pitch + u], x4);
+  s = add_if(zl, s, c[5 * pitch + u], x5);
+  s = add_if(zh, s, c[6 * pitch + u], x6);
   return xc + w * (b[t] - s) * dinv[u];
 }
 
@@ -199,51 +261,12 @@ struct UpOp {
   T *y;
   T w;
   ZMap zm;  // rows of c, dinv, wm and wp
-  // Loads of corrected<T, D>: C points use e1 only; at F points an absent coarse
-  // neighbour reads as a zero weight and value (0 + wp e1 == wp e1 exactly).
-  struct Raw {
-    T b, dinv, wm, wp, e0, e1;
-    bool f;
-    long long u;
-  };
+  using Raw = UpLoads<T>;
   __device__ long long at(int i, int j, int k) const { return ((long long)k * ny + j) * nx + i; }
   __device__ Raw fetch(int i, int j, int k) const {
-    const long long t = at(i, j, k);
-    const int pd = (D == 0) ? i : (D == 1 ? j : k);
-    const long long cstride = (D == 0) ? 1 : (D == 1 ? (long long)cnx : (long long)cnx * cny);
-    const int q = (pd & 1) ? ((pd - 1) >> 1) : (pd >> 1);
-    const int ci = (D == 0) ? q : i, cj = (D == 1) ? q : j, ck = (D == 2) ? q : k;
-    const long long cr = ((long long)ck * cny + cj) * cnx + ci;
-    const long long u = zm(t, k);
-    Raw r;
-    r.u = u;
-    r.b = b[t];
-    r.dinv = dinv[u];
-    r.f = !(pd & 1);
-    if (pd & 1) {
-      r.wm = r.wp = r.e0 = T(0);
-      r.e1 = yc[cr];
-    } else {
-      const bool lo = q - 1 >= 0, hi = q < ncd;
-      r.wm = lo ? wm[u] : T(0);
-      r.e0 = lo ? yc[cr - cstride] : T(0);
-      r.wp = hi ? wp[u] : T(0);
-      r.e1 = hi ? yc[cr] : T(0);
-    }
-    return r;
+    return up_loads<T, D, false>(b, dinv, wm, wp, yc, zm, i, j, k, nx, ny, cnx, cny, ncd);
   }
-  __device__ T finish(const Raw &r) const {
-    T x = w * r.b * r.dinv;
-    if (r.f) {
-      T s = (T)0;
-      s += r.wm * r.e0;
-      s += r.wp * r.e1;
-      x += s;
-    } else {
-      x += r.e1;
-    }
-    return x;
-  }
+  __device__ T finish(const Raw &r) const { return up_value<T>(r, w); }
   __device__ void out(const Coef7<T> &a, const Raw &own, int i, int j, int k, const Nbr<T> &nb) {
     y[at(i, j, k)] = nb.c + w * (own.b - a.apply(nb)) * own.dinv;
   }

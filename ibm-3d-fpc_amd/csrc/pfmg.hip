@@ -263,6 +263,21 @@ std::vector<double> dense_inverse(const HLevel &L) {
 }
 
 // ---------------------------------------------------------------- device kernels
+// Read when a kernel gets no skip flag (always 0).
+__device__ int g_no_skip = 0;
+
+// The one-point kernels of the small levels are latency-bound: they read the
+// skip flag with a vector load issued together with their data loads and test
+// it only before the store, so the flag costs no round trip of its own.
+__device__ inline int skip_flag(const int *skip) {
+  return __hip_atomic_load(skip ? skip : &g_no_skip, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// No instruction: makes v be computed before what follows (the skip test).
+template <typename T>
+__device__ inline void pin(T v) {
+  asm volatile("" ::"v"(v));
+}
+
 template <typename T>
 __global__ void k_relax_zero(const T *__restrict__ dinv, const T *__restrict__ b, T *__restrict__ x, T w, long long n,
                              const int *skip) {
@@ -342,14 +357,16 @@ __global__ void k_relax_rb(const T *__restrict__ c, long long pitch, int nx, int
 // b_c = r(F0) + w+(Fm) r(Fm) + w-(Fp) r(Fp), one thread per coarse point.
 template <typename T>
 __global__ void k_restrict(int fnx, int fny, int fnz, int d, const T *__restrict__ r, const T *__restrict__ wm,
-                           const T *__restrict__ wp, ZMap zm, int cnx, int cny, int cnz, T *__restrict__ bc,
-                           const int *skip) {
-  if (skip && *skip) return;
+                           const T *__restrict__ wp, int cnx, int cny, int cnz, T *__restrict__ bc, const int *skip) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y;
   int k = blockIdx.z;
   if (i >= cnx || j >= cny) return;
-  bc[((long long)k * cny + j) * cnx + i] = mgd::restrict_point<T>(fnx, fny, fnz, d, r, wm, wp, zm, i, j, k);
+  const int sk = skip_flag(skip);
+  const T v = mgd::restrict_point<T>(fnx, fny, fnz, d, r, wm, wp, FullMap(), i, j, k);
+  pin(v);
+  if (sk) return;
+  bc[((long long)k * cny + j) * cnx + i] = v;
 }
 
 // x_f += P x_c, one thread per fine point.
@@ -384,30 +401,36 @@ __global__ void k_prolong_add(int fnx, int fny, int fnz, int d, const T *__restr
 // x = w b dinv is recomputed on the fly at every stencil point, so only the
 // residual r = b - A x is written (x is recomputed by the up-sweep).
 template <typename T>
-__global__ void k_down_jacobi(const T *__restrict__ c, long long pitch, ZMap zm, int nx, int ny, int nz,
+__global__ void k_down_jacobi(const T *__restrict__ c, long long pitch, int nx, int ny, int nz,
                               const T *__restrict__ dinv, const T *__restrict__ b, T *__restrict__ r, T w,
                               const int *skip) {
-  if (skip && *skip) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y;
   int k = blockIdx.z;
   if (i >= nx || j >= ny) return;
   long long t = ((long long)k * ny + j) * nx + i;
-  r[t] = mgd::down_point<T>(c, pitch, zm, nx, ny, nz, dinv, b, w, i, j, k, t);
+  const int sk = skip_flag(skip);
+  const T v = mgd::down_point<T>(c, pitch, FullMap(), nx, ny, nz, dinv, b, w, i, j, k, t);
+  pin(v);
+  if (sk) return;
+  r[t] = v;
 }
 
 template <typename T, int D>
-__global__ void k_up_jacobi(const T *__restrict__ c, long long pitch, ZMap zm, int nx, int ny, int nz,
+__global__ void k_up_jacobi(const T *__restrict__ c, long long pitch, int nx, int ny, int nz,
                             const T *__restrict__ dinv, const T *__restrict__ b, const T *__restrict__ wm,
                             const T *__restrict__ wp, const T *__restrict__ yc, int cnx, int cny, int ncd,
                             T *__restrict__ y, T w, const int *skip) {
-  if (skip && *skip) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y;
   int k = blockIdx.z;
   if (i >= nx || j >= ny) return;
   long long t = ((long long)k * ny + j) * nx + i;
-  y[t] = mgd::up_point<T, D>(c, pitch, zm, nx, ny, nz, dinv, b, wm, wp, yc, cnx, cny, ncd, w, i, j, k, t);
+  const int sk = skip_flag(skip);
+  const T v = mgd::up_point<T, D>(c, pitch, FullMap(), nx, ny, nz, dinv, b, wm, wp, yc, cnx, cny, ncd, w, i, j, k, t);
+  pin(v);
+  if (sk) return;
+  y[t] = v;
 }
 
 // x = Ainv b on the coarsest level (FP64 inverse), one block.
@@ -811,8 +834,7 @@ int vcycle(PfmgImpl *P, const double *in, double *out, const int *skip) {
         k_down_tiled<T, 8><<<tl.grid, tl.block, 0, st>>>(hc, H.pitch, H.zm, F.d.nx, F.d.ny, F.d.nz, hd, bof(l),
                                                           (T *)F.r, w_of(l), tl.kz, skip);
       } else {
-        k_down_jacobi<T><<<g, bl, 0, st>>>(hc, H.pitch, H.zm, F.d.nx, F.d.ny, F.d.nz, hd, bof(l), (T *)F.r, w_of(l),
-                                           skip);
+        k_down_jacobi<T><<<g, bl, 0, st>>>(c, F.pitch, F.d.nx, F.d.ny, F.d.nz, dv, bof(l), (T *)F.r, w_of(l), skip);
       }
       FPC_PROF_END(cls, (rows * 8.0 + (double)F.d.N * 2.0) * sz);
       FPC_LAUNCH_CHECK();
@@ -821,9 +843,9 @@ int vcycle(PfmgImpl *P, const double *in, double *out, const int *skip) {
       {
         // bytes: r at every fine point, the weights at the F points, b of each coarse point
         FPC_PROF_BEGIN("mg_restrict");
-        k_restrict<T><<<gc, bc, 0, st>>>(F.d.nx, F.d.ny, F.d.nz, F.cdir, (const T *)F.r, hwm, hwp, H.zm, C.d.nx,
-                                        C.d.ny, C.d.nz, (T *)C.b, skip);
-        FPC_PROF_END("mg_restrict", ((double)F.d.N + rows + (double)C.d.N) * sz);
+        k_restrict<T><<<gc, bc, 0, st>>>(F.d.nx, F.d.ny, F.d.nz, F.cdir, (const T *)F.r, (const T *)F.wm,
+                                        (const T *)F.wp, C.d.nx, C.d.ny, C.d.nz, (T *)C.b, skip);
+        FPC_PROF_END("mg_restrict", ((double)F.d.N * 2.0 + (double)C.d.N) * sz);
       }
       FPC_LAUNCH_CHECK();
       continue;
@@ -846,7 +868,7 @@ int vcycle(PfmgImpl *P, const double *in, double *out, const int *skip) {
     dim3 gc, bc;
     launch_shape(C.d, gc, bc);
     k_restrict<T><<<gc, bc, 0, st>>>(F.d.nx, F.d.ny, F.d.nz, F.cdir, (const T *)F.r, (const T *)F.wm, (const T *)F.wp,
-                                    ZMap(), C.d.nx, C.d.ny, C.d.nz, (T *)C.b, skip);
+                                    C.d.nx, C.d.ny, C.d.nz, (T *)C.b, skip);
     FPC_LAUNCH_CHECK();
   }
   {
@@ -883,8 +905,8 @@ int vcycle(PfmgImpl *P, const double *in, double *out, const int *skip) {
 #undef FPC_UP_TILED
       } else {
 #define FPC_UP_SIMPLE(D)                                                                                        \
-  k_up_jacobi<T, D><<<g, bl, 0, st>>>(hc, H.pitch, H.zm, F.d.nx, F.d.ny, F.d.nz, hd, bof(l), wm, wp, yc, C.d.nx, \
-                                      C.d.ny, ncd, yof(l), w_of(l), skip)
+  k_up_jacobi<T, D><<<g, bl, 0, st>>>(c, F.pitch, F.d.nx, F.d.ny, F.d.nz, dv, bof(l), (const T *)F.wm,            \
+                                      (const T *)F.wp, yc, C.d.nx, C.d.ny, ncd, yof(l), w_of(l), skip)
         if (F.cdir == 0) FPC_UP_SIMPLE(0);
         else if (F.cdir == 1) FPC_UP_SIMPLE(1);
         else FPC_UP_SIMPLE(2);

@@ -241,7 +241,8 @@ __global__ void k_matvec(const double *__restrict__ c, long long pitch, int nx, 
 }
 
 // r = b - A x with per-block partial sums of r^2.
-__global__ void __launch_bounds__(256) k_residual(const double *__restrict__ c, long long pitch, ZMap zm, int nx, int ny,
+template <class M>
+__global__ void __launch_bounds__(256) k_residual(const double *__restrict__ c, long long pitch, M zm, int nx, int ny,
                                                   int nz, const double *__restrict__ b, const double *__restrict__ x,
                                                   double *__restrict__ r, double *__restrict__ part) {
   kry::residual_body(c, pitch, zm, nx, ny, nz, b, x, r, part, blockIdx.x, gridDim.x);
@@ -299,8 +300,13 @@ int matvec(double alpha, const fpc_StructMatrix_s *A, const double *x, double be
 int residual_norm_partials(const fpc_StructMatrix_s *A, const double *b, const double *x, double *r, double *part) {
   Dims d = dims_of(A->grid);
   // the hot copy on boxes the z-marching kernels take; latency-bound small boxes the full planes
-  const Planes<double> P = tile_worthwhile(d) ? planes64(A) : Planes<double>{A->coef, A->dinv, A->pitch, ZMap()};
-  k_residual<<<red_blocks(d.N), 256, 0, stream()>>>(P.c, P.pitch, P.zm, d.nx, d.ny, d.nz, b, x, r, part);
+  if (A->zmap.pl && tile_worthwhile(d)) {
+    const Planes<double> P = planes64(A);
+    k_residual<ZMap><<<red_blocks(d.N), 256, 0, stream()>>>(P.c, P.pitch, P.zm, d.nx, d.ny, d.nz, b, x, r, part);
+  } else {
+    k_residual<FullMap><<<red_blocks(d.N), 256, 0, stream()>>>(A->coef, A->pitch, FullMap(), d.nx, d.ny, d.nz, b, x, r,
+                                                               part);
+  }
   FPC_LAUNCH_CHECK();
   return 0;
 }
