@@ -261,18 +261,19 @@ __device__ inline const double *lev_b(const BLev &L, const BSys &s, int in_mode,
   return L.lvl == 0 ? (in_mode == 0 ? s.V + (long long)step * s.vp : s.W) : L.b;
 }
 
-__global__ void __launch_bounds__(256) k_b_down(const BLev *lev, const BSys *sys, const int *item, const int *first,
-                                                int in_mode, int step, const int *flags) {
+// one-point levels: residual and restriction in one launch, one thread per coarse point
+__global__ void __launch_bounds__(256) k_b_down_restrict(const BLev *lev, const BSys *sys, const int *item,
+                                                         const int *first, int in_mode, int step, const int *flags) {
   const int q = item[blockIdx.x];
   const BLev &L = lev[q];
   const BSys &s = sys[L.sys];
   if (lev_skip(L, s, flags)) return;
   const long long t = (long long)(blockIdx.x - first[q]) * 256 + threadIdx.x;
-  if (t >= L.N) return;
+  if (t >= L.Nc) return;
   int i, j, k;
-  ijk_of(t, L.nx, L.ny, i, j, k);
-  L.r[t] = mgd::down_point<double>(L.fc, L.fpitch, FullMap(), L.nx, L.ny, L.nz, L.fdinv, lev_b(L, s, in_mode, step),
-                                   L.w, i, j, k, t);
+  ijk_of(t, L.cnx, L.cny, i, j, k);
+  L.bc[t] = mgd::down_restrict_point<double>(L.fc, L.fpitch, FullMap(), L.nx, L.ny, L.nz, L.fdinv,
+                                             lev_b(L, s, in_mode, step), L.w, L.fwm, L.fwp, L.cdir, i, j, k);
 }
 
 __global__ void __launch_bounds__(256) k_b_restrict(const BLev *lev, const BSys *sys, const int *item, const int *first,
@@ -471,7 +472,8 @@ class BatchGroup {
   // V-cycle: per level l, the items (fine levels l of systems deeper than l+1)
   std::vector<BLev> hlev;
   BLev *dlev = nullptr;
-  std::vector<Table> down, restr;  // one-point down/up, restriction
+  std::vector<Table> down, dres, restr;  // one-point up (fine points), one-point down + restriction
+                                         // (coarse points), restriction of tiled x/y-coarsened levels
   std::vector<TileTable> downT;    // z-marching down/up
   Table coarse;
   int *flags = nullptr;
@@ -492,6 +494,7 @@ class BatchGroup {
     free_table(pts);
     free_tiles(ptsT);
     for (auto &t : down) free_table(t);
+    for (auto &t : dres) free_table(t);
     for (auto &t : downT) free_tiles(t);
     for (auto &t : restr) free_table(t);
     free_table(coarse);
@@ -529,10 +532,10 @@ class BatchGroup {
   int vcycle(int in_mode, int step, const int *fl) {
     const int nl = (int)down.size();
     for (int l = 0; l < nl; ++l) {
-      const Table &d = down[(size_t)l], &r = restr[(size_t)l];
+      const Table &dr = dres[(size_t)l], &r = restr[(size_t)l];
       const TileTable &dt = downT[(size_t)l];
-      if (d.nblocks) {
-        k_b_down<<<d.nblocks, 256, 0, stream()>>>(dlev, dsys, d.item, d.first, in_mode, step, fl);
+      if (dr.nblocks) {
+        k_b_down_restrict<<<dr.nblocks, 256, 0, stream()>>>(dlev, dsys, dr.item, dr.first, in_mode, step, fl);
         FPC_LAUNCH_CHECK();
       }
       if (dt.nblocks) {
@@ -596,6 +599,7 @@ class BatchGroup {
     free_table(pts);
     free_tiles(ptsT);
     for (auto &t : down) free_table(t);
+    for (auto &t : dres) free_table(t);
     for (auto &t : restr) free_table(t);
     for (auto &t : downT) free_tiles(t);
     tiles = cfg;
@@ -618,13 +622,15 @@ class BatchGroup {
     if (rc || pc != 3) return rc;
     const int maxl = *std::max_element(nlev.begin(), nlev.end());
     down.assign((size_t)std::max(0, maxl - 1), Table{});
+    dres.assign(down.size(), Table{});
     restr.assign(down.size(), Table{});
     downT.assign(down.size(), TileTable{});
     for (int l = 0; l + 1 < maxl && !rc; ++l) {
-      // fine level l of every system with a level below it: one-point or tiled;
-      // the tiled z-coarsened ones restrict inside their down kernel
+      // fine level l of every system with a level below it: one-point (residual and
+      // restriction in one launch over the coarse points; the up-sweep over the
+      // fine ones) or tiled (the z-coarsened ones restrict inside their down kernel)
       std::vector<int> onel, tl, rit, lnx, lny, lnz;
-      std::vector<long long> fine, crs;
+      std::vector<long long> fine, coarse_one, crs;
       for (int q = 0; q < n; ++q) {
         if (nlev[(size_t)q] <= l + 1) continue;
         const int it = base[(size_t)q] + l;
@@ -638,14 +644,16 @@ class BatchGroup {
         } else {
           onel.push_back(it);
           fine.push_back(e.N);
+          coarse_one.push_back(e.Nc);
         }
-        if (!tiled || e.cdir != 2) {
+        if (tiled && e.cdir != 2) {
           rit.push_back(it);
           crs.push_back(e.Nc);
         }
       }
       const int nids = (int)hlev.size();
       rc = make_table(onel, fine, 256, nids, down[(size_t)l]);
+      if (!rc) rc = make_table(onel, coarse_one, 256, nids, dres[(size_t)l]);
       if (!rc) rc = make_tile_table(tl, lnx, lny, lnz, cfg.kz, downT[(size_t)l]);
       if (!rc) rc = make_table(rit, crs, 256, nids, restr[(size_t)l]);
     }

@@ -70,6 +70,30 @@ __device__ inline T restrict_point(int fnx, int fny, int fnz, int d, const T *__
   return add_if(hp, v, wm[zm(fp, hp ? fk + dk : fk)], r[fp]);
 }
 
+// The same restriction with the fine residuals (down_point) evaluated on the fly
+// instead of read back: one launch less per level for the latency-bound small
+// levels.  Fine box nx x ny x nz; coarse point (i, j, k).  Sums as restrict_point.
+template <typename T, class M>
+__device__ inline T down_restrict_point(const T *__restrict__ c, long long pitch, M zm, int nx, int ny, int nz,
+                                        const T *__restrict__ dinv, const T *__restrict__ b, T w,
+                                        const T *__restrict__ wm, const T *__restrict__ wp, int d, int i, int j,
+                                        int k) {
+  const int di = (d == 0), dj = (d == 1), dk = (d == 2);
+  const int fi = di ? 2 * i + 1 : i, fj = dj ? 2 * j + 1 : j, fk = dk ? 2 * k + 1 : k;
+  const int pd = di ? fi : (dj ? fj : fk);
+  const int fnd = di ? nx : (dj ? ny : nz);
+  const long long stride = di ? 1 : (dj ? (long long)nx : (long long)nx * ny);
+  const long long f0 = ((long long)fk * ny + fj) * nx + fi;
+  const bool hp = pd + 1 < fnd;
+  const int pi = hp ? fi + di : fi, pj = hp ? fj + dj : fj, pk = hp ? fk + dk : fk;
+  const long long fp = hp ? f0 + stride : f0;
+  const T r0 = down_point<T>(c, pitch, zm, nx, ny, nz, dinv, b, w, fi, fj, fk, f0);
+  const T rm = down_point<T>(c, pitch, zm, nx, ny, nz, dinv, b, w, fi - di, fj - dj, fk - dk, f0 - stride);
+  const T rp = down_point<T>(c, pitch, zm, nx, ny, nz, dinv, b, w, pi, pj, pk, fp);
+  const T v = r0 + wp[zm(f0 - stride, fk - dk)] * rm;
+  return add_if(hp, v, wm[zm(fp, pk)], rp);
+}
+
 // What the interpolated value x of fine point (i, j, k) needs, the level coarsened
 // in direction D: C points (odd) inject e1; F points take wm e0 + wp e1, an
 // absent coarse neighbour reading as a zero weight and value.

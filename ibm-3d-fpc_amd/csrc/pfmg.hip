@@ -397,23 +397,22 @@ __global__ void k_prolong_add(int fnx, int fny, int fnz, int d, const T *__restr
   }
 }
 
-// Down-sweep of one level with a one-sweep zero-guess Jacobi smoother, fused:
-// x = w b dinv is recomputed on the fly at every stencil point, so only the
-// residual r = b - A x is written (x is recomputed by the up-sweep).
+// Down-sweep and restriction of a small level in one launch, one thread per
+// coarse point (fine box nx x ny x nz, coarse cnx x cny x cnz).
 template <typename T>
-__global__ void k_down_jacobi(const T *__restrict__ c, long long pitch, int nx, int ny, int nz,
-                              const T *__restrict__ dinv, const T *__restrict__ b, T *__restrict__ r, T w,
-                              const int *skip) {
+__global__ void k_down_restrict(const T *__restrict__ c, long long pitch, int nx, int ny, int nz,
+                                const T *__restrict__ dinv, const T *__restrict__ b, const T *__restrict__ wm,
+                                const T *__restrict__ wp, int d, T w, int cnx, int cny, T *__restrict__ bc,
+                                const int *skip) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y;
   int k = blockIdx.z;
-  if (i >= nx || j >= ny) return;
-  long long t = ((long long)k * ny + j) * nx + i;
+  if (i >= cnx || j >= cny) return;
   const int sk = skip_flag(skip);
-  const T v = mgd::down_point<T>(c, pitch, FullMap(), nx, ny, nz, dinv, b, w, i, j, k, t);
+  const T v = mgd::down_restrict_point<T>(c, pitch, FullMap(), nx, ny, nz, dinv, b, w, wm, wp, d, i, j, k);
   pin(v);
   if (sk) return;
-  r[t] = v;
+  bc[((long long)k * cny + j) * cnx + i] = v;
 }
 
 template <typename T, int D>
@@ -828,18 +827,25 @@ int vcycle(PfmgImpl *P, const double *in, double *out, const int *skip) {
         FPC_LAUNCH_CHECK();
         continue;
       }
+      dim3 gc, bc;
+      launch_shape(C.d, gc, bc);
+      if (!tile_worthwhile(F.d)) {
+        // small level: residual and restriction in one launch (the fine residual is not stored)
+        FPC_PROF_BEGIN(cls);
+        k_down_restrict<T><<<gc, bc, 0, st>>>(c, F.pitch, F.d.nx, F.d.ny, F.d.nz, dv, bof(l), (const T *)F.wm,
+                                              (const T *)F.wp, F.cdir, w_of(l), C.d.nx, C.d.ny, (T *)C.b, skip);
+        FPC_PROF_END(cls, ((double)F.d.N * 10.0 + (double)C.d.N) * sz);
+        FPC_LAUNCH_CHECK();
+        continue;
+      }
       FPC_PROF_BEGIN(cls);
-      if (tile_worthwhile(F.d)) {
+      {
         TileLaunch tl = tile_launch<8>(F.d);
         k_down_tiled<T, 8><<<tl.grid, tl.block, 0, st>>>(hc, H.pitch, H.zm, F.d.nx, F.d.ny, F.d.nz, hd, bof(l),
                                                           (T *)F.r, w_of(l), tl.kz, skip);
-      } else {
-        k_down_jacobi<T><<<g, bl, 0, st>>>(c, F.pitch, F.d.nx, F.d.ny, F.d.nz, dv, bof(l), (T *)F.r, w_of(l), skip);
       }
       FPC_PROF_END(cls, (rows * 8.0 + (double)F.d.N * 2.0) * sz);
       FPC_LAUNCH_CHECK();
-      dim3 gc, bc;
-      launch_shape(C.d, gc, bc);
       {
         // bytes: r at every fine point, the weights at the F points, b of each coarse point
         FPC_PROF_BEGIN("mg_restrict");
