@@ -151,7 +151,7 @@ inline TileTuning &tile_tuning() {
   static TileTuning t = [] {
     TileTuning x;
     const char *e = std::getenv("FPC_TILE_MIN_POINTS");
-    x.min_points = e ? std::max(0LL, std::atoll(e)) : (1LL << 20);
+    x.min_points = e ? std::max(0LL, std::atoll(e)) : (1LL << 19);
     e = std::getenv("FPC_STENCIL_KZ");
     x.kz = e ? std::max(0, std::atoi(e)) : 0;
     e = std::getenv("FPC_Z_EXTRUDE");

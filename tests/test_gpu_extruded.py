@@ -33,7 +33,7 @@ pytestmark = pytest.mark.gpu
 def restore_tuning(torch_cuda):
     yield
     F.set_tuning("z_extrude", 1)
-    F.set_tuning("tile_min_points", 1 << 20)
+    F.set_tuning("tile_min_points", 1 << 19)
     F.set_tuning("stencil_kz", 0)
 
 

@@ -25,7 +25,7 @@ def tiled_everywhere(request, torch_cuda):
     F.set_tuning("tile_min_points", 0)
     F.set_tuning("stencil_kz", request.param)
     yield
-    F.set_tuning("tile_min_points", 1 << 20)
+    F.set_tuning("tile_min_points", 1 << 19)
     F.set_tuning("stencil_kz", 0)
 
 
