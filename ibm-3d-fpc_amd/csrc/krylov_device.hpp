@@ -150,6 +150,8 @@ __device__ inline void orth_update_body(double *__restrict__ V, long long vp, lo
   long long t = bid * (long long)blockDim.x + threadIdx.x;
   for (; t + stride < n; t += 2 * stride) {
     double va = w[t], vb = w[t + stride];
+    // eight basis vectors' loads in flight per trip (the sums stay in order q = 0..cur)
+#pragma unroll 8
     for (int q = 0; q <= cur; ++q) {
       const double *vq = V + (long long)q * vp;
       va -= coef[q] * vq[t];

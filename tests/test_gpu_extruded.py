@@ -148,3 +148,26 @@ def test_simulation_steps_equal_full_storage():
         out.append({k: sim.get(k) for k in ("u", "v", "w", "p")})
     for k in ("u", "v", "w", "p"):
         assert bits_equal(out[0][k], out[1][k]), k
+
+
+def test_new_values_replace_the_hot_copies():
+    """Values set again after assembly drop the copies; the next solve plans them anew
+    (an extruded operator replaced by one that is not, and back) and matches a fresh box."""
+    nx, ny, nz = 96, 40, 21
+    F.set_tuning("tile_min_points", 0)
+    a1 = _extruded(nx, ny, nz, (1.0, 1.0, 4.0), seed=31, flagged=0.002)
+    a2 = _aniso(nx, ny, nz, (1.0, 1.0, 4.0), seed=32)
+    b = np.random.default_rng(33).standard_normal(nx * ny * nz)
+    box = F.StructBox(nx, ny, nz, a1)
+    sol = F.Solver(box, tol=1e-11, maxit=300, kdim=30, precond="pfmg")
+    sol.solve(b)
+    for vals, extruded in ((a2, False), (a1, True)):
+        vals = np.ascontiguousarray(vals)
+        F.check(F.lib().fpc_SetMatrix3D(box.A, nx, ny, nz, vals.ctypes.data), "SetMatrix3D")
+        x, its, _, rc = sol.solve(b)
+        fresh = F.StructBox(nx, ny, nz, vals)
+        x_f, its_f, _, rc_f = F.Solver(fresh, tol=1e-11, maxit=300, kdim=30, precond="pfmg").solve(b)
+        assert rc == 0 and rc_f == 0 and its == its_f
+        assert bits_equal(x, x_f)
+        assert box.extruded() == fresh.extruded()
+        assert (box.extruded()[0] > 0) == extruded
