@@ -176,32 +176,47 @@ __device__ inline void orth_update_body(double *__restrict__ V, long long vp, lo
 // tail); one block of 256, nred partials.
 __device__ inline void arnoldi_tail_body(State *S, const double *__restrict__ part, int nred) {
   __shared__ double lds[4];
+  // The Hessenberg column and the rotations are staged in LDS by all threads
+  // while the norm is reduced, then rotated there by thread 0: the same
+  // operations in the same order as on the State, without a chain of dependent
+  // global loads.
+  __shared__ double hc[KM + 2], cs[KM + 1], sn[KM + 1];
+  const int cur = S->st.i;  // HYPRE's i - 1
+  const int i = cur + 1;
+  for (int q = threadIdx.x; q < i; q += blockDim.x) hc[q] = H(S, q, cur);
+  for (int q = threadIdx.x; q < cur; q += blockDim.x) {
+    cs[q] = S->c[q];
+    sn[q] = S->s[q];
+  }
   double acc = 0.0;
   for (int t = threadIdx.x; t < nred; t += blockDim.x) acc += part[t];
-  double tot = block_sum<256>(acc, lds);
-  if (threadIdx.x != 0) return;
-  int cur = S->st.i;  // HYPRE's i - 1
-  int i = cur + 1;
-  double t = sqrt(tot);
-  H(S, i, cur) = t;
-  S->sigma[i] = (t != 0.0) ? 1.0 / t : 1.0;
-  for (int j = 1; j < i; ++j) {
-    double tt = H(S, j - 1, cur);
-    H(S, j - 1, cur) = S->s[j - 1] * H(S, j, cur) + S->c[j - 1] * tt;
-    H(S, j, cur) = -S->s[j - 1] * tt + S->c[j - 1] * H(S, j, cur);
+  double tot = block_sum<256>(acc, lds);  // its barriers also publish the staging
+  if (threadIdx.x == 0) {
+    double t = sqrt(tot);
+    hc[i] = t;
+    S->sigma[i] = (t != 0.0) ? 1.0 / t : 1.0;
+    for (int j = 1; j < i; ++j) {
+      double tt = hc[j - 1];
+      hc[j - 1] = sn[j - 1] * hc[j] + cs[j - 1] * tt;
+      hc[j] = -sn[j - 1] * tt + cs[j - 1] * hc[j];
+    }
+    double g = sqrt(hc[i] * hc[i] + hc[cur] * hc[cur]);
+    if (g == 0.0) g = 2.220446049250313e-16;
+    const double c = hc[cur] / g, s = hc[i] / g;
+    S->c[cur] = c;
+    S->s[cur] = s;
+    const double rsc = S->rs[cur];
+    S->rs[i] = -hc[i] * rsc / g;
+    S->rs[cur] = c * rsc;
+    hc[cur] = s * hc[i] + c * hc[cur];
+    S->st.r_norm = fabs(S->rs[i]);
+    S->st.i = i;
+    S->st.iter += 1;
+    if (S->st.r_norm <= S->st.eps) S->st.done = 1;
+    else if (i >= S->st.kdim || S->st.iter >= S->st.maxit) S->st.done = 2;
   }
-  double g = sqrt(H(S, i, cur) * H(S, i, cur) + H(S, cur, cur) * H(S, cur, cur));
-  if (g == 0.0) g = 2.220446049250313e-16;
-  S->c[cur] = H(S, cur, cur) / g;
-  S->s[cur] = H(S, i, cur) / g;
-  S->rs[i] = -H(S, i, cur) * S->rs[cur] / g;
-  S->rs[cur] = S->c[cur] * S->rs[cur];
-  H(S, cur, cur) = S->s[cur] * H(S, i, cur) + S->c[cur] * H(S, cur, cur);
-  S->st.r_norm = fabs(S->rs[i]);
-  S->st.i = i;
-  S->st.iter += 1;
-  if (S->st.r_norm <= S->st.eps) S->st.done = 1;
-  else if (i >= S->st.kdim || S->st.iter >= S->st.maxit) S->st.done = 2;
+  __syncthreads();
+  for (int q = threadIdx.x; q <= i; q += blockDim.x) H(S, q, cur) = hc[q];
 }
 
 // Upper-triangular solve; y_j = sigma_j * rs_j (coefficients of the stored p_j). One thread.
