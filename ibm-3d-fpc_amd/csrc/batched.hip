@@ -180,7 +180,6 @@ __global__ void __launch_bounds__(256) k_b_multidot(const BSys *sys) {
 
 __global__ void __launch_bounds__(256) k_b_reduce_h(const BSys *sys) {
   const BSys &s = sys[blockIdx.y];
-  if (s.S->st.done) return;
   kry::reduce_h_body(s.S, s.part, s.nred, blockIdx.x);
 }
 
@@ -192,7 +191,6 @@ __global__ void __launch_bounds__(256) k_b_orth_update(const BSys *sys) {
 
 __global__ void __launch_bounds__(256) k_b_tail(const BSys *sys) {
   const BSys &s = sys[blockIdx.x];
-  if (s.S->st.done) return;
   kry::arnoldi_tail_body(s.S, s.part, s.nred);
 }
 

@@ -146,6 +146,13 @@ struct Stencil {
   int nx, ny, nz;
 };
 
+// No instruction: v is computed before what follows, so loads it depends on are
+// not sunk below a later early return (a status test after the loads).
+template <typename T>
+__device__ inline void pin(T v) {
+  asm volatile("" ::"v"(v));
+}
+
 // Deterministic block reduction of one double per thread (wave64 shuffles + LDS).
 template <int BS>
 __device__ inline double block_sum(double v, double *lds) {

@@ -107,7 +107,6 @@ __global__ void __launch_bounds__(256) k_multidot(const double *__restrict__ V, 
 
 // h_{j,cur} = sigma_j * sum_b part[j][b]; one block per j.
 __global__ void __launch_bounds__(256) k_reduce_h(State *S, const double *__restrict__ part, int nred) {
-  if (S->st.done) return;
   reduce_h_body(S, part, nred, blockIdx.x);
 }
 
@@ -120,7 +119,6 @@ __global__ void __launch_bounds__(256) k_orth_update(double *__restrict__ V, lon
 
 // Norm of the new vector, Givens update, convergence flag (HYPRE's Arnoldi tail).
 __global__ void __launch_bounds__(256) k_arnoldi_tail(State *S, const double *__restrict__ part, int nred) {
-  if (S->st.done) return;
   arnoldi_tail_body(S, part, nred);
 }
 

@@ -126,12 +126,27 @@ __device__ inline void multidot_body(const double *__restrict__ V, long long vp,
 }
 
 // h_{jv,cur} = sigma_jv * sum of the nred partials of row jv (one block of 256).
+// Returns at once, for the whole block, once the solve is done or j > the current index.
 __device__ inline void reduce_h_body(State *S, const double *__restrict__ part, int nred, int jv) {
   __shared__ double lds[4];
-  int cur = S->st.i;
-  if (jv > cur) return;
+  // the partials' addresses depend on neither the status nor the basis index:
+  // the status (vector loads) and all NB / 256 partials of a thread are loaded
+  // together (an index past nred reads entry 0 of the row, and its term is
+  // skipped), summed in order, and only then is the status tested
+  const int done = __hip_atomic_load(&S->st.done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int cur = __hip_atomic_load(&S->st.i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const double *row = part + (long long)jv * NB;
+  double v[NB / 256];
+#pragma unroll
+  for (int r = 0; r < NB / 256; ++r) {
+    const int t = (int)threadIdx.x + 256 * r;
+    v[r] = row[t < nred ? t : 0];
+  }
   double acc = 0.0;
-  for (int t = threadIdx.x; t < nred; t += blockDim.x) acc += part[(long long)jv * NB + t];
+#pragma unroll
+  for (int r = 0; r < NB / 256; ++r) acc = ((int)threadIdx.x + 256 * r < nred) ? acc + v[r] : acc;
+  pin(acc);
+  if (done || jv > cur) return;
   double tot = block_sum<256>(acc, lds);
   if (threadIdx.x == 0) H(S, jv, cur) = S->sigma[jv] * tot;
 }
@@ -174,22 +189,33 @@ __device__ inline void orth_update_body(double *__restrict__ V, long long vp, lo
 
 // Norm of the new vector, Givens update, convergence flag (HYPRE's Arnoldi
 // tail); one block of 256, nred partials.
+// Returns at once, for the whole block, once the solve is done.
 __device__ inline void arnoldi_tail_body(State *S, const double *__restrict__ part, int nred) {
   __shared__ double lds[4];
   // The Hessenberg column and the rotations are staged in LDS by all threads
   // while the norm is reduced, then rotated there by thread 0: the same
   // operations in the same order as on the State, without a chain of dependent
-  // global loads.
+  // global loads.  The status is loaded with the partials and tested after.
   __shared__ double hc[KM + 2], cs[KM + 1], sn[KM + 1];
-  const int cur = S->st.i;  // HYPRE's i - 1
+  const int done = __hip_atomic_load(&S->st.done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int cur = __hip_atomic_load(&S->st.i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // HYPRE's i - 1
+  double v[NB / 256];
+#pragma unroll
+  for (int r = 0; r < NB / 256; ++r) {
+    const int t = (int)threadIdx.x + 256 * r;
+    v[r] = part[t < nred ? t : 0];
+  }
+  double acc = 0.0;
+#pragma unroll
+  for (int r = 0; r < NB / 256; ++r) acc = ((int)threadIdx.x + 256 * r < nred) ? acc + v[r] : acc;
+  pin(acc);
+  if (done) return;
   const int i = cur + 1;
   for (int q = threadIdx.x; q < i; q += blockDim.x) hc[q] = H(S, q, cur);
   for (int q = threadIdx.x; q < cur; q += blockDim.x) {
     cs[q] = S->c[q];
     sn[q] = S->s[q];
   }
-  double acc = 0.0;
-  for (int t = threadIdx.x; t < nred; t += blockDim.x) acc += part[t];
   double tot = block_sum<256>(acc, lds);  // its barriers also publish the staging
   if (threadIdx.x == 0) {
     double t = sqrt(tot);

@@ -272,11 +272,6 @@ __device__ int g_no_skip = 0;
 __device__ inline int skip_flag(const int *skip) {
   return __hip_atomic_load(skip ? skip : &g_no_skip, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// No instruction: makes v be computed before what follows (the skip test).
-template <typename T>
-__device__ inline void pin(T v) {
-  asm volatile("" ::"v"(v));
-}
 
 template <typename T>
 __global__ void k_relax_zero(const T *__restrict__ dinv, const T *__restrict__ b, T *__restrict__ x, T w, long long n,
