@@ -141,3 +141,37 @@ def test_mixed_precision_close_to_fp64(torch_cuda):
     for f in ("u", "v", "w"):
         assert rel_l2(b.get(f), a.get(f)) < 1e-6, f
     assert abs(a.scalar("cd") - b.scalar("cd")) <= 1e-6 * abs(a.scalar("cd"))
+
+
+def test_nonfinite_check_flags_every_position(torch_cuda):
+    """The device check behind the 911 / 912 exits (main.cu:301-305, 333-336): one
+    non-finite value anywhere in a 1M-point field is found; a finite field is not flagged."""
+    import ctypes
+    import torch
+    n = 1 << 20
+    for pos, val in ((None, 0.0), (0, np.inf), (n // 2, -np.inf), (n - 1, np.nan)):
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        if pos is not None:
+            x[pos] = val
+        torch.cuda.synchronize()
+        f = ctypes.c_int(-1)
+        F.check(F.lib().fpc_any_nonfinite(ctypes.c_void_p(x.data_ptr()), n, ctypes.byref(f)), "any_nonfinite")
+        assert f.value == (0 if pos is None else 1), pos
+
+
+def test_nonfinite_state_fails_the_momentum_solve(torch_cuda):
+    """A non-finite velocity makes the momentum right-hand side non-finite.  The solve
+    then returns the generic error (code 1), as HYPRE's GMRES does for such input and
+    HYPRE_SOLVE_CALL aborts with (common.hpp:36-43); 911 / 912 are for solves that
+    return with non-finite output."""
+    kw = dict(TINY)
+    sim = F.Sim(F.default_params(**kw))
+    sim.step(1)
+    nx, ny, nz = kw["nx"], kw["ny"], kw["nz"]
+    act = O.Setup(O.default_params(**kw)).get("active_u").reshape(nz, ny, nx + 1).astype(bool)
+    i = next(i for i in range(nx // 2, 0, -1) if act[nz // 2, ny // 2, i])  # an interior face in the flow
+    u = sim.get("u")
+    u[((nz // 2) * ny + ny // 2) * (nx + 1) + i] = np.nan
+    sim.set("u", u)
+    with pytest.raises(F.FpcError, match="code 1:"):
+        sim.step(1)
