@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include "fpc_dense.h"
+#include "fpc_internal.hpp"
 
 namespace fpc {
 
@@ -96,21 +97,47 @@ __device__ inline double gradz_at(const double *p, const u8 *fluid, int i, int j
   return s;
 }
 
-// D q at a fluid cell in CSR column order; qout supplies the outflow faces when non-null.
-__device__ inline double div_at(const fpc_flow_ops &o, const double *u, const double *v, const double *w,
-                                const double *qout, int i, int j, int k) {
-  int nx = o.nx, ny = o.ny, nz = o.nz;
+// D q at a fluid cell in CSR column order (-x, +x, -y, +y, -z, +z terms, a term skipped
+// where its coefficient is zero or its face is a wall); qout supplies the outflow faces
+// when non-null.  Two steps: div_loads reads every value of cell (i, j, k) at an address
+// that is always in the arrays (faces i and i+1, j and j+1, k and k+1 exist for every
+// cell), div_eval sums them under the tests.  Between the two the caller pins the loads
+// so that all of them are in flight before the first wait.
+struct DivLoads {
+  double c[6], x[6], q;  // q: the outflow value of the cell's row (qout given), else 0
+  bool hq;
+};
+
+__device__ inline DivLoads div_loads(const fpc_flow_ops &o, const double *u, const double *v, const double *w,
+                                     const double *qout, int i, int j, int k) {
+  const int nx = o.nx, ny = o.ny;
   const double *c = o.dcoef + 6 * ((long long)j * nx + i);
+  DivLoads L;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) L.c[q] = c[q];
+  L.x[0] = u[iU(i, j, k, nx, ny)];
+  L.x[1] = u[iU(i + 1, j, k, nx, ny)];
+  L.hq = qout != nullptr;
+  L.q = L.hq ? qout[(long long)j + (long long)ny * k] : 0.0;
+  L.x[2] = v[iV(i, j, k, nx, ny)];
+  L.x[3] = v[iV(i, j + 1, k, nx, ny)];
+  L.x[4] = w[iW(i, j, k, nx, ny)];
+  L.x[5] = w[iW(i, j, k + 1, nx, ny)];
+  return L;
+}
+
+__device__ inline void pin_div(const DivLoads &L) {
+  pin_all(L.c[0], L.c[1], L.c[2], L.c[3], L.c[4], L.c[5], L.x[0], L.x[1], L.x[2], L.x[3], L.x[4], L.x[5], L.q);
+}
+
+__device__ inline double div_eval(const DivLoads &L, int i, int j, int k, int nx, int ny, int nz) {
   double s = 0.0;
-  if (i >= 1 && c[0] != 0.0) s += c[0] * u[iU(i, j, k, nx, ny)];
-  if (c[1] != 0.0) {
-    double qe = (qout && i == nx - 1) ? qout[(long long)j + (long long)ny * k] : u[iU(i + 1, j, k, nx, ny)];
-    s += c[1] * qe;
-  }
-  if (j >= 1 && c[2] != 0.0) s += c[2] * v[iV(i, j, k, nx, ny)];
-  if (j <= ny - 2 && c[3] != 0.0) s += c[3] * v[iV(i, j + 1, k, nx, ny)];
-  if (k >= 1 && c[4] != 0.0) s += c[4] * w[iW(i, j, k, nx, ny)];
-  if (k <= nz - 2 && c[5] != 0.0) s += c[5] * w[iW(i, j, k + 1, nx, ny)];
+  if (i >= 1 && L.c[0] != 0.0) s += L.c[0] * L.x[0];
+  if (L.c[1] != 0.0) s += L.c[1] * ((L.hq && i == nx - 1) ? L.q : L.x[1]);
+  if (j >= 1 && L.c[2] != 0.0) s += L.c[2] * L.x[2];
+  if (j <= ny - 2 && L.c[3] != 0.0) s += L.c[3] * L.x[3];
+  if (k >= 1 && L.c[4] != 0.0) s += L.c[4] * L.x[4];
+  if (k <= nz - 2 && L.c[5] != 0.0) s += L.c[5] * L.x[5];
   return s;
 }
 
@@ -217,10 +244,15 @@ __device__ inline void explicit_row(int comp, const fpc_flow_ops &o, const doubl
 __device__ inline void divergence_cell(const fpc_flow_ops &o, const double *ut, const double *vt, const double *wt,
                                        double ramp, double *rhs, double *x0, int i, int j, int k) {
   long long c = iP(i, j, k, o.nx, o.ny);
+  const int fluid = o.fluid_p[c];
+  const DivLoads L = div_loads(o, ut, vt, wt, nullptr, i, j, k);
+  const double bcv = o.div_bc[(long long)j + (long long)o.ny * k];
+  pin_div(L);
+  pin_all(fluid, bcv);
   double r = 0.0;
-  if (o.fluid_p[c]) {
-    double s = div_at(o, ut, vt, wt, nullptr, i, j, k);
-    double bc = (i == 0) ? o.div_bc[(long long)j + (long long)o.ny * k] : 0.0;
+  if (fluid) {
+    double s = div_eval(L, i, j, k, o.nx, o.ny, o.nz);
+    double bc = (i == 0) ? bcv : 0.0;
     r = -(o.rho / o.dt) * (s + ramp * bc);
   }
   rhs[c] = r;
@@ -268,31 +300,54 @@ __device__ inline void project_point(const fpc_flow_ops &o, long long tid, const
   }
 }
 
+// Blocks of the diagnostics of an nx x ny x nz box: one per 2048 entries of the longest
+// face array, at most kDiagBlocks.  The partition -- and so the bits of the sums --
+// depends on the box only, for one simulation and an ensemble's instances alike.
+constexpr int kDiagBlocks = 2048;
+__host__ __device__ inline int diag_blocks(int nx, int ny, int nz) {
+  const long long Nu = (long long)(nx + 1) * ny * nz, Nv = (long long)nx * (ny + 1) * nz,
+                  Nw = (long long)nx * ny * (nz + 1);
+  const long long Nm = Nu > Nv ? (Nu > Nw ? Nu : Nw) : (Nv > Nw ? Nv : Nw);
+  const long long b = (Nm + 2047) / 2048;
+  return (int)(b < 1 ? 1 : (b > kDiagBlocks ? kDiagBlocks : b));
+}
+
 // Block bid of nblk of the diagnostics (main.cu:352-363 on the device): partial sum of the
 // squared divergence, max |u|, |v|, |w|, partial forces; part[q * nblk + bid].
 __device__ inline void diag_partials_body(const fpc_flow_ops &o, const double *u, const double *v, const double *w,
                                           const double *p, const double *qout, double ramp, double *part, int bid,
                                           int nblk) {
   __shared__ double lds[8];
-  int nx = o.nx, ny = o.ny, nz = o.nz;
-  long long Np = (long long)nx * ny * nz;
-  long long Nu = (long long)(nx + 1) * ny * nz, Nv = (long long)nx * (ny + 1) * nz, Nw = (long long)nx * ny * (nz + 1);
-  long long stride = (long long)nblk * blockDim.x, t0 = bid * (long long)blockDim.x + threadIdx.x;
+  const int nx = o.nx, ny = o.ny, nz = o.nz;
+  const long long Np = (long long)nx * ny * nz, pl = (long long)nx * ny;
+  const long long Nu = (long long)(nx + 1) * ny * nz, Nv = (long long)nx * (ny + 1) * nz, Nw = pl * (nz + 1);
+  const long long Nm = Nu > Nv ? (Nu > Nw ? Nu : Nw) : (Nv > Nw ? Nv : Nw);
+  const long long stride = (long long)nblk * blockDim.x, t0 = bid * (long long)blockDim.x + threadIdx.x;
+  // cell coordinates of t, walked by the stride's own (i, j, k)
+  const int si = (int)(stride % nx), sj = (int)((stride / nx) % ny), sk = (int)(stride / pl);
+  int i = (int)(t0 % nx), j = (int)((t0 / nx) % ny), k = (int)(t0 / pl);
   double d2 = 0.0, mu = 0.0, mv = 0.0, mw = 0.0, fx = 0.0, fy = 0.0;
-  for (long long c = t0; c < Np; c += stride) {
-    if (!o.fluid_p[c]) continue;
-    int k = (int)(c / ((long long)nx * ny));
-    long long rem = c - (long long)k * nx * ny;
-    int j = (int)(rem / nx);
-    int i = (int)(rem - (long long)j * nx);
-    double s = div_at(o, u, v, w, qout, i, j, k);
-    double bc = (i == 0) ? o.div_bc[(long long)j + (long long)ny * k] : 0.0;
-    double d = s + ramp * bc;
-    d2 += d * d;
+  // One pass: cell t (t < Np) and face t of each component (max |.| is exact in any order).
+  for (long long t = t0; t < Nm; t += stride) {
+    const bool cell = t < Np;
+    const int ci = cell ? i : 0, cj = cell ? j : 0, ck = cell ? k : 0;
+    const int fluid = o.fluid_p[cell ? t : 0];
+    const DivLoads L = div_loads(o, u, v, w, qout, ci, cj, ck);
+    const double bcv = o.div_bc[(long long)cj + (long long)ny * ck];
+    const double au = u[t < Nu ? t : 0], av = v[t < Nv ? t : 0], aw = w[t < Nw ? t : 0];
+    pin_div(L);
+    pin_all(fluid, bcv, au, av, aw);
+    if (cell && fluid) {
+      const double s = div_eval(L, ci, cj, ck, nx, ny, nz);
+      const double bc = (ci == 0) ? bcv : 0.0;
+      const double d = s + ramp * bc;
+      d2 += d * d;
+    }
+    if (t < Nu) mu = fmax(mu, fabs(au));
+    if (t < Nv) mv = fmax(mv, fabs(av));
+    if (t < Nw) mw = fmax(mw, fabs(aw));
+    ijk_advance(i, j, k, si, sj, sk, nx, ny);
   }
-  for (long long t = t0; t < Nu; t += stride) mu = fmax(mu, fabs(u[t]));
-  for (long long t = t0; t < Nv; t += stride) mv = fmax(mv, fabs(v[t]));
-  for (long long t = t0; t < Nw; t += stride) mw = fmax(mw, fabs(w[t]));
   for (long long t = t0; t < o.n_fu; t += stride) fx += o.fu_w[t] * u[o.fu_idx[t]];
   for (long long t = t0; t < o.n_px; t += stride) fx += o.px_w[t] * p[o.px_idx[t]];
   for (long long t = t0; t < o.n_fv; t += stride) fy += o.fv_w[t] * v[o.fv_idx[t]];

@@ -422,8 +422,8 @@ int stage_projection_async(const fpc_flow_ops *o, const double *phi, const doubl
 
 int stage_diagnostics_async(const fpc_flow_ops *o, const double *u, const double *v, const double *w, const double *p,
                             const double *qout, double ramp, int Np, double *out) {
-  const int NBD = 512;
-  double *part = scratch_doubles((size_t)NBD * 6, 6);
+  const int NBD = diag_blocks(o->nx, o->ny, o->nz);
+  double *part = scratch_doubles((size_t)kDiagBlocks * 6, 6);
   if (!part) return FPC_ERROR_MEMORY;
   k_diag_partials<<<NBD, 256, 0, stream()>>>(*o, u, v, w, p, qout, ramp, part);
   FPC_LAUNCH_CHECK();

@@ -140,7 +140,6 @@ struct BInst {
   long long ncell;   // nx * ny * nz
 };
 
-constexpr int kDiagBlocks = 512;  // the partition of stage_diagnostics_async
 
 struct BTable {
   int *item = nullptr, *first = nullptr;
@@ -153,6 +152,7 @@ struct BStage {
   BTable inlet, faces, rows, scat, cells;
   double *ramp = nullptr, *hramp = nullptr;     // per instance, device / pinned host
   double *part = nullptr;                       // n * 6 * kDiagBlocks
+  int diag_nb = 1;                              // most diagnostics blocks of an instance
   double *ddiag = nullptr, *hdiag = nullptr;    // n * 8, device / pinned host
 };
 
@@ -230,14 +230,16 @@ __global__ void kb_add_phi(const BInst *I, const int *item, const int *first) {
 
 __global__ void __launch_bounds__(256) kb_diag_partials(const BInst *I, const double *ramp, double *part) {
   const BInst &B = I[blockIdx.y];
+  const int nb = diag_blocks(B.o.nx, B.o.ny, B.o.nz);  // the partition of stage_diagnostics_async
+  if ((int)blockIdx.x >= nb) return;
   diag_partials_body(B.o, B.u, B.v, B.w, B.p, B.qout, ramp[blockIdx.y], part + (long long)blockIdx.y * 6 * kDiagBlocks,
-                     blockIdx.x, kDiagBlocks);
+                     blockIdx.x, nb);
 }
 
 __global__ void __launch_bounds__(1024) kb_diag_final(const BInst *I, const double *part, double *out) {
   const BInst &B = I[blockIdx.x];
   double *o = out + (long long)blockIdx.x * 8;
-  diag_final_body(part + (long long)blockIdx.x * 6 * kDiagBlocks, kDiagBlocks, B.Np, o);
+  diag_final_body(part + (long long)blockIdx.x * 6 * kDiagBlocks, diag_blocks(B.o.nx, B.o.ny, B.o.nz), B.Np, o);
   __syncthreads();
   if (threadIdx.x < 6) B.diag[threadIdx.x] = o[threadIdx.x];
 }
@@ -305,6 +307,7 @@ int stage_create(const std::vector<fpc_sim> &sims, BStage **out) {
     scat.push_back(Nv);
     scat.push_back(Nw);
     cells.push_back(nx * ny * nz);
+    S->diag_nb = std::max(S->diag_nb, diag_blocks((int)nx, (int)ny, (int)nz));
   }
   int rc = btable(inlet, S->inlet);
   if (!rc) rc = btable(faces, S->faces);
@@ -368,7 +371,7 @@ int stage_post(BStage *S) {
   FPC_LAUNCH_CHECK();
   kb_add_phi<<<S->cells.nblocks, 256, 0, stream()>>>(S->inst, S->cells.item, S->cells.first);
   FPC_LAUNCH_CHECK();
-  kb_diag_partials<<<dim3(kDiagBlocks, S->n), 256, 0, stream()>>>(S->inst, S->ramp, S->part);
+  kb_diag_partials<<<dim3(S->diag_nb, S->n), 256, 0, stream()>>>(S->inst, S->ramp, S->part);
   FPC_LAUNCH_CHECK();
   kb_diag_final<<<S->n, 1024, 0, stream()>>>(S->inst, S->part, S->ddiag);
   FPC_LAUNCH_CHECK();

@@ -116,6 +116,47 @@ inline void launch_shape(const Dims &d, dim3 &grid, dim3 &block) {
   grid = dim3((d.nx + tx - 1) / tx, (d.ny + ty - 1) / ty, d.nz);
 }
 
+// v unchanged, but opaque to the optimiser: an index chosen by a select stays one
+// address (x[in ? t + o : t] is not split into a branch around a second load).
+template <typename T>
+__device__ inline T opaque(T v) {
+  asm("" : "+v"(v));
+  return v;
+}
+
+// No instruction: v is computed before what follows, so loads it depends on are
+// not sunk below a later early return (a status test after the loads).
+template <typename T>
+__device__ inline void pin(T v) {
+  asm volatile("" ::"v"(v));
+}
+
+// Each value computed here (one wait for all the loads they come from): code after
+// this uses them under its own conditions without the loads being sunk into those.
+template <typename... T>
+__device__ inline void pin_all(const T &...v) {
+  (pin(v), ...);
+}
+
+// (i, j, k) += (si, sj, sk) with carries; si < nx and sj < ny, so one carry per direction at most.
+__device__ inline void ijk_advance(int &i, int &j, int &k, int si, int sj, int sk, int nx, int ny) {
+  i += si;
+  j += sj;
+  k += sk;
+  if (i >= nx) { i -= nx; ++j; }
+  if (j >= ny) { j -= ny; ++k; }
+}
+
+// A uniform value (held in scalar registers) made opaque in the same way as opaque().
+template <typename T>
+__device__ inline T opaque_s(T v) {
+  asm("" : "+s"(v));
+  return v;
+}
+
+// The flag read by points of a map without flagged points.
+static __device__ const unsigned char kNoFlags[4] = {0, 0, 0, 0};
+
 // Row of the stored planes that holds point t (on z-plane k).  pl == 0: full
 // storage, row t.  Otherwise the hot copy: a flagged point reads row xoff + t;
 // any other reads compact plane kc -- k for the m bottom planes, m for the
@@ -131,12 +172,41 @@ struct ZMap {
     const int kc = (k < m) ? k : ((k >= nz - m) ? k - nz + 2 * m + 1 : m);
     return t + (long long)(kc - k) * pl;
   }
+  // The rows of n points at once, in two steps so that other loads can go between
+  // them: flags() issues every flag load (box-wide tests once), rows() uses them.
+  // No branch: without flags every lane reads one zero byte, so the loads' wait can
+  // move down to rows() behind whatever the caller issues in between.
+  template <int n>
+  __device__ inline void flags(const long long *t, unsigned char *f) const {
+    const bool on = pl && xf;
+    typedef const __attribute__((address_space(1))) unsigned char *gptr;  // a global load, not flat
+    const gptr p = (gptr)opaque_s(on ? xf : kNoFlags);                     // not folded to a zero
+    for (int q = 0; q < n; ++q) f[q] = p[on ? t[q] : 0];
+  }
+  template <int n>
+  __device__ inline void rows(const long long *t, const int *k, const unsigned char *f, long long *u) const {
+    if (!pl) {
+      for (int q = 0; q < n; ++q) u[q] = t[q];
+      return;
+    }
+    for (int q = 0; q < n; ++q) {
+      const int kc = (k[q] < m) ? k[q] : ((k[q] >= nz - m) ? k[q] - nz + 2 * m + 1 : m);
+      const long long h = t[q] + (long long)(kc - k[q]) * pl, x = xoff + t[q];
+      u[q] = f[q] ? x : h;
+    }
+  }
 };
 
 // The full-storage map as a type: row t, no flags, no branches (the one-point
 // kernels, whose neighbour loads must issue together).
 struct FullMap {
   __host__ __device__ inline long long operator()(long long t, int) const { return t; }
+  template <int n>
+  __device__ inline void flags(const long long *, unsigned char *) const {}
+  template <int n>
+  __device__ inline void rows(const long long *t, const int *, const unsigned char *, long long *u) const {
+    for (int q = 0; q < n; ++q) u[q] = t[q];
+  }
 };
 
 template <typename T>
@@ -146,12 +216,12 @@ struct Stencil {
   int nx, ny, nz;
 };
 
-// No instruction: v is computed before what follows, so loads it depends on are
-// not sunk below a later early return (a status test after the loads).
-template <typename T>
-__device__ inline void pin(T v) {
-  asm volatile("" ::"v"(v));
-}
+// s + a v (one fused multiply-add) when `in`, else s: the fma always runs with a zeroed
+// coefficient, so a and v are used on every path and their loads cannot be sunk into a
+// branch of their own (a wait per neighbour).  Bitwise s + a v, or s, for finite v
+// (s = -0 becomes +0, which only changes the sign of a zero result).
+__device__ inline double fma_if(bool in, double s, double a, double v) { return __builtin_fma(in ? a : 0.0, v, s); }
+__device__ inline float fma_if(bool in, float s, float a, float v) { return __builtin_fmaf(in ? a : 0.0f, v, s); }
 
 // Deterministic block reduction of one double per thread (wave64 shuffles + LDS).
 template <int BS>

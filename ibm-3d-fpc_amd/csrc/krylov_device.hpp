@@ -299,33 +299,79 @@ __device__ inline void cycle_start_body(State *S, double r_norm, double eps, int
   S->sigma[0] = 1.0 / r_norm;
 }
 
+// The loads of (b - A x) at point t = (i, j, k): the vector values first (neighbours at
+// clamped indices; an absent one reads point t and is zeroed in residual_eval), then the
+// coefficients of row u.
+struct ResidualLoads {
+  double a[7], v[7], b;
+  bool in[6];
+};
+
+__device__ inline void residual_loads_x(ResidualLoads &L, int nx, int ny, int nz, long long pl,
+                                        const double *__restrict__ b, const double *__restrict__ x, long long t, int i,
+                                        int j, int k) {
+  L.in[0] = i > 0, L.in[1] = i < nx - 1, L.in[2] = j > 0, L.in[3] = j < ny - 1, L.in[4] = k > 0, L.in[5] = k < nz - 1;
+  const long long off[6] = {-1, 1, -(long long)nx, (long long)nx, -pl, pl};
+  L.v[0] = x[t];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) L.v[q + 1] = x[t + opaque(L.in[q] ? off[q] : 0LL)];
+  L.b = b[t];
+}
+
+__device__ inline void residual_loads_a(ResidualLoads &L, const double *__restrict__ c, long long pitch, long long u) {
+#pragma unroll
+  for (int q = 0; q < 7; ++q) L.a[q] = c[q * pitch + u];
+}
+
+__device__ inline double residual_eval(const ResidualLoads &L) {
+  double s = L.a[0] * L.v[0];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) s = fma_if(L.in[q], s, L.a[q + 1], L.v[q + 1]);
+  return L.b - s;
+}
+
 // r = b - A x (r optional) with the block's partial sum of r^2 into part[bid]; block bid of nblk.
+// Each thread takes the points t, t + S, t + 2S, ... (S = nblk * blockDim.x) and sums their r^2 in
+// that order; two of them per trip so that both points' loads are in flight together.  The point
+// coordinates walk by the stride's own (i, j, k) instead of being divided out of t.
 template <class M>
 __device__ inline void residual_body(const double *__restrict__ c, long long pitch, M zm, int nx, int ny, int nz,
                                      const double *__restrict__ b, const double *__restrict__ x, double *__restrict__ r,
                                      double *__restrict__ part, int bid, int nblk) {
   __shared__ double lds[8];
-  long long N = (long long)nx * ny * nz, pl = (long long)nx * ny;
+  const long long N = (long long)nx * ny * nz, pl = (long long)nx * ny;
+  const long long S = (long long)nblk * blockDim.x;
+  const int si = (int)(S % nx), sj = (int)((S / nx) % ny), sk = (int)(S / pl);
+  long long t = bid * (long long)blockDim.x + threadIdx.x;
+  int i = (int)(t % nx), j = (int)((t / nx) % ny), k = (int)(t / pl);
   double acc = 0.0;
-  for (long long t = bid * (long long)blockDim.x + threadIdx.x; t < N; t += (long long)nblk * blockDim.x) {
-    int i = (int)(t % nx);
-    long long q = t / nx;
-    int j = (int)(q % ny);
-    int k = (int)(q / ny);
-    const long long u = zm(t, k);
-    const bool xm = i > 0, xp = i < nx - 1, ym = j > 0, yp = j < ny - 1, zl = k > 0, zh = k < nz - 1;
-    const double x1 = x[xm ? t - 1 : t], x2 = x[xp ? t + 1 : t], x3 = x[ym ? t - nx : t], x4 = x[yp ? t + nx : t];
-    const double x5 = x[zl ? t - pl : t], x6 = x[zh ? t + pl : t];
-    double s = c[u] * x[t];
-    s = xm ? s + c[pitch + u] * x1 : s;
-    s = xp ? s + c[2 * pitch + u] * x2 : s;
-    s = ym ? s + c[3 * pitch + u] * x3 : s;
-    s = yp ? s + c[4 * pitch + u] * x4 : s;
-    s = zl ? s + c[5 * pitch + u] * x5 : s;
-    s = zh ? s + c[6 * pitch + u] * x6 : s;
-    double rv = b[t] - s;
-    if (r) r[t] = rv;
-    acc += rv * rv;
+  for (; t < N; t += 2 * S) {
+    int i1 = i, j1 = j, k1 = k;
+    ijk_advance(i1, j1, k1, si, sj, sk, nx, ny);
+    const bool two = t + S < N;
+    const long long t1 = two ? t + S : t;
+    if (!two) i1 = i, j1 = j, k1 = k;
+    const long long tq[2] = {t, t1};
+    const int kq[2] = {k, k1};
+    unsigned char fq[2];
+    long long uq[2];
+    ResidualLoads L0, L1;
+    zm.template flags<2>(tq, fq);
+    residual_loads_x(L0, nx, ny, nz, pl, b, x, t, i, j, k);
+    residual_loads_x(L1, nx, ny, nz, pl, b, x, t1, i1, j1, k1);
+    zm.template rows<2>(tq, kq, fq, uq);
+    residual_loads_a(L0, c, pitch, uq[0]);
+    residual_loads_a(L1, c, pitch, uq[1]);
+    __builtin_amdgcn_sched_barrier(0);  // every load of both points issued before the first wait
+    const double r0 = residual_eval(L0), r1 = residual_eval(L1);
+    if (r) r[t] = r0;
+    acc += r0 * r0;
+    if (two) {
+      if (r) r[t1] = r1;
+      acc += r1 * r1;
+    }
+    i = i1, j = j1, k = k1;
+    ijk_advance(i, j, k, si, sj, sk, nx, ny);
   }
   double tot = block_sum<256>(acc, lds);
   if (threadIdx.x == 0) part[bid] = tot;
