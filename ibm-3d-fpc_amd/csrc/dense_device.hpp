@@ -97,6 +97,146 @@ __device__ inline double gradz_at(const double *p, const u8 *fluid, int i, int j
   return s;
 }
 
+// ---- the fused stages' convection and G p in two steps: every load of a face at an
+// address inside its array (out-of-box neighbours read entry 0 and take the fallback
+// in the evaluation), then the same expressions as conv_*_at and grad*_at on the loaded
+// values -- the same bits.  Between the two the caller pins the loads so that all of
+// them are in flight before the first wait.
+__device__ inline double ld_u(const double *u, int i, int j, int k, int nx, int ny, int nz, bool &in) {
+  in = !((i < 0) | (i > nx) | (j < 0) | (j >= ny) | (k < 0) | (k >= nz));  // no short-circuit branches
+  return u[opaque(in ? iU(i, j, k, nx, ny) : 0LL)];
+}
+__device__ inline double ld_v(const double *v, int i, int j, int k, int nx, int ny, int nz, bool &in) {
+  in = !((i < 0) | (i >= nx) | (j < 0) | (j > ny) | (k < 0) | (k >= nz));
+  return v[opaque(in ? iV(i, j, k, nx, ny) : 0LL)];
+}
+__device__ inline double ld_w(const double *w, int i, int j, int k, int nx, int ny, int nz, bool &in) {
+  in = !((i < 0) | (i >= nx) | (j < 0) | (j >= ny) | (k < 0) | (k > nz));
+  return w[opaque(in ? iW(i, j, k, nx, ny) : 0LL)];
+}
+
+// Convection at a face of component comp: the centre c, the two other components' four
+// values each (a, b; fallback 0) in conv_*_at's order, and the six neighbours of the same
+// component (n: +x, -x, +y, -y, +z, -z; fallback c).
+struct ConvLoads {
+  double c, a[4], b[4], n[6];
+  bool ia[4], ib[4], in[6];
+};
+
+template <int comp>
+__device__ inline ConvLoads conv_loads(const double *u, const double *v, const double *w, int i, int j, int k, int nx,
+                                       int ny, int nz) {
+  ConvLoads L;
+  bool ic;
+  if (comp == 0) {
+    L.c = ld_u(u, i, j, k, nx, ny, nz, ic);
+    L.a[0] = ld_v(v, i - 1, j, k, nx, ny, nz, L.ia[0]);
+    L.a[1] = ld_v(v, i - 1, j + 1, k, nx, ny, nz, L.ia[1]);
+    L.a[2] = ld_v(v, i, j, k, nx, ny, nz, L.ia[2]);
+    L.a[3] = ld_v(v, i, j + 1, k, nx, ny, nz, L.ia[3]);
+    L.b[0] = ld_w(w, i - 1, j, k, nx, ny, nz, L.ib[0]);
+    L.b[1] = ld_w(w, i - 1, j, k + 1, nx, ny, nz, L.ib[1]);
+    L.b[2] = ld_w(w, i, j, k, nx, ny, nz, L.ib[2]);
+    L.b[3] = ld_w(w, i, j, k + 1, nx, ny, nz, L.ib[3]);
+    L.n[0] = ld_u(u, i + 1, j, k, nx, ny, nz, L.in[0]);
+    L.n[1] = ld_u(u, i - 1, j, k, nx, ny, nz, L.in[1]);
+    L.n[2] = ld_u(u, i, j + 1, k, nx, ny, nz, L.in[2]);
+    L.n[3] = ld_u(u, i, j - 1, k, nx, ny, nz, L.in[3]);
+    L.n[4] = ld_u(u, i, j, k + 1, nx, ny, nz, L.in[4]);
+    L.n[5] = ld_u(u, i, j, k - 1, nx, ny, nz, L.in[5]);
+  } else if (comp == 1) {
+    L.c = ld_v(v, i, j, k, nx, ny, nz, ic);
+    L.a[0] = ld_u(u, i, j - 1, k, nx, ny, nz, L.ia[0]);
+    L.a[1] = ld_u(u, i + 1, j - 1, k, nx, ny, nz, L.ia[1]);
+    L.a[2] = ld_u(u, i, j, k, nx, ny, nz, L.ia[2]);
+    L.a[3] = ld_u(u, i + 1, j, k, nx, ny, nz, L.ia[3]);
+    L.b[0] = ld_w(w, i, j - 1, k, nx, ny, nz, L.ib[0]);
+    L.b[1] = ld_w(w, i, j, k, nx, ny, nz, L.ib[1]);
+    L.b[2] = ld_w(w, i, j - 1, k + 1, nx, ny, nz, L.ib[2]);
+    L.b[3] = ld_w(w, i, j, k + 1, nx, ny, nz, L.ib[3]);
+    L.n[0] = ld_v(v, i + 1, j, k, nx, ny, nz, L.in[0]);
+    L.n[1] = ld_v(v, i - 1, j, k, nx, ny, nz, L.in[1]);
+    L.n[2] = ld_v(v, i, j + 1, k, nx, ny, nz, L.in[2]);
+    L.n[3] = ld_v(v, i, j - 1, k, nx, ny, nz, L.in[3]);
+    L.n[4] = ld_v(v, i, j, k + 1, nx, ny, nz, L.in[4]);
+    L.n[5] = ld_v(v, i, j, k - 1, nx, ny, nz, L.in[5]);
+  } else {
+    L.c = ld_w(w, i, j, k, nx, ny, nz, ic);
+    L.a[0] = ld_u(u, i, j, k - 1, nx, ny, nz, L.ia[0]);
+    L.a[1] = ld_u(u, i + 1, j, k - 1, nx, ny, nz, L.ia[1]);
+    L.a[2] = ld_u(u, i, j, k, nx, ny, nz, L.ia[2]);
+    L.a[3] = ld_u(u, i + 1, j, k, nx, ny, nz, L.ia[3]);
+    L.b[0] = ld_v(v, i, j, k - 1, nx, ny, nz, L.ib[0]);
+    L.b[1] = ld_v(v, i, j + 1, k - 1, nx, ny, nz, L.ib[1]);
+    L.b[2] = ld_v(v, i, j, k, nx, ny, nz, L.ib[2]);
+    L.b[3] = ld_v(v, i, j + 1, k, nx, ny, nz, L.ib[3]);
+    L.n[0] = ld_w(w, i + 1, j, k, nx, ny, nz, L.in[0]);
+    L.n[1] = ld_w(w, i - 1, j, k, nx, ny, nz, L.in[1]);
+    L.n[2] = ld_w(w, i, j + 1, k, nx, ny, nz, L.in[2]);
+    L.n[3] = ld_w(w, i, j - 1, k, nx, ny, nz, L.in[3]);
+    L.n[4] = ld_w(w, i, j, k + 1, nx, ny, nz, L.in[4]);
+    L.n[5] = ld_w(w, i, j, k - 1, nx, ny, nz, L.in[5]);
+  }
+  return L;
+}
+
+__device__ inline void pin_conv(const ConvLoads &L) {
+  pin_all(L.c, L.a[0], L.a[1], L.a[2], L.a[3], L.b[0], L.b[1], L.b[2], L.b[3]);
+  pin_all(L.n[0], L.n[1], L.n[2], L.n[3], L.n[4], L.n[5]);
+}
+
+template <int comp>
+__device__ inline double conv_eval(const ConvLoads &L, double dx, double dy, double dz) {
+  const double c = L.c;
+  const double A0 = L.ia[0] ? L.a[0] : 0.0, A1 = L.ia[1] ? L.a[1] : 0.0, A2 = L.ia[2] ? L.a[2] : 0.0,
+               A3 = L.ia[3] ? L.a[3] : 0.0;
+  const double B0 = L.ib[0] ? L.b[0] : 0.0, B1 = L.ib[1] ? L.b[1] : 0.0, B2 = L.ib[2] ? L.b[2] : 0.0,
+               B3 = L.ib[3] ? L.b[3] : 0.0;
+  const double e = L.in[0] ? L.n[0] : c, wv = L.in[1] ? L.n[1] : c, n = L.in[2] ? L.n[2] : c,
+               s = L.in[3] ? L.n[3] : c, t = L.in[4] ? L.n[4] : c, b = L.in[5] ? L.n[5] : c;
+  const double pa = 0.25 * (A0 + A1 + A2 + A3), pb = 0.25 * (B0 + B1 + B2 + B3);
+  if (comp == 0) return c * (e - wv) / (2 * dx) + pa * (n - s) / (2 * dy) + pb * (t - b) / (2 * dz);
+  if (comp == 1) return pa * (e - wv) / (2 * dx) + c * (n - s) / (2 * dy) + pb * (t - b) / (2 * dz);
+  return pa * (e - wv) / (2 * dx) + pb * (n - s) / (2 * dy) + c * (t - b) / (2 * dz);
+}
+
+// G p at a face of component comp (grad*_at): the fluid flags and p of the cell before
+// (lo) and after (hi) the face along comp, at clamped cells; a u face at i == nx has
+// only its lo cell (nx - 1).
+struct GradLoads {
+  int flo, fhi;
+  double plo, phi;
+};
+
+template <int comp>
+__device__ inline GradLoads grad_loads(const double *p, const u8 *fluid, int i, int j, int k, int nx, int ny, int nz) {
+  int li = i, lj = j, lk = k, hi_ = i, hj = j, hk = k;
+  if (comp == 0) li = (i >= 1) ? i - 1 : 0, hi_ = (i < nx) ? i : nx - 1;
+  if (comp == 1) lj = (j >= 1) ? j - 1 : 0, hj = (j < ny) ? j : ny - 1;
+  if (comp == 2) lk = (k >= 1) ? k - 1 : 0, hk = (k < nz) ? k : nz - 1;
+  const long long lo = iP(li, lj, lk, nx, ny), hi = iP(hi_, hj, hk, nx, ny);
+  GradLoads G;
+  G.flo = fluid[lo];
+  G.fhi = fluid[hi];
+  G.plo = p[lo];
+  G.phi = p[hi];
+  return G;
+}
+
+__device__ inline void pin_grad(const GradLoads &G) { pin_all(G.flo, G.fhi, G.plo, G.phi); }
+
+template <int comp>
+__device__ inline double grad_eval(const GradLoads &G, int i, int nx, double h) {
+  double s = 0.0;
+  if (comp == 0 && i == nx) {
+    if (G.flo) s += (-2.0 / h) * G.plo;
+    return s;
+  }
+  if (G.flo) s += (-1.0 / h) * G.plo;
+  if (G.fhi) s += (1.0 / h) * G.phi;
+  return s;
+}
+
 // D q at a fluid cell in CSR column order (-x, +x, -y, +y, -z, +z terms, a term skipped
 // where its coefficient is zero or its face is a wall); qout supplies the outflow faces
 // when non-null.  Two steps: div_loads reads every value of cell (i, j, k) at an address
@@ -198,46 +338,48 @@ __device__ inline void scatter_point(int comp, long long tid, double *o, const d
 
 // Momentum rhs and x0 of box row (i, j, k) of component comp (gpu_dense_ops.hpp:16-18
 // with the convection and G p of the same face).
-__device__ inline void explicit_row(int comp, const fpc_flow_ops &o, const double *u, const double *v, const double *w,
-                                    const double *p, double *rhs, double *x0, int zero, int i, int j, int k) {
-  int nx = o.nx, ny = o.ny, nz = o.nz;
-  int bx = nx - (comp == 0), by = ny - (comp == 1);
-  long long row = ((long long)k * by + j) * bx + i;
-  double rr = 0.0, xv;
-  double at = o.rho / o.dt;
-  if (comp == 0) {
-    int fi = i + 1;
-    long long t = iU(fi, j, k, nx, ny);
-    double f = u[t];
-    xv = f;
-    if (o.active_u[t]) {
-      double cv = conv_u_at(u, v, w, fi, j, k, nx, ny, nz, o.dx, o.dy, o.dz);
-      double g = o.active_qu[t] ? gradx_at(p, o.fluid_p, fi, j, k, nx, ny, o.dx) : 0.0;
-      rr = (at * f - o.rho * cv - g + o.rhs_u_bc[row]) * o.scale_u[row];
-    }
-  } else if (comp == 1) {
-    int fj = j + 1;
-    long long t = iV(i, fj, k, nx, ny);
-    double f = v[t];
-    xv = f;
-    if (o.active_v[t]) {
-      double cv = conv_v_at(u, v, w, i, fj, k, nx, ny, nz, o.dx, o.dy, o.dz);
-      double g = o.active_qv[t] ? grady_at(p, o.fluid_p, i, fj, k, nx, ny, o.dy) : 0.0;
-      rr = (at * f - o.rho * cv - g + 0.0) * o.scale_v[row];
-    }
-  } else {
-    int fk = k + 1;
-    long long t = iW(i, j, fk, nx, ny);
-    double f = w[t];
-    xv = f;
-    if (o.active_w[t]) {
-      double cv = o.active_w[iW(i, j, fk - 1, nx, ny)] ? conv_w_at(u, v, w, i, j, fk, nx, ny, nz, o.dx, o.dy, o.dz) : 0.0;
-      double g = o.active_qw[t] ? gradz_at(p, o.fluid_p, i, j, fk, nx, ny, o.dz) : 0.0;
-      rr = (at * f - o.rho * cv - g + 0.0) * o.scale_w[row];
-    }
+template <int comp>
+__device__ inline void explicit_face(const fpc_flow_ops &o, const double *u, const double *v, const double *w,
+                                     const double *p, double *rhs, double *x0, int zero, int i, int j, int k,
+                                     long long row) {
+  const int nx = o.nx, ny = o.ny, nz = o.nz;
+  const int fi = i + (comp == 0), fj = j + (comp == 1), fk = k + (comp == 2);
+  const long long t = comp == 0 ? iU(fi, fj, fk, nx, ny) : (comp == 1 ? iV(fi, fj, fk, nx, ny) : iW(fi, fj, fk, nx, ny));
+  const u8 *act = comp == 0 ? o.active_u : (comp == 1 ? o.active_v : o.active_w);
+  const u8 *actq = comp == 0 ? o.active_qu : (comp == 1 ? o.active_qv : o.active_qw);
+  const double *scale = comp == 0 ? o.scale_u : (comp == 1 ? o.scale_v : o.scale_w);
+  const double h = comp == 0 ? o.dx : (comp == 1 ? o.dy : o.dz);
+  const int a = act[t], aq = actq[t];
+  const int below = (comp == 2) ? act[iW(fi, fj, fk - 1, nx, ny)] : 1;  // w: convection only over an active face
+  const ConvLoads C = conv_loads<comp>(u, v, w, fi, fj, fk, nx, ny, nz);
+  const GradLoads G = grad_loads<comp>(p, o.fluid_p, fi, fj, fk, nx, ny, nz);
+  const double sc = scale[row];
+  const double bc = comp == 0 ? o.rhs_u_bc[row] : 0.0;
+  pin_conv(C);
+  pin_grad(G);
+  pin_all(a, aq, below, sc, bc);
+  const double at = o.rho / o.dt;
+  const double f = C.c;
+  double rr = 0.0;
+  if (a) {
+    const double cv = below ? conv_eval<comp>(C, o.dx, o.dy, o.dz) : 0.0;
+    const double g = aq ? grad_eval<comp>(G, fi, nx, h) : 0.0;
+    rr = (at * f - o.rho * cv - g + bc) * sc;
   }
   rhs[row] = rr;
-  x0[row] = zero ? 0.0 : xv;
+  x0[row] = zero ? 0.0 : f;
+}
+
+__device__ inline void explicit_row(int comp, const fpc_flow_ops &o, const double *u, const double *v, const double *w,
+                                    const double *p, double *rhs, double *x0, int zero, int i, int j, int k) {
+  const int bx = o.nx - (comp == 0), by = o.ny - (comp == 1);
+  const long long row = ((long long)k * by + j) * bx + i;
+  if (comp == 0)
+    explicit_face<0>(o, u, v, w, p, rhs, x0, zero, i, j, k, row);
+  else if (comp == 1)
+    explicit_face<1>(o, u, v, w, p, rhs, x0, zero, i, j, k, row);
+  else
+    explicit_face<2>(o, u, v, w, p, rhs, x0, zero, i, j, k, row);
 }
 
 // Pressure rhs and x0 = 0 of cell (i, j, k) (gpu_dense_ops.hpp:80-90, main.cu:314-321).
@@ -262,40 +404,42 @@ __device__ inline void divergence_cell(const fpc_flow_ops &o, const double *ut, 
 // Projection of the faces at index tid of each of u, v, w (gpu_dense_ops.hpp:92-106).
 __device__ inline void project_point(const fpc_flow_ops &o, long long tid, const double *phi, const double *ut,
                                      const double *vt, const double *wt, double *u, double *v, double *w, double *qout) {
-  int nx = o.nx, ny = o.ny, nz = o.nz;
-  double f = o.dt / o.rho;
-  long long Nu = (long long)(nx + 1) * ny * nz, Nv = (long long)nx * (ny + 1) * nz, Nw = (long long)nx * ny * (nz + 1);
-  if (tid < Nu) {
-    long long pl = (long long)(nx + 1) * ny;
-    int k = (int)(tid / pl);
-    long long rem = tid - k * pl;
-    int j = (int)(rem / (nx + 1));
-    int i = (int)(rem - (long long)j * (nx + 1));
-    double val = ut[tid];
-    if (o.active_qu[tid] && i >= 1) {
-      val = val - f * gradx_at(phi, o.fluid_p, i, j, k, nx, ny, o.dx);
-      if (i == nx) qout[(long long)j + (long long)ny * k] = val;
+  const int nx = o.nx, ny = o.ny, nz = o.nz;
+  const double f = o.dt / o.rho;
+  const long long Nu = (long long)(nx + 1) * ny * nz, Nv = (long long)nx * (ny + 1) * nz, Nw = (long long)nx * ny * (nz + 1);
+  const bool hu = tid < Nu, hv = tid < Nv, hw = tid < Nw;
+  // face coordinates (entry 0 for an index past a component's faces)
+  const long long tu = hu ? tid : 0, tv = hv ? tid : 0, tw = hw ? tid : 0;
+  const long long plu = (long long)(nx + 1) * ny, plv = (long long)nx * (ny + 1), plw = (long long)nx * ny;
+  const int uk = (int)(tu / plu), uj = (int)((tu - uk * plu) / (nx + 1)), ui = (int)(tu - uk * plu - (long long)uj * (nx + 1));
+  const int vk = (int)(tv / plv), vj = (int)((tv - vk * plv) / nx), vi = (int)(tv - vk * plv - (long long)vj * nx);
+  const int wk = (int)(tw / plw), wj = (int)((tw - wk * plw) / nx), wi = (int)(tw - wk * plw - (long long)wj * nx);
+  // every load of the three faces, then one wait
+  const double xu = ut[tu], xv = vt[tv], xw = wt[tw];
+  const int qu = o.active_qu[tu], qv = o.active_qv[tv], qw = o.active_qw[tw];
+  const GradLoads Gu = grad_loads<0>(phi, o.fluid_p, ui, uj, uk, nx, ny, nz);
+  const GradLoads Gv = grad_loads<1>(phi, o.fluid_p, vi, vj, vk, nx, ny, nz);
+  const GradLoads Gw = grad_loads<2>(phi, o.fluid_p, wi, wj, wk, nx, ny, nz);
+  pin_grad(Gu);
+  pin_grad(Gv);
+  pin_grad(Gw);
+  pin_all(xu, xv, xw, qu, qv, qw);
+  if (hu) {
+    double val = xu;
+    if (qu && ui >= 1) {
+      val = val - f * grad_eval<0>(Gu, ui, nx, o.dx);
+      if (ui == nx) qout[(long long)uj + (long long)ny * uk] = val;
     }
     u[tid] = val;
   }
-  if (tid < Nv) {
-    long long pl = (long long)nx * (ny + 1);
-    int k = (int)(tid / pl);
-    long long rem = tid - k * pl;
-    int j = (int)(rem / nx);
-    int i = (int)(rem - (long long)j * nx);
-    double val = vt[tid];
-    if (o.active_qv[tid] && j >= 1 && j <= ny - 1) val = val - f * grady_at(phi, o.fluid_p, i, j, k, nx, ny, o.dy);
+  if (hv) {
+    double val = xv;
+    if (qv && vj >= 1 && vj <= ny - 1) val = val - f * grad_eval<1>(Gv, vi, nx, o.dy);
     v[tid] = val;
   }
-  if (tid < Nw) {
-    long long pl = (long long)nx * ny;
-    int k = (int)(tid / pl);
-    long long rem = tid - k * pl;
-    int j = (int)(rem / nx);
-    int i = (int)(rem - (long long)j * nx);
-    double val = wt[tid];
-    if (o.active_qw[tid] && k >= 1 && k <= nz - 1) val = val - f * gradz_at(phi, o.fluid_p, i, j, k, nx, ny, o.dz);
+  if (hw) {
+    double val = xw;
+    if (qw && wk >= 1 && wk <= nz - 1) val = val - f * grad_eval<2>(Gw, wi, nx, o.dz);
     w[tid] = val;
   }
 }
