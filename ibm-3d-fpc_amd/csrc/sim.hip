@@ -27,6 +27,7 @@
 #include "fpc_setup.hpp"
 #include "fpc_sim.h"
 #include "fpc_struct.h"
+#include "batched.hpp"
 #include "sim_internal.hpp"
 
 namespace fpc {
@@ -133,15 +134,18 @@ int fpc::sim_run_phase(fpc_sim M, int ph) {
         fpc_StructGMRESGetFinalRelativeResidualNorm(s3[c], &M->rel[c]);
       }
       M->t_mom += now_s() - t0;
-      int bad = 0, f;
-      long long nb[3] = {(long long)(nx - 1) * ny * nz, (long long)nx * (ny - 1) * nz, (long long)nx * ny * (nz - 1)};
-      double *xs[3] = {M->pxu, M->pxv, M->pxw};
-      for (int c = 0; c < 3; ++c) {
-        rc = fpc_any_nonfinite(xs[c], nb[c], &f);
+      if (!M->nf_mom) {
+        const std::vector<const double *> xs = {M->pxu, M->pxv, M->pxw};
+        const std::vector<long long> nb = {(long long)(nx - 1) * ny * nz, (long long)nx * (ny - 1) * nz,
+                                           (long long)nx * ny * (nz - 1)};
+        rc = nonfinite_create(xs, nb, &M->nf_mom);
         if (rc) return rc;
-        bad |= f;
       }
-      if (bad) {
+      rc = nonfinite_enqueue(M->nf_mom);
+      if (rc) return rc;
+      FPC_HIP(hipStreamSynchronize(st));
+      const int *f = nonfinite_flags(M->nf_mom);
+      if (f[0] | f[1] | f[2]) {
         fpc::set_error(FPC_ERROR_GENERIC, "non-finite momentum solve output at step %d (reference exit 911)", M->step);
         return 911;
       }
@@ -479,6 +483,7 @@ int fpc_sim_destroy(fpc_sim M) {
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (M->hdiag) (void)hipHostFree(M->hdiag);
+  if (M->nf_mom) fpc::nonfinite_destroy(M->nf_mom);
   if (M->stream) (void)hipStreamDestroy(M->stream);
   delete M;
   return 0;

@@ -11,6 +11,10 @@
 #include "fpc_sim.h"
 #include "fpc_struct.h"
 
+namespace fpc {
+class NonfiniteCheck;
+}
+
 struct fpc_sim_s {
   fpc_params par{};
   hipStream_t stream = nullptr;  // every call on this handle runs on its own stream
@@ -44,6 +48,7 @@ struct fpc_sim_s {
   double rel[4] = {0, 0, 0, 0};
   int err[4] = {0, 0, 0, 0};
   double t_build = 0, t_mom = 0, t_p = 0, t_proj = 0, t_diag = 0;
+  fpc::NonfiniteCheck *nf_mom = nullptr;  // the three momentum solutions, one check and one sync
 };
 
 namespace fpc {
