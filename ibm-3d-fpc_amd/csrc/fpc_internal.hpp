@@ -70,11 +70,12 @@ bool prof_on(const char *cls);
 void prof_begin(const char *cls);
 void prof_end(const char *cls, double bytes);
 // Launches enqueued ahead of a device-side stop flag may return at once; they are
-// tagged with the Arnoldi step they belong to and dropped once the host knows how
-// many steps ran (records from index `from` with tag >= ran).
+// tagged with the Arnoldi step they belong to and its solver (owner), and dropped once
+// the host knows how many steps ran (that owner's records from index `from` with
+// tag >= ran; solves advanced in lockstep interleave their records).
 size_t prof_mark();
-void prof_tag(int tag);
-void prof_drop_skipped(size_t from, int ran);
+void prof_tag(int tag, const void *owner);
+void prof_drop_skipped(size_t from, int ran, const void *owner);
 #define FPC_PROF_BEGIN(cls)                    \
   const bool prof_on_ = ::fpc::prof_on(cls);   \
   if (prof_on_) ::fpc::prof_begin(cls)
@@ -321,6 +322,12 @@ int dot_partials(const double *a, const double *b, long long n, double *part);
 int matvec(double alpha, const fpc_StructMatrix_s *A, const double *x, double beta, double *y);
 int ensure_dinv(fpc_StructMatrix_s *A);
 int ensure_fp32(fpc_StructMatrix_s *A);
+
+// n GMRES solves (fpc_StructGMRESSolve each) advanced in lockstep with one host sync
+// per round for all of them (krylov.hip); rc[q] is what solve q alone would return.
+// Returns nonzero (and abandons the others) when a solve fails outside convergence.
+int gmres_solve_many(const fpc_StructSolver *s, const fpc_StructMatrix *A, const fpc_StructVector *b,
+                     const fpc_StructVector *x, int n, int *rc);
 
 // The planes the solve loops' stencil kernels read: the extruded copy when the
 // operator has one, else the full planes (ensure_dinv / ensure_fp32 first).

@@ -448,7 +448,8 @@ int fpc_StructGMRESSetup(fpc_StructSolver s, fpc_StructMatrix A, fpc_StructVecto
   return 0;
 }
 
-// ||b - A x|| into V[0] (or W), and optionally ||b||; values land in hscal[0], hscal[1].
+// ||b - A x|| into V[0] (or W), and optionally ||b||; the values land in hscal[0], hscal[1]
+// once the stream has been synchronised (enqueued only).
 static int true_residual(GmresImpl *G, const double *b, const double *x, double *rout, bool with_b) {
   int rc = residual_norm_partials(G->A, b, x, rout, G->part);
   if (rc) return rc;
@@ -460,7 +461,6 @@ static int true_residual(GmresImpl *G, const double *b, const double *x, double 
     if (rc) return rc;
   }
   FPC_HIP(hipMemcpyAsync(G->hscal, G->dscal, 2 * sizeof(double), hipMemcpyDeviceToHost, stream()));
-  FPC_HIP(hipStreamSynchronize(stream()));
   return 0;
 }
 
@@ -521,124 +521,241 @@ static int run_step(GmresImpl *G, int step) {
   return arnoldi_tail(G, step);
 }
 
-int fpc_StructGMRESSolve(fpc_StructSolver s, fpc_StructMatrix A, fpc_StructVector bv, fpc_StructVector xv) {
-  GIMPL(s);
-  FPC_CHECK_ARG(A && bv && xv && G->A == A && G->N == A->grid.N, "call Setup with this matrix first");
-  FPC_CHECK_ARG(bv->grid.N == G->N && xv->grid.N == G->N, "size mismatch");
-  const double *b = bv->data;
-  double *x = xv->data;
-  long long N = G->N;
-  int rc = ensure_dinv(A);
-  if (rc) return rc;
-  if (G->pc_kind == PC_PFMG) {
-    rc = pfmg_setup(G->pc->impl, A);  // no-op unless A changed
+// One GMRES solve as a resumable run.  Each stage enqueues its launches and ends with
+// the host copy its decision needs; after the stream is synchronised, run_advance reads
+// that copy and enqueues the next stage.  The launches and the decisions are those of
+// the solve loop in order (restarted GMRES with HYPRE's convergence test: true residual
+// once the recursive one is below tol, stop when it is below tol or no longer falls),
+// so runs advanced together -- one sync for all of them -- give the bits of the same
+// runs made one after another.
+struct GmresRun {
+  enum Stage { kInit, kPre, kCycle, kPost, kDone };
+  GmresImpl *G = nullptr;
+  const double *b = nullptr;
+  double *x = nullptr;
+  Stage stage = kInit;
+  double r_norm = 0.0, b_norm = 0.0, eps = 0.0, real_old = 0.0;
+  int iter = 0, issued = 0, batch = 1;
+  bool first_cycle = true;
+  size_t prof_from = 0;
+};
+
+static unsigned combine_blocks(long long N) { return (unsigned)std::max(1LL, std::min((N + 255) / 256, 65536LL)); }
+
+// A batch of Arnoldi steps and the copy of the status record.
+static int run_batch(GmresRun &R) {
+  GmresImpl *G = R.G;
+  int nb = std::min(R.batch, G->kd - R.issued);
+  nb = std::min(nb, G->maxit - R.iter - R.issued);
+  if (nb <= 0) nb = 1;
+  for (int q = 0; q < nb; ++q) {
+    prof_tag(R.issued + q, G);
+    int rc = run_step(G, R.issued + q);
     if (rc) return rc;
   }
-  // initial residual into V[0]
-  rc = true_residual(G, b, x, G->V, true);
-  if (rc) return rc;
-  double r_norm = std::sqrt(G->hscal[0]);
-  double b_norm = std::sqrt(G->hscal[1]);
-  G->its = 0;
-  if (!std::isfinite(b_norm) || !std::isfinite(r_norm)) {
-    G->relres = NAN;
-    set_error(FPC_ERROR_GENERIC, "GMRES: non-finite right-hand side or initial residual");
-    return FPC_ERROR_GENERIC;
+  prof_tag(-1, nullptr);
+  R.issued += nb;
+  FPC_HIP(hipMemcpyAsync(G->hst, &G->S->st, sizeof(Status), hipMemcpyDeviceToHost, stream()));
+  R.stage = GmresRun::kCycle;
+  return 0;
+}
+
+// A new cycle: i = 0 from r_norm, then the first batch of steps (as many as the last
+// solve's first cycle took).
+static int run_cycle_start(GmresRun &R) {
+  GmresImpl *G = R.G;
+  k_cycle_start<<<1, 1, 0, stream()>>>(G->S, R.r_norm, R.eps, R.iter, G->maxit, G->kd);
+  FPC_LAUNCH_CHECK();
+  R.issued = 0;
+  R.prof_from = prof_mark();
+  R.batch = std::max(1, std::min(G->pred, G->kd));
+  if (G->pc_kind == PC_CALLBACK) R.batch = 1;
+  return run_batch(R);
+}
+
+// The top of the restart loop (host decisions only).
+static int run_top(GmresRun &R) {
+  GmresImpl *G = R.G;
+  if (R.iter >= G->maxit || R.r_norm == 0.0) {
+    R.stage = GmresRun::kDone;
+    return 0;
   }
-  double den = (b_norm > 0.0) ? b_norm : r_norm;
-  double eps = std::max(G->atol, G->tol * den);
-  double real_old = b_norm;
-  int iter = 0;
-  bool converged = false;
-  unsigned nblk_n = (unsigned)std::max(1LL, std::min((N + 255) / 256, 65536LL));
-  bool first_cycle = true;
-  while (iter < G->maxit) {
-    if (r_norm == 0.0) { converged = true; break; }
-    if (r_norm <= eps) {
-      rc = true_residual(G, b, x, G->W, false);
-      if (rc) return rc;
-      r_norm = std::sqrt(G->hscal[0]);
-      if (r_norm <= eps) { converged = true; break; }
+  if (R.r_norm <= R.eps) {  // check the true residual before another cycle
+    R.stage = GmresRun::kPre;
+    return true_residual(G, R.b, R.x, G->W, false);
+  }
+  return run_cycle_start(R);
+}
+
+static int run_begin(GmresRun &R) {
+  R.stage = GmresRun::kInit;
+  return true_residual(R.G, R.b, R.x, R.G->V, true);  // initial residual into V[0]
+}
+
+// After a sync: read this run's host copy and enqueue its next stage.
+static int run_advance(GmresRun &R) {
+  GmresImpl *G = R.G;
+  const long long N = G->N;
+  int rc = 0;
+  switch (R.stage) {
+    case GmresRun::kInit: {
+      R.r_norm = std::sqrt(G->hscal[0]);
+      R.b_norm = std::sqrt(G->hscal[1]);
+      G->its = 0;
+      if (!std::isfinite(R.b_norm) || !std::isfinite(R.r_norm)) {
+        G->relres = NAN;
+        R.stage = GmresRun::kDone;
+        set_error(FPC_ERROR_GENERIC, "GMRES: non-finite right-hand side or initial residual");
+        return FPC_ERROR_GENERIC;
+      }
+      const double den = (R.b_norm > 0.0) ? R.b_norm : R.r_norm;
+      R.eps = std::max(G->atol, G->tol * den);
+      R.real_old = R.b_norm;
+      R.iter = 0;
+      R.first_cycle = true;
+      return run_top(R);
+    }
+    case GmresRun::kPre:
+      R.r_norm = std::sqrt(G->hscal[0]);
+      if (R.r_norm <= R.eps) {
+        R.stage = GmresRun::kDone;
+        return 0;
+      }
       rc = vec_copy(G->V, G->W, N);
       if (rc) return rc;
-    }
-    k_cycle_start<<<1, 1, 0, stream()>>>(G->S, r_norm, eps, iter, G->maxit, G->kd);
-    FPC_LAUNCH_CHECK();
-    // Arnoldi steps in batches; the status record says when to stop.
-    int issued = 0;
-    const size_t prof_from = prof_mark();
-    int batch = std::max(1, std::min(G->pred, G->kd));
-    if (G->pc_kind == PC_CALLBACK) batch = 1;
-    for (;;) {
-      int nb = std::min(batch, G->kd - issued);
-      nb = std::min(nb, G->maxit - iter - issued);
-      if (nb <= 0) nb = 1;
-      for (int q = 0; q < nb; ++q) {
-        prof_tag(issued + q);
-        rc = run_step(G, issued + q);
+      return run_cycle_start(R);
+    case GmresRun::kCycle: {
+      if (!G->hst->done && R.issued < G->kd) {
+        R.batch = (G->pc_kind == PC_PFMG || G->pc_kind == PC_CALLBACK) ? 1 : 2;
+        return run_batch(R);
+      }
+      const int i = G->hst->i;
+      prof_drop_skipped(R.prof_from, i, G);
+      R.iter = G->hst->iter;
+      R.r_norm = G->hst->r_norm;
+      if (R.first_cycle) G->pred = std::max(1, i);
+      R.first_cycle = false;
+      // x update
+      fpc_StructMatrix_s *A = G->A;
+      const unsigned nblk_n = combine_blocks(N);
+      k_backsolve<<<1, 1, 0, stream()>>>(G->S);
+      FPC_LAUNCH_CHECK();
+      if (G->flexible) {
+        k_combine<double><<<nblk_n, 256, 0, stream()>>>(G->Z, G->vp, N, G->S, nullptr, R.x, 1);
+        FPC_LAUNCH_CHECK();
+      } else if (G->pc_kind == PC_JACOBI1 && ((JacobiImpl *)G->pc->impl)->bits == 32) {
+        k_combine<float><<<nblk_n, 256, 0, stream()>>>(G->V, G->vp, N, G->S, A->dinv32, R.x, 1);
+        FPC_LAUNCH_CHECK();
+      } else if (G->pc_kind == PC_JACOBI1 || G->pc_kind == PC_NONE) {
+        k_combine<double><<<nblk_n, 256, 0, stream()>>>(G->V, G->vp, N, G->S,
+                                                        G->pc_kind == PC_JACOBI1 ? A->dinv : nullptr, R.x, 1);
+        FPC_LAUNCH_CHECK();
+      } else {
+        k_combine<double><<<nblk_n, 256, 0, stream()>>>(G->V, G->vp, N, G->S, nullptr, G->W, 0);
+        FPC_LAUNCH_CHECK();
+        rc = precond_apply(G, G->W, G->Z, nullptr);
+        if (rc) return rc;
+        rc = axpy_inplace(R.x, G->Z, N);
         if (rc) return rc;
       }
-      issued += nb;
-      FPC_HIP(hipMemcpyAsync(G->hst, &G->S->st, sizeof(Status), hipMemcpyDeviceToHost, stream()));
-      FPC_HIP(hipStreamSynchronize(stream()));
-      if (G->hst->done) break;
-      if (issued >= G->kd) break;  // cannot happen without done, kept as a guard
-      batch = (G->pc_kind == PC_PFMG || G->pc_kind == PC_CALLBACK) ? 1 : 2;
+      if (R.r_norm <= R.eps) {
+        R.stage = GmresRun::kPost;
+        return true_residual(G, R.b, R.x, G->V, false);
+      }
+      // full cycle without convergence: Givens-based residual as the new p_0
+      k_restart_coefs<<<1, 1, 0, stream()>>>(G->S);
+      FPC_LAUNCH_CHECK();
+      k_restart_vec<<<nblk_n, 256, 0, stream()>>>(G->V, G->vp, N, G->S);
+      FPC_LAUNCH_CHECK();
+      return run_top(R);
     }
-    int i = G->hst->i;
-    prof_tag(-1);
-    prof_drop_skipped(prof_from, i);
-    iter = G->hst->iter;
-    r_norm = G->hst->r_norm;
-    if (first_cycle) G->pred = std::max(1, i);
-    first_cycle = false;
-    // x update
-    k_backsolve<<<1, 1, 0, stream()>>>(G->S);
-    FPC_LAUNCH_CHECK();
-    if (G->flexible) {
-      k_combine<double><<<nblk_n, 256, 0, stream()>>>(G->Z, G->vp, N, G->S, nullptr, x, 1);
-      FPC_LAUNCH_CHECK();
-    } else if (G->pc_kind == PC_JACOBI1 && ((JacobiImpl *)G->pc->impl)->bits == 32) {
-      k_combine<float><<<nblk_n, 256, 0, stream()>>>(G->V, G->vp, N, G->S, A->dinv32, x, 1);
-      FPC_LAUNCH_CHECK();
-    } else if (G->pc_kind == PC_JACOBI1 || G->pc_kind == PC_NONE) {
-      k_combine<double><<<nblk_n, 256, 0, stream()>>>(G->V, G->vp, N, G->S, G->pc_kind == PC_JACOBI1 ? A->dinv : nullptr,
-                                                      x, 1);
-      FPC_LAUNCH_CHECK();
-    } else {
-      k_combine<double><<<nblk_n, 256, 0, stream()>>>(G->V, G->vp, N, G->S, nullptr, G->W, 0);
-      FPC_LAUNCH_CHECK();
-      rc = precond_apply(G, G->W, G->Z, nullptr);
-      if (rc) return rc;
-      rc = axpy_inplace(x, G->Z, N);
-      if (rc) return rc;
+    case GmresRun::kPost: {
+      const double real_new = std::sqrt(G->hscal[0]);
+      R.r_norm = real_new;
+      if (R.r_norm <= R.eps || real_new >= R.real_old) {
+        R.stage = GmresRun::kDone;
+        return 0;
+      }
+      R.real_old = real_new;
+      return run_top(R);  // V[0] holds the true residual
     }
-    if (r_norm <= eps) {
-      rc = true_residual(G, b, x, G->V, false);
-      if (rc) return rc;
-      double real_new = std::sqrt(G->hscal[0]);
-      r_norm = real_new;
-      if (r_norm <= eps) { converged = true; break; }
-      if (real_new >= real_old) { converged = true; break; }
-      real_old = real_new;
-      continue;  // V[0] holds the true residual
-    }
-    // full cycle without convergence: Givens-based residual as the new p_0
-    k_restart_coefs<<<1, 1, 0, stream()>>>(G->S);
-    FPC_LAUNCH_CHECK();
-    k_restart_vec<<<nblk_n, 256, 0, stream()>>>(G->V, G->vp, N, G->S);
-    FPC_LAUNCH_CHECK();
+    case GmresRun::kDone:
+      return 0;
   }
-  FPC_HIP(hipStreamSynchronize(stream()));
-  G->its = iter;
-  G->relres = (b_norm > 0.0) ? r_norm / b_norm : r_norm;
-  if (G->print > 0) std::printf("GMRES: %d iterations, relative residual %.6e\n", iter, G->relres);
-  (void)converged;
-  if (iter >= G->maxit && r_norm > eps && eps > 0) {
+  return 0;
+}
+
+// Checks and set-up of one solve (a PFMG hierarchy is rebuilt only when A changed).
+static int run_prepare(GmresImpl *G, fpc_StructMatrix A, fpc_StructVector bv, fpc_StructVector xv) {
+  FPC_CHECK_ARG(A && bv && xv && G->A == A && G->N == A->grid.N, "call Setup with this matrix first");
+  FPC_CHECK_ARG(bv->grid.N == G->N && xv->grid.N == G->N, "size mismatch");
+  int rc = ensure_dinv(A);
+  if (rc) return rc;
+  if (G->pc_kind == PC_PFMG) rc = pfmg_setup(G->pc->impl, A);  // no-op unless A changed
+  return rc;
+}
+
+// Result of a finished run: iterations, relative residual, FPC_ERROR_CONV at maxit.
+static int run_result(GmresRun &R) {
+  GmresImpl *G = R.G;
+  G->its = R.iter;
+  G->relres = (R.b_norm > 0.0) ? R.r_norm / R.b_norm : R.r_norm;
+  if (G->print > 0) std::printf("GMRES: %d iterations, relative residual %.6e\n", R.iter, G->relres);
+  if (R.iter >= G->maxit && R.r_norm > R.eps && R.eps > 0) {
     set_error(FPC_ERROR_CONV, "GMRES: maximum iterations reached");
     return FPC_ERROR_CONV;
   }
   return 0;
+}
+
+}  // extern "C"
+
+namespace fpc {
+
+// n solves in lockstep on the calling thread's stream, one sync per round for all of
+// them; rc[q] as fpc_StructGMRESSolve returns it for solve q.  A solve that fails
+// outside convergence ends the call: its code is returned, the others are abandoned.
+int gmres_solve_many(const fpc_StructSolver *s, const fpc_StructMatrix *A, const fpc_StructVector *b,
+                     const fpc_StructVector *x, int n, int *rc) {
+  std::vector<GmresRun> runs((size_t)n);
+  for (int q = 0; q < n; ++q) {
+    GmresRun &R = runs[(size_t)q];
+    R.G = gimpl(s[q]);
+    FPC_CHECK_ARG(R.G, "not a GMRES solver handle");
+    int e = run_prepare(R.G, A[q], b[q], x[q]);
+    if (e) return e;
+    R.b = b[q]->data;
+    R.x = x[q]->data;
+  }
+  for (GmresRun &R : runs) {
+    int e = run_begin(R);
+    if (e) return e;
+  }
+  for (;;) {
+    bool live = false;
+    for (const GmresRun &R : runs) live = live || R.stage != GmresRun::kDone;
+    FPC_HIP(hipStreamSynchronize(stream()));
+    if (!live) break;
+    for (GmresRun &R : runs) {
+      if (R.stage == GmresRun::kDone) continue;
+      int e = run_advance(R);
+      if (e) return e;
+    }
+  }
+  for (int q = 0; q < n; ++q) rc[q] = run_result(runs[(size_t)q]);
+  return 0;
+}
+
+}  // namespace fpc
+
+extern "C" {
+
+int fpc_StructGMRESSolve(fpc_StructSolver s, fpc_StructMatrix A, fpc_StructVector bv, fpc_StructVector xv) {
+  GIMPL(s);
+  (void)G;
+  int r = 0;
+  int rc = gmres_solve_many(&s, &A, &bv, &xv, 1, &r);
+  return rc ? rc : r;
 }
 
 int fpc_StructGMRESGetNumIterations(fpc_StructSolver s, int *n) {

@@ -125,11 +125,14 @@ int fpc::sim_run_phase(fpc_sim M, int ph) {
       fpc_StructSolver s3[3] = {M->su, M->sv, M->sw};
       fpc_StructMatrix A3[3] = {M->Au, M->Av, M->Aw};
       fpc_StructVector b3[3] = {M->bu, M->bv, M->bw}, x3[3] = {M->xu, M->xv, M->xw};
+      // the three solves in lockstep: each computes what it computes alone, with the
+      // host syncs of the three shared
+      int rc3[3] = {0, 0, 0};
+      rc = gmres_solve_many(s3, A3, b3, x3, 3, rc3);
+      if (rc) return rc;
       for (int c = 0; c < 3; ++c) {
-        rc = fpc_StructGMRESSolve(s3[c], A3[c], b3[c], x3[c]);
-        M->err[c] = rc;
-        if (rc & ~FPC_ERROR_CONV) return rc;
-        if (rc & FPC_ERROR_CONV) fpc_ClearError(FPC_ERROR_CONV);  // common.hpp:39
+        M->err[c] = rc3[c];
+        if (rc3[c] & FPC_ERROR_CONV) fpc_ClearError(FPC_ERROR_CONV);  // common.hpp:39
         fpc_StructGMRESGetNumIterations(s3[c], &M->its[c]);
         fpc_StructGMRESGetFinalRelativeResidualNorm(s3[c], &M->rel[c]);
       }

@@ -113,6 +113,7 @@ struct ProfRec {
   double bytes;
   hipEvent_t e0, e1;
   int tag;
+  const void *owner;
   bool dropped;
 };
 std::string g_prof_filter;
@@ -122,15 +123,19 @@ std::vector<std::pair<hipEvent_t, hipEvent_t>> g_prof_pool;
 size_t g_prof_used = 0;
 hipEvent_t g_prof_open = nullptr;
 int g_prof_tag = -1;
+const void *g_prof_owner = nullptr;
 }  // namespace
 
 size_t prof_mark() { return g_prof_recs.size(); }
 
-void prof_tag(int tag) { g_prof_tag = tag; }
+void prof_tag(int tag, const void *owner) {
+  g_prof_tag = tag;
+  g_prof_owner = owner;
+}
 
-void prof_drop_skipped(size_t from, int ran) {
+void prof_drop_skipped(size_t from, int ran, const void *owner) {
   for (size_t q = from; q < g_prof_recs.size(); ++q)
-    if (g_prof_recs[q].tag >= ran) g_prof_recs[q].dropped = true;
+    if (g_prof_recs[q].owner == owner && g_prof_recs[q].tag >= ran) g_prof_recs[q].dropped = true;
 }
 
 bool prof_on(const char *cls) {
@@ -155,7 +160,7 @@ void prof_end(const char *cls, double bytes) {
   if (!g_prof_open || g_prof_used >= g_prof_pool.size()) return;
   auto &pr = g_prof_pool[g_prof_used++];
   (void)hipEventRecord(pr.second, stream());
-  g_prof_recs.push_back({cls, bytes, pr.first, pr.second, g_prof_tag, false});
+  g_prof_recs.push_back({cls, bytes, pr.first, pr.second, g_prof_tag, g_prof_owner, false});
   g_prof_open = nullptr;
 }
 
