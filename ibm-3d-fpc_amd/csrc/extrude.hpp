@@ -52,6 +52,18 @@ inline ExtrudePlan plan_extrusion(int nx, int ny, int nz, const std::vector<cons
   return best;
 }
 
+// The flags as bits per column: bit (k & 63) of word (k >> 6) * nx ny + (j nx + i) is
+// set when point (i, j, k) is flagged.  A z-marching tile keeps its columns' words in
+// registers, so knowing a point's row needs no load of its own.
+inline std::vector<unsigned long long> flag_words(const ExtrudePlan &plan, int nx, int ny, int nz) {
+  const long long pl = (long long)nx * ny, nw = (nz + 63) / 64;
+  std::vector<unsigned long long> w((size_t)(nw * pl), 0ull);
+  for (int k = 0; k < nz; ++k)
+    for (long long c = 0; c < pl; ++c)
+      if (plan.xf[(size_t)(k * pl + c)]) w[(size_t)((k >> 6) * pl + c)] |= 1ull << (k & 63);
+  return w;
+}
+
 // Source plane of compact plane kc (0 .. 2m).
 inline long long hot_source_plane(int kc, int m, int nz) {
   return (kc < m) ? kc : (kc == m ? m : (long long)nz - 2 * m - 1 + kc);

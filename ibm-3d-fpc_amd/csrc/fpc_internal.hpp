@@ -155,34 +155,48 @@ __device__ inline T opaque_s(T v) {
   return v;
 }
 
-// The flag read by points of a map without flagged points.
-static __device__ const unsigned char kNoFlags[4] = {0, 0, 0, 0};
+// The flag word read by points of a map without flagged points.
+static __device__ const unsigned long long kNoFlags[1] = {0ull};
 
 // Row of the stored planes that holds point t (on z-plane k).  pl == 0: full
 // storage, row t.  Otherwise the hot copy: a flagged point reads row xoff + t;
 // any other reads compact plane kc -- k for the m bottom planes, m for the
-// interior, m+1 .. 2m for the m top planes -- at row t + (kc - k) * pl.
+// interior, m+1 .. 2m for the m top planes -- at row t + (kc - k) * pl.  The
+// flags are bits per column: bit (k & 63) of xb[(k >> 6) * pl + (t - k pl)].
 struct ZMap {
   long long pl = 0;    // nx * ny of the box, or 0
   int nz = 0, m = 0;   // box depth, wall planes kept per side
   long long xoff = 0;  // first row of the full part
-  const unsigned char *xf = nullptr;  // per point: 1 = read the full part (null: none)
-  __host__ __device__ inline long long operator()(long long t, int k) const {
-    if (!pl) return t;
-    if (xf && xf[t]) return xoff + t;
+  const unsigned long long *xb = nullptr;  // flag words (null: no flagged point)
+  __host__ __device__ inline long long hot(long long t, int k) const {
     const int kc = (k < m) ? k : ((k >= nz - m) ? k - nz + 2 * m + 1 : m);
     return t + (long long)(kc - k) * pl;
+  }
+  __host__ __device__ inline long long operator()(long long t, int k) const {
+    if (!pl) return t;
+    if (xb && ((xb[(long long)(k >> 6) * pl + (t - (long long)k * pl)] >> (k & 63)) & 1ull)) return xoff + t;
+    return hot(t, k);
+  }
+  // The row when the caller knows the point's flag f.
+  __host__ __device__ inline long long row(long long t, int k, int f) const {
+    if (!pl) return t;
+    return f ? xoff + t : hot(t, k);
+  }
+  // Flag word w of column c (j nx + i); 0 without flags.
+  __device__ inline unsigned long long word(long long c, int w) const {
+    return (pl && xb) ? xb[(long long)w * pl + c] : 0ull;
   }
   // The rows of n points at once, in two steps so that other loads can go between
   // them: flags() issues every flag load (box-wide tests once), rows() uses them.
   // No branch: without flags every lane reads one zero byte, so the loads' wait can
   // move down to rows() behind whatever the caller issues in between.
   template <int n>
-  __device__ inline void flags(const long long *t, unsigned char *f) const {
-    const bool on = pl && xf;
-    typedef const __attribute__((address_space(1))) unsigned char *gptr;  // a global load, not flat
-    const gptr p = (gptr)opaque_s(on ? xf : kNoFlags);                     // not folded to a zero
-    for (int q = 0; q < n; ++q) f[q] = p[on ? t[q] : 0];
+  __device__ inline void flags(const long long *t, const int *k, unsigned char *f) const {
+    const bool on = pl && xb;
+    typedef const __attribute__((address_space(1))) unsigned long long *gptr;  // a global load, not flat
+    const gptr p = (gptr)opaque_s(on ? xb : kNoFlags);                          // not folded to a zero
+    for (int q = 0; q < n; ++q)
+      f[q] = (unsigned char)((p[on ? (long long)(k[q] >> 6) * pl + (t[q] - (long long)k[q] * pl) : 0] >> (k[q] & 63)) & 1ull);
   }
   template <int n>
   __device__ inline void rows(const long long *t, const int *k, const unsigned char *f, long long *u) const {
@@ -191,8 +205,7 @@ struct ZMap {
       return;
     }
     for (int q = 0; q < n; ++q) {
-      const int kc = (k[q] < m) ? k[q] : ((k[q] >= nz - m) ? k[q] - nz + 2 * m + 1 : m);
-      const long long h = t[q] + (long long)(kc - k[q]) * pl, x = xoff + t[q];
+      const long long h = hot(t[q], k[q]), x = xoff + t[q];
       u[q] = f[q] ? x : h;
     }
   }
@@ -203,7 +216,8 @@ struct ZMap {
 struct FullMap {
   __host__ __device__ inline long long operator()(long long t, int) const { return t; }
   template <int n>
-  __device__ inline void flags(const long long *, unsigned char *) const {}
+  __device__ inline void flags(const long long *, const int *, unsigned char *) const {}
+  __device__ inline long long row(long long t, int, int) const { return t; }
   template <int n>
   __device__ inline void rows(const long long *t, const int *, const unsigned char *, long long *u) const {
     for (int q = 0; q < n; ++q) u[q] = t[q];
@@ -268,7 +282,7 @@ struct fpc_StructMatrix_s {
   // hot copies of an extruded operator (zpitch apart, rows by zmap; zmap.pl != 0 when present)
   double *zcoef = nullptr, *zdinv = nullptr;
   float *zcoef32 = nullptr, *zdinv32 = nullptr;
-  unsigned char *zxf = nullptr;  // zmap.xf
+  unsigned long long *zxf = nullptr;  // zmap.xb
   long long zpitch = 0, zexc = 0;  // zexc: flagged points
   fpc::ZMap zmap;
   bool initialized = false, assembled = false;

@@ -356,7 +356,7 @@ __device__ inline void residual_body(const double *__restrict__ c, long long pit
     unsigned char fq[2];
     long long uq[2];
     ResidualLoads L0, L1;
-    zm.template flags<2>(tq, fq);
+    zm.template flags<2>(tq, kq, fq);
     residual_loads_x(L0, nx, ny, nz, pl, b, x, t, i, j, k);
     residual_loads_x(L1, nx, ny, nz, pl, b, x, t1, i1, j1, k1);
     zm.template rows<2>(tq, kq, fq, uq);
@@ -412,8 +412,8 @@ struct ArnoldiOp {
     long long u;  // row of the point's coefficients and 1/diag
   };
   __device__ long long at(int i, int j, int k) const { return ((long long)k * ny + j) * nx + i; }
-  __device__ Raw fetch(int i, int j, int k) const {
-    const long long t = at(i, j, k), u = zm(t, k);
+  __device__ Raw fetch(int i, int j, int k, int f) const {
+    const long long t = at(i, j, k), u = zm.row(t, k, f);
     return Raw{z[t], kDinv ? (double)dinv[u] : 1.0, u};
   }
   __device__ double finish(const Raw &r) const { return kDinv ? r.z * r.dinv : r.z; }

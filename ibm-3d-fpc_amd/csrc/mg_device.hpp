@@ -202,6 +202,12 @@ __device__ inline double gemv_row(const double *__restrict__ inv, int n, const T
 }
 
 // ---------------------------------------------------------------- z-marching forms
+// A map that answers with a row already known (the tile's fetch knows the flag).
+struct RowOf {
+  long long u;
+  __device__ inline long long operator()(long long, int) const { return u; }
+};
+
 // The same arithmetic on the z-marching tiles (stencil_tile.hpp): every row read
 // once, z-neighbours carried in registers; bit for bit the one-point results.
 
@@ -220,8 +226,8 @@ struct DownOp {
     long long u;
   };
   __device__ long long at(int i, int j, int k) const { return ((long long)k * ny + j) * nx + i; }
-  __device__ Raw fetch(int i, int j, int k) const {
-    const long long t = at(i, j, k), u = zm(t, k);
+  __device__ Raw fetch(int i, int j, int k, int f) const {
+    const long long t = at(i, j, k), u = zm.row(t, k, f);
     return Raw{b[t], dinv[u], u};
   }
   __device__ T finish(const Raw &q) const { return w * q.b * q.dinv; }
@@ -252,8 +258,8 @@ struct DownRestrictZOp {
     long long u;
   };
   __device__ long long at(int i, int j, int k) const { return ((long long)k * ny + j) * nx + i; }
-  __device__ Raw fetch(int i, int j, int k) const {
-    const long long t = at(i, j, k), u = zm(t, k);
+  __device__ Raw fetch(int i, int j, int k, int f) const {
+    const long long t = at(i, j, k), u = zm.row(t, k, f);
     return Raw{b[t], dinv[u], wm[u], wp[u], u};
   }
   __device__ T finish(const Raw &q) const { return w * q.b * q.dinv; }
@@ -287,8 +293,9 @@ struct UpOp {
   ZMap zm;  // rows of c, dinv, wm and wp
   using Raw = UpLoads<T>;
   __device__ long long at(int i, int j, int k) const { return ((long long)k * ny + j) * nx + i; }
-  __device__ Raw fetch(int i, int j, int k) const {
-    return up_loads<T, D, false>(b, dinv, wm, wp, yc, zm, i, j, k, nx, ny, cnx, cny, ncd);
+  __device__ Raw fetch(int i, int j, int k, int f) const {
+    const RowOf fixed{zm.row(at(i, j, k), k, f)};
+    return up_loads<T, D, false>(b, dinv, wm, wp, yc, fixed, i, j, k, nx, ny, cnx, cny, ncd);
   }
   __device__ T finish(const Raw &r) const { return up_value<T>(r, w); }
   __device__ void out(const Coef7<T> &a, const Raw &own, int i, int j, int k, const Nbr<T> &nb) {

@@ -510,11 +510,11 @@ struct DLevel {
   void *wm = nullptr, *wp = nullptr;
   void *x = nullptr, *y = nullptr, *b = nullptr, *r = nullptr, *tmp = nullptr;
   // What the fused down/up kernels read: hot copies of the coefficients, 1/diag
-  // and weights when the level is z-extruded (zm.pl != 0; xf flags, nexc of
-  // them), else aliases of the full arrays.  own_hc / own_hw: hc, hd / hwm, hwp
+  // and weights when the level is z-extruded (zm.pl != 0; xf flag words for the
+  // nexc flagged points), else aliases of the full arrays.  own_hc / own_hw: hc, hd / hwm, hwp
   // allocated for the level.
   void *hc = nullptr, *hd = nullptr, *hwm = nullptr, *hwp = nullptr;
-  unsigned char *xf = nullptr;
+  unsigned long long *xf = nullptr;
   long long hpitch = 0, nexc = 0;
   ZMap zm;
   bool own_hc = false, own_hw = false;
@@ -646,16 +646,18 @@ int build_hot(const HLevel &h, size_t l, DLevel &dl) {
     if ((rc = upload<T>(&dl.hwp, hwp, hwp.size(), hwp.size()))) return rc;
   }
   if (plan.nexc) {
-    FPC_HIP(hipMalloc(&dl.xf, (size_t)h.N));
-    FPC_HIP(hipMemcpyAsync(dl.xf, plan.xf.data(), (size_t)h.N, hipMemcpyHostToDevice, stream()));
-    FPC_HIP(hipStreamSynchronize(stream()));  // plan.xf is released on return
+    const std::vector<unsigned long long> words = flag_words(plan, h.n[0], h.n[1], h.n[2]);
+    FPC_HIP(hipMalloc(&dl.xf, words.size() * sizeof(unsigned long long)));
+    FPC_HIP(hipMemcpyAsync(dl.xf, words.data(), words.size() * sizeof(unsigned long long), hipMemcpyHostToDevice,
+                           stream()));
+    FPC_HIP(hipStreamSynchronize(stream()));  // words is released on return
   }
   dl.nexc = plan.nexc;
   dl.zm.pl = (long long)h.n[0] * h.n[1];
   dl.zm.nz = h.n[2];
   dl.zm.m = plan.m;
   dl.zm.xoff = plan.xoff;
-  dl.zm.xf = dl.xf;
+  dl.zm.xb = dl.xf;
   return 0;
 }
 

@@ -26,6 +26,9 @@
 namespace fpc {
 
 constexpr int kTileX = 64;
+// Longest z-chunk of a marching tile: its fetches then span at most two 64-plane flag
+// words per column (march_tile).
+constexpr int kMaxChunk = 62;
 
 // Halo ring point q of a TX x TY tile -> offset (ox, oy) in [-1, TX] x [-1, TY].
 template <int TX, int TY>
@@ -78,8 +81,9 @@ struct Coef7 {
 };
 
 // Walk planes k0 .. kend-1 of tile (bx, by), owned by this workgroup.  The op
-// splits its point function so loads run ahead of use: op.fetch(i, j, k) issues
-// a point's loads (an Op::Raw, with Raw::u the row of its coefficients) and
+// splits its point function so loads run ahead of use: op.fetch(i, j, k, f) issues
+// a point's loads given its hot-copy flag f (an Op::Raw, with Raw::u the row of its
+// coefficients) and
 // op.finish(raw) gives the value its neighbours see; op.out(a, own, i, j, k, nb)
 // writes the point from its coefficients a, its own raw and the neighbour values.
 // The own column is fetched two planes ahead and the halo ring one plane ahead;
@@ -100,11 +104,23 @@ __device__ inline void march_tile(Op &op, int nx, int ny, int nz, int k0, int ke
   const bool ring = tile_ring<TX, TY>(ty * TX + tx, hx, hy);
   const int hi = bx * TX + hx, hj = by * TY + hy;
   const bool ring_in = ring && hi >= 0 && hi < nx && hj >= 0 && hj < ny;
-  T zm = (in && k0 > 0) ? op.finish(op.fetch(i, j, k0 - 1)) : T(0);
+  // The own and ring columns' flag words of a hot copy (bit k & 63 of word k >> 6; 0
+  // without flags).  A chunk's fetches cover planes k0-1 .. kend, at most 65 planes
+  // (chunks are at most kMaxChunk planes), so two words: both loaded here, and a
+  // fetch takes its point's flag from them by a select -- no load per plane, whose
+  // wait would also wait for every prefetch in flight.
+  const int w0 = (k0 > 0 ? k0 - 1 : 0) >> 6, w1 = (w0 + 1) * 64 < nz ? w0 + 1 : w0;
+  const long long co = in ? (long long)j * nx + i : 0, cr = ring_in ? (long long)hj * nx + hi : 0;
+  const unsigned long long o0 = op.zm.word(co, w0), o1 = op.zm.word(co, w1);
+  const unsigned long long r0 = op.zm.word(cr, w0), r1 = op.zm.word(cr, w1);
+  pin_all(o0, o1, r0, r1);  // their wait here, once, not at every use in the loop
+  auto fown = [&](int kk) -> int { return (int)((((kk >> 6) == w0 ? o0 : o1) >> (kk & 63)) & 1ull); };
+  auto fring = [&](int kk) -> int { return (int)((((kk >> 6) == w0 ? r0 : r1) >> (kk & 63)) & 1ull); };
+  T zm = (in && k0 > 0) ? op.finish(op.fetch(i, j, k0 - 1, fown(k0 - 1))) : T(0);
   Raw own0{}, own1{}, ring0{};
-  if (in && k0 < nz) own0 = op.fetch(i, j, k0);
-  if (in && k0 + 1 < nz) own1 = op.fetch(i, j, k0 + 1);
-  if (ring_in && k0 < kend) ring0 = op.fetch(hi, hj, k0);
+  if (in && k0 < nz) own0 = op.fetch(i, j, k0, fown(k0));
+  if (in && k0 + 1 < nz) own1 = op.fetch(i, j, k0 + 1, fown(k0 + 1));
+  if (ring_in && k0 < kend) ring0 = op.fetch(hi, hj, k0, fring(k0));
   T zc = (in && k0 < nz) ? op.finish(own0) : T(0);
   Coef7<T> a;
   long long arow = -1;
@@ -114,8 +130,8 @@ __device__ inline void march_tile(Op &op, int nx, int ny, int nz, int k0, int ke
       a.load(op.c, op.pitch, own0.u);
       arow = own0.u;
     }
-    if (in && k + 2 < nz && k + 1 < kend) own2 = op.fetch(i, j, k + 2);
-    if (ring_in && k + 1 < kend) ring1 = op.fetch(hi, hj, k + 1);
+    if (in && k + 2 < nz && k + 1 < kend) own2 = op.fetch(i, j, k + 2, fown(k + 2));
+    if (ring_in && k + 1 < kend) ring1 = op.fetch(hi, hj, k + 1, fring(k + 1));
     const T zp = (in && k + 1 < nz) ? op.finish(own1) : T(0);
     T(*P)[TX + 2] = plane[(k - k0) & 1];
     P[ty + 1][tx + 1] = zc;
@@ -185,7 +201,8 @@ inline int tile_kz(const Dims &d, int ty) {
     long long chunks = std::max(1LL, 2048 / std::max(1LL, tiles));
     kz = std::max((int)((d.nz + chunks - 1) / chunks), 4);
   }
-  return kz + (kz & 1);
+  kz += kz & 1;
+  return kz < kMaxChunk ? kz : kMaxChunk;
 }
 
 template <int TY>

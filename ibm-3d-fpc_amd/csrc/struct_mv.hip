@@ -363,18 +363,20 @@ int make_extruded(fpc_StructMatrix_s *A) {
   std::vector<double> hot = hot_planes(h.data(), A->pitch, kSlots, nx, ny, nz, plan, A->zpitch);
   FPC_HIP(hipMalloc(&A->zcoef, hot.size() * sizeof(double)));
   FPC_HIP(hipMalloc(&A->zdinv, (size_t)A->zpitch * sizeof(double)));
-  FPC_HIP(hipMalloc(&A->zxf, (size_t)N));
+  const std::vector<unsigned long long> words = flag_words(plan, nx, ny, nz);
+  FPC_HIP(hipMalloc(&A->zxf, words.size() * sizeof(unsigned long long)));
   FPC_HIP(hipMemcpyAsync(A->zcoef, hot.data(), hot.size() * sizeof(double), hipMemcpyHostToDevice, stream()));
-  FPC_HIP(hipMemcpyAsync(A->zxf, plan.xf.data(), (size_t)N, hipMemcpyHostToDevice, stream()));
+  FPC_HIP(hipMemcpyAsync(A->zxf, words.data(), words.size() * sizeof(unsigned long long), hipMemcpyHostToDevice,
+                         stream()));
   k_dinv<double><<<blocks_for(A->zpitch), 256, 0, stream()>>>(A->zcoef, A->zdinv, A->zpitch);
   FPC_LAUNCH_CHECK();
-  FPC_HIP(hipStreamSynchronize(stream()));  // hot and plan.xf are released on return
+  FPC_HIP(hipStreamSynchronize(stream()));  // hot and words are released on return
   A->zexc = plan.nexc;
   A->zmap.pl = (long long)nx * ny;
   A->zmap.nz = nz;
   A->zmap.m = plan.m;
   A->zmap.xoff = plan.xoff;
-  A->zmap.xf = plan.nexc ? A->zxf : nullptr;
+  A->zmap.xb = plan.nexc ? A->zxf : nullptr;
   return pack_extruded32(A);
 }
 

@@ -214,7 +214,7 @@ int fpc_DeviceSynchronize(void);
  * "tile_min_points": boxes with at least this many points (and nx >= 32,
  * ny >= 8) use the z-marching stencil kernels (default 524288, or the
  * FPC_TILE_MIN_POINTS environment variable); "stencil_kz": z-planes per
- * workgroup of those kernels, 0 = automatic (FPC_STENCIL_KZ); "z_extrude":
+ * workgroup of those kernels, 0 = automatic, at most 62 used (FPC_STENCIL_KZ); "z_extrude":
  * 1 (default) = operators and MG levels whose z-planes 1..nz-2 are equal keep a
  * three-plane copy that the solve loops read, 0 = off (FPC_Z_EXTRUDE); applies
  * to operators assembled and solvers set up afterwards, with identical results. */

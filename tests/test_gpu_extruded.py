@@ -57,8 +57,9 @@ CASES = [  # dims, per-direction coupling scale, expected first coarsening direc
     ((80, 24, 36), (1.0, 1.0, 4.0), 2),
     ((96, 40, 37), (1.0, 4.0, 1.0), 1),
     ((96, 40, 21), (4.0, 1.0, 1.0), 0),
+    ((64, 24, 100), (1.0, 1.0, 4.0), 2),  # two flag words per column (planes 0-63, 64-99)
 ]
-IDS = ["z-odd", "z-even", "y", "x"]
+IDS = ["z-odd", "z-even", "y", "x", "z-deep"]
 
 
 def test_reference_operators_are_extruded():
@@ -122,6 +123,26 @@ def test_extruded_matches_oracle_and_full_storage(dims, scale, cdir, tiles, flag
         assert bits_equal(ext[key], full[key]), key
     for key in ("itp", "itj", "itj32"):
         assert ext[key] == full[key], key
+
+
+@pytest.mark.parametrize("kz", [6, 10])
+def test_flag_words_across_chunks(kz):
+    """Flags of planes 0-63 and 64-99 in two words per column, with z-chunks of the
+    marching tiles that cross plane 64 inside their loop: the same bits as the full
+    planes, run after run."""
+    nx, ny, nz = 64, 24, 100
+    F.set_tuning("tile_min_points", 0)
+    F.set_tuning("stencil_kz", kz)
+    vals = _extruded(nx, ny, nz, (1.0, 1.0, 4.0), seed=41, flagged=0.01)
+    b = np.random.default_rng(42).standard_normal(nx * ny * nz)
+    box = F.StructBox(nx, ny, nz, vals)
+    assert box.extruded()[1] > 0
+    ext = [_calls(box, b) for _ in range(2)]
+    F.set_tuning("z_extrude", 0)
+    full = _calls(F.StructBox(nx, ny, nz, vals), b)
+    for e in ext:
+        for key in ("y", "xp", "xj", "xj32"):
+            assert bits_equal(e[key], full[key]), key
 
 
 @pytest.mark.parametrize("dims,scale,cdir", [CASES[0], CASES[3]], ids=["z-odd", "x"])
