@@ -188,7 +188,7 @@ struct ZMap {
   }
   // The rows of n points at once, in two steps so that other loads can go between
   // them: flags() issues every flag load (box-wide tests once), rows() uses them.
-  // No branch: without flags every lane reads one zero byte, so the loads' wait can
+  // No branch: without flags every lane reads one zero word, so the loads' wait can
   // move down to rows() behind whatever the caller issues in between.
   template <int n>
   __device__ inline void flags(const long long *t, const int *k, unsigned char *f) const {

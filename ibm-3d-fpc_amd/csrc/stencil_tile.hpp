@@ -183,13 +183,16 @@ inline bool tile_worthwhile(const Dims &d) {
 }
 
 // Algorithmic bytes of one stencil pass over A: per_point bytes of vectors at
-// every point (plus the flag byte of a hot copy), and the 7 coefficient planes
-// and a dinv_bytes-wide 1/diag plane (0: none) over the rows read.
+// every point (plus, for a hot copy with flagged points, its flag words: 8 bytes
+// per 64 planes of a column), and the 7 coefficient planes and a dinv_bytes-wide
+// 1/diag plane (0: none) over the rows read.
 // hot: the pass reads the hot copy when A has one (tiled launches), else the full planes.
 inline double stencil_bytes(const fpc_StructMatrix_s *A, int dinv_bytes, double per_point, bool hot) {
   const bool z = hot && A->zmap.pl && tile_worthwhile(dims_of(A->grid));
   const double rows = z ? stored_rows(A->zmap, A->grid.N, A->zexc) : (double)A->grid.N;
-  return (double)A->grid.N * (per_point + (z ? 1.0 : 0.0)) + rows * (8.0 * kSlots + dinv_bytes);
+  const int nz = A->grid.n[2];
+  const double flags = (z && A->zmap.xb) ? 8.0 * ((nz + 63) / 64) / nz : 0.0;
+  return (double)A->grid.N * (per_point + flags) + rows * (8.0 * kSlots + dinv_bytes);
 }
 
 // z-chunk length: about 2048 workgroups in flight, chunks of at least 4 planes,
