@@ -125,11 +125,12 @@ def test_extruded_matches_oracle_and_full_storage(dims, scale, cdir, tiles, flag
         assert ext[key] == full[key], key
 
 
-@pytest.mark.parametrize("kz", [6, 10])
+@pytest.mark.parametrize("kz", [6, 10, 100])
 def test_flag_words_across_chunks(kz):
     """Flags of planes 0-63 and 64-99 in two words per column, with z-chunks of the
-    marching tiles that cross plane 64 inside their loop: the same bits as the full
-    planes, run after run."""
+    marching tiles that cross plane 64 inside their loop (a chunk length above 62 is
+    clamped to 62, so a chunk's planes still span at most two words): the same bits
+    as the full planes, run after run."""
     nx, ny, nz = 64, 24, 100
     F.set_tuning("tile_min_points", 0)
     F.set_tuning("stencil_kz", kz)
