@@ -399,6 +399,7 @@ __device__ inline void reduce_body(const double *__restrict__ part, int n, doubl
 // The same matvec on z-marching tiles (each row of A and z read once).
 template <bool kDinv, typename DT = double>
 struct ArnoldiOp {
+  static constexpr int kAhead = 1;  // prefetch distance of the march (stencil_tile.hpp)
   const double *c;
   long long pitch;
   int nx, ny;

@@ -214,6 +214,7 @@ struct RowOf {
 // Jacobi sweep from zero and residual: r = b - A (w D^-1 b).
 template <typename T>
 struct DownOp {
+  static constexpr int kAhead = 2;  // prefetch distance of the march (stencil_tile.hpp)
   const T *c;
   long long pitch;
   int nx, ny;
@@ -243,6 +244,7 @@ struct DownOp {
 // of a point come with its fetch, ahead of use.
 template <typename T>
 struct DownRestrictZOp {
+  static constexpr int kAhead = 2;  // prefetch distance of the march (stencil_tile.hpp)
   const T *c;
   long long pitch;
   int nx, ny;
@@ -283,6 +285,7 @@ struct DownRestrictZOp {
 // y = x + P e, then y + w D^-1 (b - A y), with x = w D^-1 b recomputed.
 template <typename T, int D>
 struct UpOp {
+  static constexpr int kAhead = 1;  // prefetch distance of the march (stencil_tile.hpp)
   const T *c;
   long long pitch;
   int nx, ny;

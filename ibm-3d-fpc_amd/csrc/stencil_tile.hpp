@@ -86,7 +86,8 @@ struct Coef7 {
 // coefficients) and
 // op.finish(raw) gives the value its neighbours see; op.out(a, own, i, j, k, nb)
 // writes the point from its coefficients a, its own raw and the neighbour values.
-// The own column is fetched two planes ahead and the halo ring one plane ahead;
+// The own column is fetched two planes ahead and the halo ring one plane ahead
+// (three and two for ops with Op::kAhead = 2);
 // a barrier waits only for LDS (s_waitcnt lgkmcnt), so those loads stay in flight
 // across it.  The coefficients stay in registers while the row does not change
 // -- the interior planes of a z-extruded operator's hot copy share one row -- so
@@ -117,21 +118,32 @@ __device__ inline void march_tile(Op &op, int nx, int ny, int nz, int k0, int ke
   auto fown = [&](int kk) -> int { return (int)((((kk >> 6) == w0 ? o0 : o1) >> (kk & 63)) & 1ull); };
   auto fring = [&](int kk) -> int { return (int)((((kk >> 6) == w0 ? r0 : r1) >> (kk & 63)) & 1ull); };
   T zm = (in && k0 > 0) ? op.finish(op.fetch(i, j, k0 - 1, fown(k0 - 1))) : T(0);
-  Raw own0{}, own1{}, ring0{};
-  if (in && k0 < nz) own0 = op.fetch(i, j, k0, fown(k0));
-  if (in && k0 + 1 < nz) own1 = op.fetch(i, j, k0 + 1, fown(k0 + 1));
-  if (ring_in && k0 < kend) ring0 = op.fetch(hi, hj, k0, fring(k0));
+  // Prefetch distance: the own column D+1 planes and the ring D planes ahead.  Op::kAhead
+  // is 2 for ops whose raws are small enough to keep one more plane of each in flight
+  // at the same occupancy (the down sweeps), else 1.
+  constexpr int D = Op::kAhead;
+  static_assert(D == 1 || D == 2, "prefetch distance 1 or 2");
+  Raw own[D + 2]{}, ringq[D + 1]{};
+#pragma unroll
+  for (int q = 0; q <= D; ++q)
+    if (in && k0 + q < nz && (q < 2 || k0 + q - 1 < kend)) own[q] = op.fetch(i, j, k0 + q, fown(k0 + q));
+#pragma unroll
+  for (int q = 0; q < D; ++q)
+    if (ring_in && k0 + q < kend) ringq[q] = op.fetch(hi, hj, k0 + q, fring(k0 + q));
+  Raw &own0 = own[0], &own1 = own[1];
   T zc = (in && k0 < nz) ? op.finish(own0) : T(0);
   Coef7<T> a;
   long long arow = -1;
   for (int k = k0; k < kend; ++k) {
-    Raw own2{}, ring1{};
     if (in && own0.u != arow) {
       a.load(op.c, op.pitch, own0.u);
       arow = own0.u;
     }
-    if (in && k + 2 < nz && k + 1 < kend) own2 = op.fetch(i, j, k + 2, fown(k + 2));
-    if (ring_in && k + 1 < kend) ring1 = op.fetch(hi, hj, k + 1, fring(k + 1));
+    own[D + 1] = Raw{};
+    ringq[D] = Raw{};
+    if (in && k + D + 1 < nz && k + D < kend) own[D + 1] = op.fetch(i, j, k + D + 1, fown(k + D + 1));
+    if (ring_in && k + D < kend) ringq[D] = op.fetch(hi, hj, k + D, fring(k + D));
+    Raw &ring0 = ringq[0];
     const T zp = (in && k + 1 < nz) ? op.finish(own1) : T(0);
     T(*P)[TX + 2] = plane[(k - k0) & 1];
     P[ty + 1][tx + 1] = zc;
@@ -144,9 +156,10 @@ __device__ inline void march_tile(Op &op, int nx, int ny, int nz, int k0, int ke
     }
     zm = zc;
     zc = zp;
-    own0 = own1;
-    own1 = own2;
-    ring0 = ring1;
+#pragma unroll
+    for (int q = 0; q <= D; ++q) own[q] = own[q + 1];
+#pragma unroll
+    for (int q = 0; q < D; ++q) ringq[q] = ringq[q + 1];
   }
 }
 
