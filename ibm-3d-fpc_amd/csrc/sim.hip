@@ -431,6 +431,12 @@ int fpc_sim_device_ptr(fpc_sim M, const char *name, void **ptr) {
 
 void *fpc_sim_stream(fpc_sim M) { return M ? (void *)M->stream : nullptr; }
 
+int fpc_sim_flow_ops(fpc_sim M, fpc_flow_ops *ops) {
+  FPC_CHECK_ARG(M && ops, "null argument");
+  *ops = M->ops;
+  return 0;
+}
+
 double fpc_sim_scalar(fpc_sim M, const char *name) {
   if (!M) return NAN;
   std::string n(name);
@@ -446,6 +452,18 @@ double fpc_sim_scalar(fpc_sim M, const char *name) {
   double ub = 4.0 * P.Um / 9.0, den = P.rho * ub * ub * (2.0 * P.Rcirc) * P.Lz;
   if (n == "cd") return 2.0 * M->hdiag[4] / den;
   if (n == "cl") return 2.0 * M->hdiag[5] / den;
+  // hot copy of each operator (planned at its first solve): wall planes m (0: full
+  // planes) and flagged points
+  const char *zm[4] = {"zext_m_u", "zext_m_v", "zext_m_w", "zext_m_p"};
+  const char *zf[4] = {"zext_flagged_u", "zext_flagged_v", "zext_flagged_w", "zext_flagged_p"};
+  const fpc_StructMatrix ops[4] = {M->Au, M->Av, M->Aw, M->Ap};
+  for (int c = 0; c < 4; ++c) {
+    if (n != zm[c] && n != zf[c]) continue;
+    int m = 0;
+    long long f = 0;
+    if (fpc_StructMatrixGetExtruded(ops[c], &m, &f)) return NAN;
+    return n == zm[c] ? (double)m : (double)f;
+  }
   const char *its[4] = {"its_u", "its_v", "its_w", "its_p"};
   const char *rel[4] = {"relres_u", "relres_v", "relres_w", "relres_p"};
   const char *err[4] = {"err_u", "err_v", "err_w", "err_p"};

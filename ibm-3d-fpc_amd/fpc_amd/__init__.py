@@ -55,6 +55,19 @@ class Params(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in PARAM_FIELDS}
 
 
+class FlowOps(ctypes.Structure):
+    """fpc_flow_ops (include/fpc_dense.h): a channel's operator data for the fused stages,
+    device pointers."""
+    _fields_ = ([(n, ctypes.c_int) for n in ("nx", "ny", "nz")]
+                + [(n, ctypes.c_double) for n in ("dx", "dy", "dz", "rho", "dt")]
+                + [(n, ctypes.c_void_p) for n in ("active_u", "active_v", "active_w", "fluid_p", "active_qu",
+                                                   "active_qv", "active_qw", "scale_u", "scale_v", "scale_w",
+                                                   "rhs_u_bc", "dcoef", "div_bc")]
+                + [(n, ctypes.c_int) for n in ("n_fu", "n_fv", "n_px", "n_py")]
+                + [(n, ctypes.c_void_p) for n in ("fu_idx", "fv_idx", "px_idx", "py_idx", "fu_w", "fv_w", "px_w",
+                                                   "py_w")])
+
+
 def build(jobs=8):
     """Compile libfpc.so for gfx950 with the package Makefile (hipcc + g++)."""
     subprocess.run(["make", "-s", "-C", CSRC, f"-j{jobs}"], check=True)
@@ -150,7 +163,8 @@ SIGNATURES = {
     "fpc_add_pressure_phi": (_i, [_vp] * 4 + [_i]), "fpc_zero_doubles": (_i, [_vp, _ll]),
     "fpc_stage_explicit": (_i, [_vp] * 12 + [_i]), "fpc_stage_scatter": (_i, [_vp] * 11),
     "fpc_stage_divergence": (_i, [_vp] * 4 + [_d, _vp, _vp]), "fpc_stage_projection": (_i, [_vp] * 11),
-    "fpc_stage_diagnostics": (_i, [_vp] * 6 + [_d, _i, _vp]), "fpc_any_nonfinite": (_i, [_vp, _ll, _pi]),
+    "fpc_stage_diagnostics": (_i, [_vp] * 6 + [_d, _i, _vp]),
+    "fpc_any_nonfinite": (_i, [_vp, _ll, _pi]),
     # fpc_sim.h
     "fpc_default_params": (None, [_PP]), "fpc_parse_args": (_i, [_i, ctypes.POINTER(_cp), _PP]),
     "fpc_setup_create": (_i, [_PP, _pvp]), "fpc_setup_len": (_ll, [_vp, _cp]), "fpc_setup_elsize": (_i, [_vp, _cp]),
@@ -159,7 +173,7 @@ SIGNATURES = {
     "fpc_sim_len": (_ll, [_vp, _cp]), "fpc_sim_copy": (_i, [_vp, _cp, _vp, _ll]),
     "fpc_sim_set": (_i, [_vp, _cp, _vp, _ll]), "fpc_sim_device_ptr": (_i, [_vp, _cp, _pvp]),
     "fpc_sim_scalar": (_d, [_vp, _cp]), "fpc_sim_timer": (_d, [_vp, _cp]), "fpc_sim_destroy": (_i, [_vp]),
-    "fpc_sim_stream": (_vp, [_vp]),
+    "fpc_sim_stream": (_vp, [_vp]), "fpc_sim_flow_ops": (_i, [_vp, ctypes.POINTER(FlowOps)]),
     "fpc_ensemble_create": (_i, [_PP, _i, _i, _pvp]), "fpc_ensemble_step": (_i, [_vp, _i]),
     "fpc_ensemble_size": (_i, [_vp]), "fpc_ensemble_member": (_vp, [_vp, _i]),
     "fpc_ensemble_tiles": (_i, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_longlong)]),
@@ -314,6 +328,16 @@ class Sim:
 
     def timer(self, name):
         return float(lib().fpc_sim_timer(self.h, name.encode()))
+
+    def flow_ops(self):
+        """The operator data of the fused stages (FlowOps, device pointers owned by the sim)."""
+        o = FlowOps()
+        check(lib().fpc_sim_flow_ops(self.h, ctypes.byref(o)), "fpc_sim_flow_ops")
+        return o
+
+    def stream(self):
+        """The simulation's HIP stream (its calls run there)."""
+        return lib().fpc_sim_stream(self.h)
 
 
 def set_tuning(key, value):

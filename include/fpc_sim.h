@@ -15,6 +15,8 @@
 #ifndef FPC_SIM_H
 #define FPC_SIM_H
 
+#include "fpc_dense.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -81,8 +83,13 @@ int fpc_sim_set(fpc_sim sim, const char *name, const void *src, long long nbytes
 int fpc_sim_device_ptr(fpc_sim sim, const char *name, void **ptr);
 /* The simulation's HIP stream (a hipStream_t). */
 void *fpc_sim_stream(fpc_sim sim);
+/* The simulation's operator data for the fused stages of fpc_dense.h (device pointers
+ * owned by the simulation; valid until fpc_sim_destroy). */
+int fpc_sim_flow_ops(fpc_sim sim, fpc_flow_ops *ops);
 /* step ramp div_l2 umax vmax wmax its_u its_v its_w its_p relres_u relres_v relres_w
- * relres_p fx fy cd cl err_u err_v err_w err_p; NaN if unknown. */
+ * relres_p fx fy cd cl err_u err_v err_w err_p; zext_m_c and zext_flagged_c (c = u, v, w, p):
+ * wall planes (0 = full planes) and flagged points of the hot copy the operator's solve
+ * loops read (planned at its first solve); NaN if unknown. */
 double fpc_sim_scalar(fpc_sim sim, const char *name);
 /* Per-stage wall time (s) accumulated over steps: t_build t_mom t_p t_proj t_diag */
 double fpc_sim_timer(fpc_sim sim, const char *name);

@@ -23,7 +23,7 @@ import pytest
 
 import fpc_amd as F
 import oracle as O
-from helpers import SMALL, bits_equal, box_csr, rel_l2
+from helpers import CFG1, SMALL, bits_equal, box_csr, rel_l2
 from test_gpu_tiles import _aniso
 
 pytestmark = pytest.mark.gpu
@@ -161,12 +161,22 @@ def test_extruded_fp32_vcycle_equals_full_storage(dims, scale, cdir):
 
 
 def test_simulation_steps_equal_full_storage():
-    kw = dict(SMALL, nsteps=4)
+    """128x32x32 with the z-marching kernels on every box of at least 32768 points: with
+    the copies on, the tiled matvecs and V-cycle levels read the hot copies (the pressure
+    operator's with flagged points); with them off, the full planes.  Same bits."""
+    kw = dict(CFG1, nsteps=4, init_profile=1, noise_amp=1e-3 * 0.45, seed=2)
+    F.set_tuning("tile_min_points", 0)
     out = []
     for flag in (1, 0):
         F.set_tuning("z_extrude", flag)
         sim = F.Sim(F.default_params(**kw))
         sim.step(4)
+        ext = {c: (sim.scalar("zext_m_" + c), sim.scalar("zext_flagged_" + c)) for c in "uvwp"}
+        if flag:
+            assert all(m >= 1 for m, _ in ext.values()), ext
+            assert ext["p"][1] > 0, ext
+        else:
+            assert all(m == 0 for m, _ in ext.values()), ext
         out.append({k: sim.get(k) for k in ("u", "v", "w", "p")})
     for k in ("u", "v", "w", "p"):
         assert bits_equal(out[0][k], out[1][k]), k

@@ -89,12 +89,35 @@ def test_run_matches_oracle(torch_cuda):
     assert abs(so["cl"] - sg["cl"]) <= 1e-7 * abs(so["cd"])
 
 
+@pytest.fixture
+def tiles_everywhere(torch_cuda):
+    """The z-marching tile kernels on every box of at least 32768 points (the floor of
+    the tiled launches), instead of the default 512K: at 128x32x32 the matvecs and the
+    first V-cycle levels then run the production kernels, reading the hot copies of the
+    operators with their flag words."""
+    F.set_tuning("tile_min_points", 0)
+    yield
+    F.set_tuning("tile_min_points", 1 << 19)
+
+
+def _assert_hot_copies(g):
+    """The four operators are read through their hot copies; the pressure operator's
+    copy has flagged points (rows that differ in the last bit from the interior plane)."""
+    for c in "uvwp":
+        assert g.scalar("zext_m_" + c) >= 1, c
+    assert g.scalar("zext_flagged_p") > 0
+
+
+@pytest.mark.parametrize("tiles", [False, True], ids=["default", "tiled"])
 @pytest.mark.parametrize("tol", [1e-12, 1e-8])
-def test_reference_cpu_case_matches_oracle(torch_cuda, tol):
+def test_reference_cpu_case_matches_oracle(torch_cuda, request, tol, tiles):
     """The reference's CPU-runnable case: 128x32x32, Um 0.45 (Re ~20), dt 1e-3, 20 steps,
     inlet profile + seeded noise.  At rtol 1e-12 the fields must agree within 1e-9; at the
     case's own rtol 1e-8 both solvers take the same iteration counts, so their iterates
-    differ by rounding only and the same bar holds."""
+    differ by rounding only and the same bar holds.  "tiled": the same with the
+    z-marching kernels on the hot copies (the kernels the 512x96x96 production run uses)."""
+    if tiles:
+        request.getfixturevalue("tiles_everywhere")
     kw = dict(CFG1, init_profile=1, noise_amp=1e-3 * 0.45, seed=0, vel_tol=tol, p_tol=tol, vel_maxit=500,
               p_maxit=500)
     o, g = _pair(kw)
@@ -108,6 +131,8 @@ def test_reference_cpu_case_matches_oracle(torch_cuda, tol):
         assert rel_l2(g.get(f), o.get(f)) < 1e-9, f
     assert abs(so["cd"] - sg["cd"]) <= 1e-7 * abs(so["cd"])
     assert abs(so["cl"] - sg["cl"]) <= 1e-7 * abs(so["cd"])
+    if tiles:
+        _assert_hot_copies(g)
 
 
 def test_reference_defaults_zero_start(torch_cuda):
@@ -131,6 +156,20 @@ def test_deterministic(torch_cuda):
         assert bits_equal(a.get(f), b.get(f)), f
 
 
+def test_deterministic_tiled(tiles_everywhere):
+    """Run-to-run bits of the z-marching kernels on the hot copies (flag words, LDS double
+    buffer, prefetch ring): two runs of 128x32x32 give the same fields bit for bit."""
+    kw = dict(CFG1, init_profile=1, noise_amp=1e-3 * 0.45, seed=5)
+    out = []
+    for _ in range(2):
+        g = F.Sim(F.default_params(**kw))
+        g.step(4)
+        _assert_hot_copies(g)
+        out.append([g.get(f) for f in FIELDS] + [np.array([g.scalar("its_p"), g.scalar("relres_p")])])
+    for f, a, b in zip(FIELDS + ("its",), out[0], out[1]):
+        assert bits_equal(a, b), f
+
+
 def test_mixed_precision_close_to_fp64(torch_cuda):
     kw = dict(SMALL, init_profile=1, noise_amp=1e-3 * 0.45, seed=0, inlet_ramp_steps=4, vel_tol=1e-8, p_tol=1e-8)
     a = F.Sim(F.default_params(**kw))
@@ -139,6 +178,21 @@ def test_mixed_precision_close_to_fp64(torch_cuda):
     b.step(5)
     assert b.scalar("relres_p") <= 1e-8
     for f in ("u", "v", "w"):
+        assert rel_l2(b.get(f), a.get(f)) < 1e-6, f
+    assert abs(a.scalar("cd") - b.scalar("cd")) <= 1e-6 * abs(a.scalar("cd"))
+
+
+def test_mixed_precision_tiled_close_to_fp64(tiles_everywhere):
+    """FP32 V-cycle on the z-marching kernels and hot copies, inside FP64 flexible GMRES:
+    the FP64 1e-8 residual is reached and the fields stay within 1e-6 of the FP64 run."""
+    kw = dict(CFG1, init_profile=1, noise_amp=1e-3 * 0.45, seed=0, inlet_ramp_steps=4, vel_tol=1e-8, p_tol=1e-8)
+    a = F.Sim(F.default_params(**kw))
+    b = F.Sim(F.default_params(**dict(kw, mg_precision=32)))
+    a.step(5)
+    b.step(5)
+    _assert_hot_copies(b)
+    assert b.scalar("relres_p") <= 1e-8
+    for f in FIELDS:
         assert rel_l2(b.get(f), a.get(f)) < 1e-6, f
     assert abs(a.scalar("cd") - b.scalar("cd")) <= 1e-6 * abs(a.scalar("cd"))
 
