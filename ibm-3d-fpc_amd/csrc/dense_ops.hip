@@ -13,10 +13,17 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "fpc_dense.h"
 #include "dense_device.hpp"
 #include "fpc_internal.hpp"
+
+// Partial sums of fpc_stage_diagnostics_noalloc: kDiagBlocks blocks x 6 sums (96 KiB),
+// one copy per GPU, held until the process ends.  In host code this name is a host-side
+// stand-in, not a device address: the entry point looks the address up per device
+// (hipGetSymbolAddress) and passes that to the kernels.
+static __device__ __align__(256) double scratch[fpc::kDiagBlocks * 6];
 
 namespace fpc {
 namespace {
@@ -420,17 +427,51 @@ int stage_projection_async(const fpc_flow_ops *o, const double *phi, const doubl
   return 0;
 }
 
-int stage_diagnostics_async(const fpc_flow_ops *o, const double *u, const double *v, const double *w, const double *p,
-                            const double *qout, double ramp, int Np, double *out) {
+// The two diagnostics kernels with their partial sums in part (kDiagBlocks * 6 doubles).
+static int diagnostics_launch(const fpc_flow_ops *o, const double *u, const double *v, const double *w,
+                              const double *p, const double *qout, double ramp, int Np, double *out, double *part) {
   const int NBD = diag_blocks(o->nx, o->ny, o->nz);
-  double *part = scratch_doubles((size_t)kDiagBlocks * 6, 6);
-  if (!part) return FPC_ERROR_MEMORY;
   k_diag_partials<<<NBD, 256, 0, stream()>>>(*o, u, v, w, p, qout, ramp, part);
   FPC_LAUNCH_CHECK();
   k_diag_final<<<1, 1024, 0, stream()>>>(part, NBD, Np, out);
   FPC_LAUNCH_CHECK();
   return 0;
 }
+
+int stage_diagnostics_async(const fpc_flow_ops *o, const double *u, const double *v, const double *w, const double *p,
+                            const double *qout, double ramp, int Np, double *out) {
+  double *part = scratch_doubles((size_t)kDiagBlocks * 6, 6);
+  if (!part) return FPC_ERROR_MEMORY;
+  return diagnostics_launch(o, u, v, w, p, qout, ramp, Np, out, part);
+}
+
+namespace {
+// Device address of `scratch` on each GPU, looked up once per device on its first call.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_on_current_device(double **ptr) {
+  int dev = -1;
+  FPC_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error(FPC_ERROR_GENERIC, "device %d beyond the %d devices of the scratch table", dev, kMaxDevices);
+    return FPC_ERROR_GENERIC;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *a = nullptr;
+    hipError_t e = hipGetSymbolAddress(&a, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !a) {
+      set_error(FPC_ERROR_GENERIC, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+      return FPC_ERROR_GENERIC;
+    }
+    g_scratch_ptr[dev] = (double *)a;
+  }
+  *ptr = g_scratch_ptr[dev];
+  return 0;
+}
+}  // namespace
 }  // namespace fpc
 
 extern "C" {
@@ -471,6 +512,20 @@ int fpc_stage_projection(const fpc_flow_ops *o, const double *phi, const double 
 int fpc_stage_diagnostics(const fpc_flow_ops *o, const double *u, const double *v, const double *w, const double *p,
                           const double *qout, double ramp, int Np, double *out) {
   return stage_diagnostics_async(o, u, v, w, p, qout, ramp, Np, out);
+}
+
+int fpc_stage_diagnostics_noalloc(const fpc_flow_ops *o, const double *u, const double *v, const double *w,
+                                  const double *p, const double *qout, double ramp, int Np, double *out) {
+  FPC_CHECK_ARG(o && u && v && w && p && qout && out, "null argument");
+  FPC_CHECK_ARG(o->nx > 0 && o->ny > 0 && o->nz > 0 && Np > 0, "empty grid");
+  // the stream must exist already: creating the library stream here would allocate
+  FPC_CHECK_ARG(stream_ready(), "no stream yet: call fpc_Initialize or fpc_SetStream first");
+  const int NBD = diag_blocks(o->nx, o->ny, o->nz);
+  FPC_CHECK_ARG(NBD >= 1 && NBD <= kDiagBlocks, "diagnostics partition beyond the 2048-block scratch");
+  double *part = nullptr;
+  int rc = scratch_on_current_device(&part);
+  if (rc) return rc;
+  return diagnostics_launch(o, u, v, w, p, qout, ramp, Np, out, part);
 }
 
 int fpc_any_nonfinite(const double *x, long long n, int *flag) {

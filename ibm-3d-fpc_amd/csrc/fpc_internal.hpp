@@ -31,6 +31,9 @@ namespace fpc {
 void set_error(int bits, const char *fmt, ...);
 int error_bits();
 hipStream_t stream();
+// Whether stream() has a stream without creating one (a handle's scoped stream, a stream
+// adopted by fpc_SetStream, or the library stream made by fpc_Initialize).
+bool stream_ready();
 void ensure_init();
 // Frees the calling thread's scratch buffers (worker threads, before they exit).
 void release_thread_state();

@@ -73,6 +73,8 @@ hipStream_t stream() {
   return g_stream;
 }
 
+bool stream_ready() { return t_state.scoped != nullptr || g_init.load(std::memory_order_acquire); }
+
 StreamScope::StreamScope(hipStream_t s) : prev_(t_state.scoped) { t_state.scoped = s; }
 StreamScope::~StreamScope() { t_state.scoped = prev_; }
 

@@ -164,6 +164,7 @@ SIGNATURES = {
     "fpc_stage_explicit": (_i, [_vp] * 12 + [_i]), "fpc_stage_scatter": (_i, [_vp] * 11),
     "fpc_stage_divergence": (_i, [_vp] * 4 + [_d, _vp, _vp]), "fpc_stage_projection": (_i, [_vp] * 11),
     "fpc_stage_diagnostics": (_i, [_vp] * 6 + [_d, _i, _vp]),
+    "fpc_stage_diagnostics_noalloc": (_i, [_vp] * 6 + [_d, _i, _vp]),
     "fpc_any_nonfinite": (_i, [_vp, _ll, _pi]),
     # fpc_sim.h
     "fpc_default_params": (None, [_PP]), "fpc_parse_args": (_i, [_i, ctypes.POINTER(_cp), _PP]),

@@ -114,6 +114,17 @@ int fpc_stage_projection(const fpc_flow_ops *ops, const double *phi, const doubl
  * out is a device array of 8 doubles; asynchronous (no host wait). */
 int fpc_stage_diagnostics(const fpc_flow_ops *ops, const double *u, const double *v, const double *w,
                           const double *p, const double *qout, double ramp, int Np, double *out);
+/* fpc_stage_diagnostics without any allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  The block partial sums live in a static
+ * device array of the library (2048 blocks x 6 doubles = 96 KiB per GPU, held until the
+ * process ends); every grid fits, as the partition never exceeds 2048 blocks (a larger
+ * one is rejected with FPC_ERROR_ARG).  The stream must exist already (fpc_Initialize
+ * or fpc_SetStream), else FPC_ERROR_ARG.  All calls on one GPU share that array: calls
+ * that overlap on one GPU (two streams, two host threads) race.  Same results, bit for
+ * bit, as fpc_stage_diagnostics; asynchronous. */
+int fpc_stage_diagnostics_noalloc(const fpc_flow_ops *ops, const double *u, const double *v, const double *w,
+                                  const double *p, const double *qout, double ramp, int Np, double *out);
 /* Non-finite check of a device vector (any_nonfinite_raw0, solver_helpers.hpp:331-335):
  * *flag (host) = 1 if any entry is NaN or Inf. */
 int fpc_any_nonfinite(const double *x, long long n, int *flag);
