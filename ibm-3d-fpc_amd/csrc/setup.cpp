@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <limits>
 #include <string>
@@ -768,6 +769,8 @@ void fpc_default_params(fpc_params *p) {
   p->p_pc_maxit = 1; p->p_pc_relax_type = 1; p->p_pc_rap_type = 1;
   p->init_profile = 0; p->noise_amp = 0.0; p->seed = 0; p->mg_precision = 64;
   p->vel_print = 0; p->p_print = 0;
+  p->write_vtk = 1; p->vtk_every = 0; p->device = 0;
+  std::snprintf(p->vtk_filename, sizeof(p->vtk_filename), "%s", "st3d_channel_cylinder_struct_mac_cuda.vtk");
 }
 
 int fpc_parse_args(int argc, char **argv, fpc_params *p) {
@@ -813,6 +816,10 @@ int fpc_parse_args(int argc, char **argv, fpc_params *p) {
     else if (!std::strcmp(f, "-noise-amp")) p->noise_amp = std::atof(v);
     else if (!std::strcmp(f, "-seed")) p->seed = std::strtoull(v, nullptr, 10);
     else if (!std::strcmp(f, "-mg-precision")) p->mg_precision = std::atoi(v);
+    else if (!std::strcmp(f, "-write-vtk")) p->write_vtk = std::atoi(v);
+    else if (!std::strcmp(f, "-vtk-every")) p->vtk_every = std::atoi(v);
+    else if (!std::strcmp(f, "-device")) p->device = std::atoi(v);
+    else if (!std::strcmp(f, "-vtk")) std::snprintf(p->vtk_filename, sizeof(p->vtk_filename), "%s", v);
     else used = false;
     if (used) ++a;
   }

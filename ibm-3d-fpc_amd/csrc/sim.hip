@@ -193,11 +193,6 @@ int fpc::sim_run_phase(fpc_sim M, int ph) {
       FPC_HIP(hipMemcpyAsync(M->hdiag, M->diag, 8 * sizeof(double), hipMemcpyDeviceToHost, st));
       FPC_HIP(hipStreamSynchronize(st));
       M->t_diag += now_s() - t0;
-      if (par.print_every > 0 && (M->step == 1 || M->step == par.nsteps || M->step % par.print_every == 0) &&
-          (par.vel_print > 0 || par.p_print > 0))
-        std::printf("%4d  %.6f  %.6e  %.6e  %.6e  %.6e  (%d,%d,%d,%d)  (%.3e,%.3e,%.3e,%.3e)\n", M->step, M->ramp,
-                    M->hdiag[1], M->hdiag[2], M->hdiag[3], M->hdiag[0], M->its[0], M->its[1], M->its[2], M->its[3],
-                    M->rel[0], M->rel[1], M->rel[2], M->rel[3]);
       return 0;
     }
     default:
@@ -431,6 +426,39 @@ int fpc_sim_device_ptr(fpc_sim M, const char *name, void **ptr) {
 
 void *fpc_sim_stream(fpc_sim M) { return M ? (void *)M->stream : nullptr; }
 
+// The simulation's u, v, w, p on the host (after its stream's work).
+static int fields_to_host(fpc_sim M, std::vector<double> *f) {
+  const fpc_params &P = M->par;
+  const size_t n[4] = {(size_t)(P.nx + 1) * P.ny * P.nz, (size_t)P.nx * (P.ny + 1) * P.nz,
+                       (size_t)P.nx * P.ny * (P.nz + 1), (size_t)P.nx * P.ny * P.nz};
+  const double *d[4] = {M->u, M->v, M->w, M->p};
+  for (int c = 0; c < 4; ++c) {
+    f[c].resize(n[c]);
+    FPC_HIP(hipMemcpyAsync(f[c].data(), d[c], n[c] * sizeof(double), hipMemcpyDeviceToHost, M->stream));
+  }
+  FPC_HIP(hipStreamSynchronize(M->stream));
+  return 0;
+}
+
+int fpc_sim_write_vtk(fpc_sim M, const char *filename) {
+  FPC_CHECK_ARG(M && filename, "null argument");
+  std::vector<double> f[4];
+  int rc = fields_to_host(M, f);
+  if (rc) return rc;
+  const fpc_params &P = M->par;
+  return fpc_write_vtk_cellcentered(filename, f[0].data(), f[1].data(), f[2].data(), f[3].data(), M->S.fluid_p.data(),
+                                    P.nx, P.ny, P.nz, P.Lx, P.Ly, P.Lz);
+}
+
+int fpc_sim_write_fields(fpc_sim M, const char *path) {
+  FPC_CHECK_ARG(M && path, "null argument");
+  std::vector<double> f[4];
+  int rc = fields_to_host(M, f);
+  if (rc) return rc;
+  const fpc_params &P = M->par;
+  return fpc_write_fields(path, M->step, P.nx, P.ny, P.nz, f[0].data(), f[1].data(), f[2].data(), f[3].data());
+}
+
 int fpc_sim_flow_ops(fpc_sim M, fpc_flow_ops *ops) {
   FPC_CHECK_ARG(M && ops, "null argument");
   *ops = M->ops;
@@ -452,6 +480,9 @@ double fpc_sim_scalar(fpc_sim M, const char *name) {
   double ub = 4.0 * P.Um / 9.0, den = P.rho * ub * ub * (2.0 * P.Rcirc) * P.Lz;
   if (n == "cd") return 2.0 * M->hdiag[4] / den;
   if (n == "cl") return 2.0 * M->hdiag[5] / den;
+  const char *counts[] = {"Np", "Nu", "Nv", "Nw", "Nqu", "Nqv", "Nqw", "n_pruned"};
+  for (const char *c : counts)
+    if (n == c) return (double)M->S.count(c);
   // hot copy of each operator (planned at its first solve): wall planes m (0: full
   // planes) and flagged points
   const char *zm[4] = {"zext_m_u", "zext_m_v", "zext_m_w", "zext_m_p"};

@@ -45,6 +45,8 @@ PARAM_FIELDS = [
     ("p_pc_maxit", ctypes.c_int), ("p_pc_relax_type", ctypes.c_int), ("p_pc_rap_type", ctypes.c_int),
     ("init_profile", ctypes.c_int), ("noise_amp", ctypes.c_double), ("seed", ctypes.c_ulonglong),
     ("mg_precision", ctypes.c_int), ("vel_print", ctypes.c_int), ("p_print", ctypes.c_int),
+    ("write_vtk", ctypes.c_int), ("vtk_every", ctypes.c_int), ("device", ctypes.c_int),
+    ("vtk_filename", ctypes.c_char * 256),
 ]
 
 
@@ -164,8 +166,7 @@ SIGNATURES = {
     "fpc_stage_explicit": (_i, [_vp] * 12 + [_i]), "fpc_stage_scatter": (_i, [_vp] * 11),
     "fpc_stage_divergence": (_i, [_vp] * 4 + [_d, _vp, _vp]), "fpc_stage_projection": (_i, [_vp] * 11),
     "fpc_stage_diagnostics": (_i, [_vp] * 6 + [_d, _i, _vp]),
-    "fpc_stage_diagnostics_noalloc": (_i, [_vp] * 6 + [_d, _i, _vp]),
-    "fpc_any_nonfinite": (_i, [_vp, _ll, _pi]),
+    "fpc_stage_diagnostics_noalloc": (_i, [_vp] * 6 + [_d, _i, _vp]), "fpc_any_nonfinite": (_i, [_vp, _ll, _pi]),
     # fpc_sim.h
     "fpc_default_params": (None, [_PP]), "fpc_parse_args": (_i, [_i, ctypes.POINTER(_cp), _PP]),
     "fpc_setup_create": (_i, [_PP, _pvp]), "fpc_setup_len": (_ll, [_vp, _cp]), "fpc_setup_elsize": (_i, [_vp, _cp]),
@@ -174,6 +175,10 @@ SIGNATURES = {
     "fpc_sim_len": (_ll, [_vp, _cp]), "fpc_sim_copy": (_i, [_vp, _cp, _vp, _ll]),
     "fpc_sim_set": (_i, [_vp, _cp, _vp, _ll]), "fpc_sim_device_ptr": (_i, [_vp, _cp, _pvp]),
     "fpc_sim_scalar": (_d, [_vp, _cp]), "fpc_sim_timer": (_d, [_vp, _cp]), "fpc_sim_destroy": (_i, [_vp]),
+    "fpc_write_vtk_cellcentered": (_i, [_cp] + [_vp] * 5 + [_i] * 3 + [_d] * 3),
+    "fpc_vtk_filename_with_step": (_i, [_cp, _i, _cp, _i]),
+    "fpc_write_fields": (_i, [_cp] + [_i] * 4 + [_vp] * 4),
+    "fpc_sim_write_vtk": (_i, [_vp, _cp]), "fpc_sim_write_fields": (_i, [_vp, _cp]),
     "fpc_sim_stream": (_vp, [_vp]), "fpc_sim_flow_ops": (_i, [_vp, ctypes.POINTER(FlowOps)]),
     "fpc_ensemble_create": (_i, [_PP, _i, _i, _pvp]), "fpc_ensemble_step": (_i, [_vp, _i]),
     "fpc_ensemble_size": (_i, [_vp]), "fpc_ensemble_member": (_vp, [_vp, _i]),
@@ -330,6 +335,14 @@ class Sim:
     def timer(self, name):
         return float(lib().fpc_sim_timer(self.h, name.encode()))
 
+    def write_vtk(self, filename):
+        """The reference driver's cell-centred VTK file of the current fields."""
+        check(lib().fpc_sim_write_vtk(self.h, os.fsencode(filename)), "fpc_sim_write_vtk")
+
+    def write_fields(self, path):
+        """Raw dump of u, v, w, p (read_fields reads it back)."""
+        check(lib().fpc_sim_write_fields(self.h, os.fsencode(path)), "fpc_sim_write_fields")
+
     def flow_ops(self):
         """The operator data of the fused stages (FlowOps, device pointers owned by the sim)."""
         o = FlowOps()
@@ -339,6 +352,47 @@ class Sim:
     def stream(self):
         """The simulation's HIP stream (its calls run there)."""
         return lib().fpc_sim_stream(self.h)
+
+
+def vtk_filename_with_step(base, step):
+    """base with "_%06d" (step) before its extension (solver_helpers.hpp:470-482)."""
+    out = ctypes.create_string_buffer(len(os.fsencode(base)) + 64)
+    check(lib().fpc_vtk_filename_with_step(os.fsencode(base), int(step), out, len(out)), "vtk_filename_with_step")
+    return os.fsdecode(out.value)
+
+
+def write_vtk_cellcentered(filename, u, v, w, p, fluid_p, nx, ny, nz, Lx, Ly, Lz):
+    """The reference's cell-centred VTK file from host arrays (byte-identical format)."""
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, w, p)]
+    fl = np.ascontiguousarray(fluid_p, dtype=np.uint8)
+    check(lib().fpc_write_vtk_cellcentered(os.fsencode(filename), *[a.ctypes.data for a in arrs], fl.ctypes.data,
+                                           nx, ny, nz, Lx, Ly, Lz), "write_vtk_cellcentered")
+
+
+def write_fields(path, step, nx, ny, nz, u, v, w, p):
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, w, p)]
+    check(lib().fpc_write_fields(os.fsencode(path), step, nx, ny, nz, *[a.ctypes.data for a in arrs]), "write_fields")
+
+
+def read_fields(path):
+    """A raw field dump (fpc_write_fields): dict with nx, ny, nz, step, u, v, w, p."""
+    with open(path, "rb") as f:
+        head = f.read(40)
+        if head[:8] != b"FPCFLD01":
+            raise FpcError(f"{path}: not a field dump")
+        nx, ny, nz, step = np.frombuffer(head[8:24], dtype="<i4")
+        total = int(np.frombuffer(head[24:32], dtype="<i8")[0])
+        data = np.frombuffer(f.read(8 * total), dtype="<f8")
+    nx, ny, nz = int(nx), int(ny), int(nz)
+    sizes = [(nx + 1) * ny * nz, nx * (ny + 1) * nz, nx * ny * (nz + 1), nx * ny * nz]
+    if sum(sizes) != total or data.size != total:
+        raise FpcError(f"{path}: truncated or inconsistent dump")
+    out = {"nx": nx, "ny": ny, "nz": nz, "step": int(step)}
+    o = 0
+    for name, n in zip("uvwp", sizes):
+        out[name] = data[o:o + n].copy()
+        o += n
+    return out
 
 
 def set_tuning(key, value):

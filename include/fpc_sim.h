@@ -39,11 +39,17 @@ typedef struct {
   unsigned long long seed;
   int mg_precision;          /* 64: FP64 GMRES+PFMG; 32: FP32 PFMG inside FP64 flexible GMRES */
   int vel_print, p_print;
+  /* output and device (common.hpp:60, 82-84): the driver executable (fpc_main) writes
+   * the cell-centred VTK file at steps 1, nsteps and every max(1, vtk_every)-th step
+   * when write_vtk is set; fpc_sim_step itself writes no file. */
+  int write_vtk, vtk_every, device;
+  char vtk_filename[256];
 } fpc_params;
 
 void fpc_default_params(fpc_params *par);
-/* Reference flag names (common.hpp:87-127) plus -init-profile, -noise-amp, -seed,
- * -mg-precision.  Unknown flags are ignored, as in the reference. */
+/* Reference flag names (common.hpp:87-127, including -write-vtk, -vtk-every, -vtk and
+ * -device) plus -init-profile, -noise-amp, -seed, -mg-precision.  Unknown flags are
+ * ignored, as in the reference. */
 int fpc_parse_args(int argc, char **argv, fpc_params *par);
 
 /* ---- one-time setup ---- */
@@ -87,13 +93,33 @@ void *fpc_sim_stream(fpc_sim sim);
  * owned by the simulation; valid until fpc_sim_destroy). */
 int fpc_sim_flow_ops(fpc_sim sim, fpc_flow_ops *ops);
 /* step ramp div_l2 umax vmax wmax its_u its_v its_w its_p relres_u relres_v relres_w
- * relres_p fx fy cd cl err_u err_v err_w err_p; zext_m_c and zext_flagged_c (c = u, v, w, p):
+ * relres_p fx fy cd cl err_u err_v err_w err_p; the setup counts Np Nu Nv Nw Nqu Nqv Nqw
+ * n_pruned; zext_m_c and zext_flagged_c (c = u, v, w, p):
  * wall planes (0 = full planes) and flagged points of the hot copy the operator's solve
  * loops read (planned at its first solve); NaN if unknown. */
 double fpc_sim_scalar(fpc_sim sim, const char *name);
 /* Per-stage wall time (s) accumulated over steps: t_build t_mom t_p t_proj t_diag */
 double fpc_sim_timer(fpc_sim sim, const char *name);
 int fpc_sim_destroy(fpc_sim sim);
+
+/* ---- on-disk formats ----
+ * The reference driver's cell-centred ASCII VTK file (write_vtk_cellcentered,
+ * immersed_operators.hpp:1055-1076), byte for byte: host arrays of the full staggered
+ * fields, fluid mask of the cells. */
+int fpc_write_vtk_cellcentered(const char *filename, const double *u, const double *v, const double *w,
+                               const double *p, const unsigned char *fluid_p, int nx, int ny, int nz, double Lx,
+                               double Ly, double Lz);
+/* base with "_%06d" (step) before its extension (vtk_filename_with_step,
+ * solver_helpers.hpp:470-482); FPC_ERROR_ARG if out (outlen bytes) is too short. */
+int fpc_vtk_filename_with_step(const char *base, int step, char *out, int outlen);
+/* Raw dump of the staggered fields (no reference counterpart): a 40-byte header
+ * ("FPCFLD01", int32 nx ny nz step, int64 number of doubles, 4 zero bytes), then u, v, w
+ * and p as little-endian float64, full staggered layouts, x fastest.  Host arrays. */
+int fpc_write_fields(const char *path, int step, int nx, int ny, int nz, const double *u, const double *v,
+                     const double *w, const double *p);
+/* The same two files from a simulation's current fields. */
+int fpc_sim_write_vtk(fpc_sim sim, const char *filename);
+int fpc_sim_write_fields(fpc_sim sim, const char *path);
 
 /* ---- ensemble of independent channels (no counterpart in the reference, which
  * runs one channel per process) ----

@@ -1879,6 +1879,39 @@ int orc_sim_set(void *s, const char *name, const void *src, long long nbytes) {
   std::memcpy(v.w, src, (size_t)nbytes);
   return 0;
 }
+// Cell-centred ASCII VTK file (immersed_operators.hpp:1055-1076): fprintf of every
+// value with "%.16e", face averages for ux, uy, uz, then pressure and the fluid mask.
+int orc_write_vtk_cellcentered(const char *filename, const double *u, const double *v, const double *w,
+                               const double *p, const unsigned char *fluid_p, int nx, int ny, int nz, double Lx,
+                               double Ly, double Lz) {
+  double dx = Lx / nx, dy = Ly / ny, dz = Lz / nz;
+  std::FILE *fp = std::fopen(filename, "w");
+  if (!fp) return 1;
+  std::fprintf(fp, "# vtk DataFile Version 3.0\n");
+  std::fprintf(fp, "3D immersed cylinder MAC cell-centered\nASCII\nDATASET STRUCTURED_POINTS\n");
+  std::fprintf(fp, "DIMENSIONS %d %d %d\n", nx + 1, ny + 1, nz + 1);
+  std::fprintf(fp, "ORIGIN 0 0 0\nSPACING %.16e %.16e %.16e\n", dx, dy, dz);
+  std::fprintf(fp, "CELL_DATA %d\n", nx * ny * nz);
+  std::fprintf(fp, "SCALARS ux double 1\nLOOKUP_TABLE default\n");
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) std::fprintf(fp, "%.16e\n", 0.5 * (u[ixU(i, j, k, nx, ny)] + u[ixU(i + 1, j, k, nx, ny)]));
+  std::fprintf(fp, "SCALARS uy double 1\nLOOKUP_TABLE default\n");
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) std::fprintf(fp, "%.16e\n", 0.5 * (v[ixV(i, j, k, nx, ny)] + v[ixV(i, j + 1, k, nx, ny)]));
+  std::fprintf(fp, "SCALARS uz double 1\nLOOKUP_TABLE default\n");
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) std::fprintf(fp, "%.16e\n", 0.5 * (w[ixW(i, j, k, nx, ny)] + w[ixW(i, j, k + 1, nx, ny)]));
+  std::fprintf(fp, "SCALARS pressure double 1\nLOOKUP_TABLE default\n");
+  for (size_t c = 0; c < (size_t)nx * ny * nz; ++c) std::fprintf(fp, "%.16e\n", p[c]);
+  std::fprintf(fp, "SCALARS fluid_mask int 1\nLOOKUP_TABLE default\n");
+  for (size_t c = 0; c < (size_t)nx * ny * nz; ++c) std::fprintf(fp, "%d\n", fluid_p[c] ? 1 : 0);
+  std::fclose(fp);
+  return 0;
+}
+
 double orc_sim_scalar(void *s, const char *name) {
   Sim &M = *static_cast<Sim *>(s);
   std::string n(name);

@@ -89,6 +89,11 @@ int orc_sim_set(void *s, const char *name, const void *src, long long nbytes);
  * cd cl fx fy; named ints via the same call cast: its_u its_v its_w its_p step */
 double orc_sim_scalar(void *s, const char *name);
 
+/* The reference's cell-centred VTK writer (immersed_operators.hpp:1055-1076), fprintf as written. */
+int orc_write_vtk_cellcentered(const char *filename, const double *u, const double *v, const double *w,
+                               const double *p, const unsigned char *fluid_p, int nx, int ny, int nz, double Lx,
+                               double Ly, double Lz);
+
 #ifdef __cplusplus
 }
 #endif

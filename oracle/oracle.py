@@ -81,6 +81,7 @@ def lib():
         L.orc_sim_set.argtypes = [vp, cp, vp, ll]
         L.orc_sim_scalar.argtypes = [vp, cp]
         L.orc_sim_scalar.restype = ctypes.c_double
+        L.orc_write_vtk_cellcentered.argtypes = [cp] + [vp] * 5 + [ctypes.c_int] * 3 + [ctypes.c_double] * 3
         _lib = L
     return _lib
 
@@ -258,3 +259,20 @@ class Sim:
 
     def scalars(self):
         return {k: self.scalar(k) for k in self.SCALARS}
+
+
+def write_vtk_cellcentered(filename, u, v, w, p, fluid_p, nx, ny, nz, Lx, Ly, Lz):
+    """The reference's cell-centred VTK writer restated with fprintf (immersed_operators.hpp:1055-1076)."""
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, w, p)]
+    fl = np.ascontiguousarray(fluid_p, dtype=np.uint8)
+    rc = lib().orc_write_vtk_cellcentered(os.fsencode(filename), *[a.ctypes.data for a in arrs], fl.ctypes.data,
+                                          nx, ny, nz, Lx, Ly, Lz)
+    if rc:
+        raise OSError(f"cannot write {filename}")
+
+
+def vtk_filename_with_step(base, step):
+    """solver_helpers.hpp:470-482: "_%06d" inserted before the last extension."""
+    dot = base.rfind(".")
+    stem, ext = (base[:dot], base[dot:]) if dot >= 0 else (base, "")
+    return "%s_%06d%s" % (stem, step, ext)
