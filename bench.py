@@ -20,10 +20,9 @@ class, else null.  copy_gbs = a 1 GiB device-to-device copy timed the same way
 frac_of_copy = achieved / copy_gbs.
 
 cpu_baseline: the oracle (C++ restatement of the same path, OpenMP over the
-OMP_NUM_THREADS host threads) on a 32-deep z-slab of the same channel
-(512x96x32, Lz scaled so the grid spacing is unchanged): twelve steps after one
-untimed step, expressed as time-steps/s of the 512x96x96 grid by the ratio of
-cells.
+OMP_NUM_THREADS host threads) on the same 512x96x96 workload: after the setup
+and the warmup steps, three steps timed, with their iteration counts beside the
+GPU run's on the same steps.
 
 A step is the whole hot path of main.cu:249-376: BC cleanup, convection,
 grad p, momentum rhs, three GMRES+Jacobi solves, scatter, divergence, the
@@ -135,29 +134,36 @@ def copy_gbs(torch, nbytes=1 << 30, reps=10):
     return 2.0 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
 
 
-def cpu_baseline(nz=32, nsteps=12):
+def cpu_baseline(warmup=2, nsteps=3):
     """The oracle (the C++ restatement of the reference path, OpenMP over the host
-    threads OMP_NUM_THREADS allows) on a 512x96xnz slab of the production channel
-    (same spacing): nsteps steps after one untimed step, expressed as time-steps/s
-    of the 512x96x96 grid by the ratio of cells."""
+    threads OMP_NUM_THREADS allows) on the benchmark workload itself: setup,
+    `warmup` untimed steps, then nsteps timed steps (the GPU run's first timed
+    steps, whose iteration counts are reported beside these)."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
     O.set_threads(threads)
-    kw = dict(WORKLOAD, nz=nz, Lz=WORKLOAD["Lz"] * nz / WORKLOAD["nz"])
-    sim = O.Sim(O.default_params(**kw))
-    sim.step(1)
     t0 = time.perf_counter()
-    sim.step(nsteps)
+    sim = O.Sim(O.default_params(**WORKLOAD))
+    t_setup = time.perf_counter() - t0
+    print(f"cpu_baseline: oracle setup {t_setup:.1f} s", file=sys.stderr, flush=True)
+    sim.step(warmup)
+    its = []
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        sim.step(1)
+        sc = sim.scalars()
+        its.append([int(sc["its_" + q]) for q in "uvwp"])
+        print(f"cpu_baseline: timed step {len(its)} its {its[-1]}", file=sys.stderr, flush=True)
     dt = time.perf_counter() - t0
-    full = WORKLOAD["nx"] * WORKLOAD["ny"] * WORKLOAD["nz"]
-    return {"value": nsteps / dt * (nz / WORKLOAD["nz"]), "unit": "time-steps/s", "cores": threads, "kind": "port",
-            "sample": f"oracle (C++ restatement of the reference path, {threads} OpenMP threads), 512x96x{nz} slab "
-                      f"of the same channel (same spacing), steps 2-{nsteps + 1} timed ({dt:.1f} s), scaled by cells "
-                      f"{512 * 96 * nz}/{full}"}
+    return {"value": nsteps / dt, "unit": "time-steps/s", "cores": threads, "kind": "port",
+            "iterations": its, "setup_s": t_setup,
+            "sample": f"oracle (C++ restatement of the reference path, {threads} OpenMP threads) on the same "
+                      f"512x96x96 workload: setup {t_setup:.1f} s, steps 1-{warmup} untimed, steps "
+                      f"{warmup + 1}-{warmup + nsteps} timed ({dt:.1f} s)"}
 
 
-def run_ensemble(F, torch, nthreads, n=64, steps=50):
+def run_ensemble(F, torch, nthreads, n=64, steps=50, modes=("looped", "concurrent", "batched")):
     """Aggregate cell-updates/s of n seeded uneven channels over `steps` steps: the
     instances one after another (the plain path looped), with `nthreads` host
     workers and one HIP stream per instance, and batched (one stream, one launch
@@ -168,6 +174,8 @@ def run_ensemble(F, torch, nthreads, n=64, steps=50):
            "instances": n, "cells_total": cells, "steps": steps, "dtype": "f64",
            "data": "synthetic (seeded geometry, Re 10-100, inlet profile + noise)"}
     for tag, nth in (("looped", 1), ("concurrent", nthreads), ("batched", 0)):
+        if tag not in modes:
+            continue
         ens = F.Ensemble(pars, nthreads=nth)
         ens.step(1)
         torch.cuda.synchronize()
@@ -180,10 +188,11 @@ def run_ensemble(F, torch, nthreads, n=64, steps=50):
             kz, tmin = ens.tiles()
             out[tag]["autotuned_tiles"] = {"kz": kz, "tile_min_points": tmin}
         ens.close()
-    best = max(("concurrent", "batched"), key=lambda k: out[k]["value"])
+    best = max((k for k in ("concurrent", "batched", "looped") if k in out), key=lambda k: out[k]["value"])
     out["value"] = out[best]["value"]
     out["mode"] = best
-    out["speedup_vs_looped"] = out[best]["value"] / out["looped"]["value"]
+    if "looped" in out:
+        out["speedup_vs_looped"] = out[best]["value"] / out["looped"]["value"]
     return out
 
 
@@ -290,6 +299,9 @@ def main():
     ap.add_argument("--ensemble", action="store_true",
                     help="64 uneven channels, 50 steps: cell-updates/s looped vs concurrent (no headline line)")
     ap.add_argument("--ensemble-threads", type=int, default=8)
+    ap.add_argument("--ensemble-steps", type=int, default=50)
+    ap.add_argument("--ensemble-modes", default="looped,concurrent,batched",
+                    help="comma list of looped, concurrent, batched")
     ap.add_argument("--adjoint", action="store_true",
                     help="256x64x64 drag-gradient run + 16 finite-difference directions (no headline line)")
     ap.add_argument("--mixed", action="store_true",
@@ -314,7 +326,8 @@ def main():
     import fpc_amd as F
     F.lib()
     if args.ensemble:
-        print(json.dumps(run_ensemble(F, torch, args.ensemble_threads)))
+        print(json.dumps(run_ensemble(F, torch, args.ensemble_threads, steps=args.ensemble_steps,
+                                      modes=tuple(args.ensemble_modes.split(",")))))
         return
     if args.adjoint:
         print(json.dumps(run_adjoint(F, torch)))
@@ -364,7 +377,8 @@ def main():
         res["roofline"]["copy_gbs"] = cg
         res["roofline"]["frac_of_copy"] = res["roofline"]["achieved"] / cg
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline()
+        res["cpu_baseline"] = cpu_baseline(args.warmup)
+        res["cpu_baseline"]["gpu_iterations"] = [[int(x) for x in r] for r in its64[:3]]
     print(json.dumps(res))
 
 

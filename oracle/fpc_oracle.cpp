@@ -102,16 +102,30 @@ void sp_add(Sp &A, int r, int c, double v) {
   A.rows[(size_t)(r - 1)][c] += v;
 }
 
+// Rows are independent: the parallel loops below give each row the same
+// insertion sequence as a serial run, hence the same container state.
 void sp_compress(Sp &A, double tol = 1.0e-14) {
-  for (auto &r : A.rows)
+  ORC_PAR
+  for (size_t i = 0; i < A.rows.size(); ++i) {
+    auto &r = A.rows[i];
     for (auto it = r.begin(); it != r.end();) {
       if (std::abs(it->second) <= tol) it = r.erase(it);
       else ++it;
     }
+  }
+}
+
+// Frees the row containers (millions of small hash maps) in parallel.
+void sp_release(Sp &A) {
+  ORC_PAR
+  for (size_t i = 0; i < A.rows.size(); ++i) std::unordered_map<int, double>().swap(A.rows[i]);
+  A.rows.clear();
+  A.rows.shrink_to_fit();
 }
 
 Sp sp_mul(const Sp &A, const Sp &B) {  // common.hpp:175-189
   Sp C(A.nrows, B.ncols);
+  ORC_PAR
   for (int i = 0; i < A.nrows; ++i)
     for (const auto &ak : A.rows[(size_t)i]) {
       int k = ak.first;
@@ -125,6 +139,7 @@ Sp sp_mul(const Sp &A, const Sp &B) {  // common.hpp:175-189
 void sp_row_scale(const Sp &A, Sp &As, std::vector<double> &scale) {  // immersed_operators.hpp:737-748
   scale.assign((size_t)A.nrows + 1, 1.0);
   As = Sp(A.nrows, A.ncols);
+  ORC_PAR
   for (int i = 1; i <= A.nrows; ++i) {
     double s = 0.0;
     for (const auto &kv : A.rows[(size_t)(i - 1)]) s += std::abs(kv.second);
@@ -145,13 +160,17 @@ CSR sp_to_csr(const Sp &A) {  // gpu_dense_ops.hpp:32-48 (columns sorted ascendi
   CSR M;
   M.nrows = A.nrows;
   M.rowptr.assign((size_t)A.nrows + 1, 0);
+  for (int i = 0; i < A.nrows; ++i) M.rowptr[(size_t)i + 1] = M.rowptr[(size_t)i] + (int)A.rows[(size_t)i].size();
+  M.colind.assign((size_t)M.rowptr[(size_t)A.nrows], 0);
+  M.vals.assign(M.colind.size(), 0.0);
+  ORC_PAR
   for (int i = 0; i < A.nrows; ++i) {
     std::vector<std::pair<int, double>> row(A.rows[(size_t)i].begin(), A.rows[(size_t)i].end());
     std::sort(row.begin(), row.end(), [](const std::pair<int, double> &a, const std::pair<int, double> &b) {
       return a.first < b.first;
     });
-    for (auto &kv : row) { M.colind.push_back(kv.first); M.vals.push_back(kv.second); }
-    M.rowptr[(size_t)i + 1] = (int)M.colind.size();
+    size_t o = (size_t)M.rowptr[(size_t)i];
+    for (auto &kv : row) { M.colind[o] = kv.first; M.vals[o] = kv.second; ++o; }
   }
   return M;
 }
@@ -483,7 +502,7 @@ void build_w(const Setup &S, Sp &A, std::vector<double> &bc) {
 }
 
 // immersed_operators.hpp:179-287
-void build_proj(const Setup &S, Sp &D, Sp &G, Sp &Ap0, std::vector<double> &div_bc) {
+void build_proj(const Setup &S, Sp &D, Sp &G, std::vector<double> &div_bc) {  // Ap = D G follows in setup_create
   const orc_params &p = S.par;
   int nx = S.nx, ny = S.ny, nz = S.nz;
   double dx = S.dx, dy = S.dy, dz = S.dz;
@@ -559,7 +578,6 @@ void build_proj(const Setup &S, Sp &D, Sp &G, Sp &Ap0, std::vector<double> &div_
       }
   sp_compress(D);
   sp_compress(G);
-  Ap0 = sp_mul(D, G);
 }
 
 // Embedding of an active-row matrix into a 7-point AoS box (solver_helpers.hpp:51-84, 227-329).
@@ -579,6 +597,7 @@ void embed(const Sp &A, const std::vector<int> &id_full, int bnx, int bny, int b
   size_t nrows = (size_t)bnx * bny * bnz;
   vals.assign(7 * nrows, 0.0);
   long long plane = (long long)fsx * fsy;
+  ORC_PAR2
   for (int k = 0; k < bnz; ++k)
     for (int j = 0; j < bny; ++j)
       for (int i = 0; i < bnx; ++i) {
@@ -614,13 +633,20 @@ Setup *setup_create(const orc_params &par) {
   geom_prune(*S);
   extrude_and_ids(*S);
   Sp Au0, Av0, Aw0, Ap0, Dm, Gm, Au, Av, Aw;
-  build_u(*S, Au0, S->rhs_u_bc);
-  build_v(*S, Av0, S->rhs_v_bc);
-  build_w(*S, Aw0, S->rhs_w_bc);
-  build_proj(*S, Dm, Gm, Ap0, S->div_bc);
-  sp_row_scale(Au0, Au, S->scale_u);
-  sp_row_scale(Av0, Av, S->scale_v);
-  sp_row_scale(Aw0, Aw, S->scale_w);
+  // the four assemblies are independent (each one's rows are filled in its own
+  // serial order); the row-parallel helpers inside run serially within a section
+#pragma omp parallel sections num_threads(g_threads > 1 ? 4 : 1)
+  {
+#pragma omp section
+    { build_u(*S, Au0, S->rhs_u_bc); sp_row_scale(Au0, Au, S->scale_u); }
+#pragma omp section
+    { build_v(*S, Av0, S->rhs_v_bc); sp_row_scale(Av0, Av, S->scale_v); }
+#pragma omp section
+    { build_w(*S, Aw0, S->rhs_w_bc); sp_row_scale(Aw0, Aw, S->scale_w); }
+#pragma omp section
+    build_proj(*S, Dm, Gm, S->div_bc);
+  }
+  Ap0 = sp_mul(Dm, Gm);
   for (auto &r : Ap0.rows)
     for (auto &kv : r) kv.second = -kv.second;  // main.cu:101-103
   int nx = S->nx, ny = S->ny, nz = S->nz;
@@ -630,6 +656,7 @@ Setup *setup_create(const orc_params &par) {
   embed(Ap0, S->id_p, nx, ny, nz, 0, 0, 0, nx, ny, S->Ap_vals);
   S->D = sp_to_csr(Dm);
   S->G = sp_to_csr(Gm);
+  for (Sp *A : {&Au0, &Av0, &Aw0, &Ap0, &Dm, &Gm, &Au, &Av, &Aw}) sp_release(*A);
   S->uin.assign((size_t)ny * nz, 0.0);  // immersed_operators.hpp:921-925
   for (int k = 0; k < nz; ++k)
     for (int j = 0; j < ny; ++j)
@@ -1520,6 +1547,8 @@ void force_weights(Sim &M) {
 void phase_diagnostics(Sim &M) {
   const Setup &S = *M.S;
   M.divv.assign((size_t)S.Np + 1, 0.0);
+  // qt exists once a step has packed it; before that the divergence is of zero
+  if (M.qt.size() != (size_t)S.Nqu + S.Nqv + S.Nqw + 1) M.qt.assign((size_t)S.Nqu + S.Nqv + S.Nqw + 1, 0.0);
   csr_spmv(S.D, M.qt.data(), M.divv.data());
   double d2 = 0.0;
   for (int i = 1; i <= S.Np; ++i) {
