@@ -529,6 +529,8 @@ int fpc_ensemble_create(const fpc_params *pars, int n, int nthreads, fpc_ensembl
 
 int fpc_ensemble_step(fpc_ensemble E, int nsteps) {
   FPC_CHECK_ARG(E && nsteps >= 0, "bad arguments");
+  FPC_CHECK_ARG(E->batched || E->workers.size() <= 1 || !fpc::prof_enabled(),
+                "kernel timing (fpc_ProfileEnable) is for one thread: disable it to step a multi-threaded ensemble");
   if (!E->batched) return dispatch(E, 2, nsteps);
   for (int s = 0; s < nsteps; ++s) {
     int rc = batched_step(E);

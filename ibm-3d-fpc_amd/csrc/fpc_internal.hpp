@@ -68,7 +68,9 @@ class StreamScope {
     }                                                                     \
   } while (0)
 
-// Kernel timing with HIP events on the library stream (fpc_ProfileEnable).
+// Kernel timing with HIP events on the library stream (fpc_ProfileEnable); for
+// one calling thread at a time.
+bool prof_enabled();
 bool prof_on(const char *cls);
 void prof_begin(const char *cls);
 void prof_end(const char *cls, double bytes);

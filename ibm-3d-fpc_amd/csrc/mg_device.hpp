@@ -30,19 +30,21 @@ __device__ inline T add_if(bool in, T s, T a, T v) {
 }
 
 // Residual of one weighted-Jacobi sweep from zero: b - A (w D^-1 b) at (i, j, k) = row t.
-template <typename T, class M>
+// b may be stored wider than T (TB): each value is rounded to T when read.
+template <typename T, class M, typename TB = T>
 __device__ inline T down_point(const T *__restrict__ c, long long pitch, M zm, int nx, int ny, int nz,
-                               const T *__restrict__ dinv, const T *__restrict__ b, T w, int i, int j, int k,
+                               const T *__restrict__ dinv, const TB *__restrict__ b, T w, int i, int j, int k,
                                long long t) {
   const long long pl = (long long)nx * ny;
   const bool xm = i > 0, xp = i < nx - 1, ym = j > 0, yp = j < ny - 1, zl = k > 0, zh = k < nz - 1;
   const long long u = zm(t, k);
   const long long t1 = xm ? t - 1 : t, t2 = xp ? t + 1 : t, t3 = ym ? t - nx : t, t4 = yp ? t + nx : t;
   const long long t5 = zl ? t - pl : t, t6 = zh ? t + pl : t;
-  const T x0 = w * b[t] * dinv[u];
-  const T x1 = w * b[t1] * dinv[zm(t1, k)], x2 = w * b[t2] * dinv[zm(t2, k)];
-  const T x3 = w * b[t3] * dinv[zm(t3, k)], x4 = w * b[t4] * dinv[zm(t4, k)];
-  const T x5 = w * b[t5] * dinv[zm(t5, zl ? k - 1 : k)], x6 = w * b[t6] * dinv[zm(t6, zh ? k + 1 : k)];
+  const T b0 = (T)b[t];
+  const T x0 = w * b0 * dinv[u];
+  const T x1 = w * (T)b[t1] * dinv[zm(t1, k)], x2 = w * (T)b[t2] * dinv[zm(t2, k)];
+  const T x3 = w * (T)b[t3] * dinv[zm(t3, k)], x4 = w * (T)b[t4] * dinv[zm(t4, k)];
+  const T x5 = w * (T)b[t5] * dinv[zm(t5, zl ? k - 1 : k)], x6 = w * (T)b[t6] * dinv[zm(t6, zh ? k + 1 : k)];
   T s = c[u] * x0;
   s = add_if(xm, s, c[pitch + u], x1);
   s = add_if(xp, s, c[2 * pitch + u], x2);
@@ -50,7 +52,7 @@ __device__ inline T down_point(const T *__restrict__ c, long long pitch, M zm, i
   s = add_if(yp, s, c[4 * pitch + u], x4);
   s = add_if(zl, s, c[5 * pitch + u], x5);
   s = add_if(zh, s, c[6 * pitch + u], x6);
-  return b[t] - s;
+  return b0 - s;
 }
 
 // Restriction to coarse point (i, j, k) of a level coarsened in direction d:
@@ -73,9 +75,9 @@ __device__ inline T restrict_point(int fnx, int fny, int fnz, int d, const T *__
 // The same restriction with the fine residuals (down_point) evaluated on the fly
 // instead of read back: one launch less per level for the latency-bound small
 // levels.  Fine box nx x ny x nz; coarse point (i, j, k).  Sums as restrict_point.
-template <typename T, class M>
+template <typename T, class M, typename TB = T>
 __device__ inline T down_restrict_point(const T *__restrict__ c, long long pitch, M zm, int nx, int ny, int nz,
-                                        const T *__restrict__ dinv, const T *__restrict__ b, T w,
+                                        const T *__restrict__ dinv, const TB *__restrict__ b, T w,
                                         const T *__restrict__ wm, const T *__restrict__ wp, int d, int i, int j,
                                         int k) {
   const int di = (d == 0), dj = (d == 1), dk = (d == 2);
@@ -87,9 +89,9 @@ __device__ inline T down_restrict_point(const T *__restrict__ c, long long pitch
   const bool hp = pd + 1 < fnd;
   const int pi = hp ? fi + di : fi, pj = hp ? fj + dj : fj, pk = hp ? fk + dk : fk;
   const long long fp = hp ? f0 + stride : f0;
-  const T r0 = down_point<T>(c, pitch, zm, nx, ny, nz, dinv, b, w, fi, fj, fk, f0);
-  const T rm = down_point<T>(c, pitch, zm, nx, ny, nz, dinv, b, w, fi - di, fj - dj, fk - dk, f0 - stride);
-  const T rp = down_point<T>(c, pitch, zm, nx, ny, nz, dinv, b, w, pi, pj, pk, fp);
+  const T r0 = down_point<T, M, TB>(c, pitch, zm, nx, ny, nz, dinv, b, w, fi, fj, fk, f0);
+  const T rm = down_point<T, M, TB>(c, pitch, zm, nx, ny, nz, dinv, b, w, fi - di, fj - dj, fk - dk, f0 - stride);
+  const T rp = down_point<T, M, TB>(c, pitch, zm, nx, ny, nz, dinv, b, w, pi, pj, pk, fp);
   const T v = r0 + wp[zm(f0 - stride, fk - dk)] * rm;
   return add_if(hp, v, wm[zm(fp, pk)], rp);
 }
@@ -106,8 +108,8 @@ struct UpLoads {
 // kSpec: load every operand unconditionally at valid addresses and select (one
 // load round trip, for the latency-bound one-point kernels), or load only what
 // the point uses (the tile forms, which prefetch ahead); the values are the same.
-template <typename T, int D, bool kSpec, class M>
-__device__ inline UpLoads<T> up_loads(const T *b, const T *dinv, const T *wm, const T *wp, const T *yc, M zm, int i,
+template <typename T, int D, bool kSpec, class M, typename TB = T>
+__device__ inline UpLoads<T> up_loads(const TB *b, const T *dinv, const T *wm, const T *wp, const T *yc, M zm, int i,
                                       int j, int k, int fnx, int fny, int cnx, int cny, int ncd) {
   const long long t = ((long long)k * fny + j) * fnx + i, u = zm(t, k);
   const int pd = (D == 0) ? i : (D == 1 ? j : k);
@@ -118,7 +120,7 @@ __device__ inline UpLoads<T> up_loads(const T *b, const T *dinv, const T *wm, co
   const bool odd = pd & 1, lo = !odd && q >= 1, hi = !odd && q < ncd;
   UpLoads<T> r;
   r.u = u;
-  r.b = b[t];
+  r.b = (T)b[t];
   r.dinv = dinv[u];
   r.f = !odd;
   if (kSpec) {
@@ -158,13 +160,13 @@ __device__ inline T up_value(const UpLoads<T> &r, T w) {
 
 // Prolongation + one weighted-Jacobi sweep at (i, j, k) = row t:
 // y = x + w D^-1 (b - A x) with x = up_value() at the point and its neighbours.
-template <typename T, int D, class M>
+template <typename T, int D, class M, typename TB = T>
 __device__ inline T up_point(const T *__restrict__ c, long long pitch, M zm, int nx, int ny, int nz,
-                             const T *__restrict__ dinv, const T *__restrict__ b, const T *__restrict__ wm,
+                             const T *__restrict__ dinv, const TB *__restrict__ b, const T *__restrict__ wm,
                              const T *__restrict__ wp, const T *__restrict__ yc, int cnx, int cny, int ncd, T w, int i,
                              int j, int k, long long t) {
 #define FPC_X(ii, jj, kk) \
-  up_value<T>(up_loads<T, D, true>(b, dinv, wm, wp, yc, zm, ii, jj, kk, nx, ny, cnx, cny, ncd), w)
+  up_value<T>(up_loads<T, D, true, M, TB>(b, dinv, wm, wp, yc, zm, ii, jj, kk, nx, ny, cnx, cny, ncd), w)
   const bool xm = i > 0, xp = i < nx - 1, ym = j > 0, yp = j < ny - 1, zl = k > 0, zh = k < nz - 1;
   const long long u = zm(t, k);
   const T xc = FPC_X(i, j, k);
@@ -179,7 +181,7 @@ __device__ inline T up_point(const T *__restrict__ c, long long pitch, M zm, int
   s = add_if(yp, s, c[4 * pitch + u], x4);
   s = add_if(zl, s, c[5 * pitch + u], x5);
   s = add_if(zh, s, c[6 * pitch + u], x6);
-  return xc + w * (b[t] - s) * dinv[u];
+  return xc + w * ((T)b[t] - s) * dinv[u];
 }
 
 // Row r of the coarsest solve x = inv b (FP64 sums), one wavefront; valid in lane 0.
@@ -211,14 +213,16 @@ struct RowOf {
 // The same arithmetic on the z-marching tiles (stencil_tile.hpp): every row read
 // once, z-neighbours carried in registers; bit for bit the one-point results.
 
-// Jacobi sweep from zero and residual: r = b - A (w D^-1 b).
-template <typename T>
+// Jacobi sweep from zero and residual: r = b - A (w D^-1 b).  b is stored as TB
+// (wider than T on the finest level of a reduced-precision cycle).
+template <typename T, typename TB = T>
 struct DownOp {
   static constexpr int kAhead = 2;  // prefetch distance of the march (stencil_tile.hpp)
   const T *c;
   long long pitch;
   int nx, ny;
-  const T *dinv, *b;
+  const T *dinv;
+  const TB *b;
   T *r;
   T w;
   ZMap zm;  // rows of c and dinv
@@ -229,7 +233,7 @@ struct DownOp {
   __device__ long long at(int i, int j, int k) const { return ((long long)k * ny + j) * nx + i; }
   __device__ Raw fetch(int i, int j, int k, int f) const {
     const long long t = at(i, j, k), u = zm.row(t, k, f);
-    return Raw{b[t], dinv[u], u};
+    return Raw{(T)b[t], dinv[u], u};
   }
   __device__ T finish(const Raw &q) const { return w * q.b * q.dinv; }
   __device__ void out(const Coef7<T> &a, const Raw &own, int i, int j, int k, const Nbr<T> &nb) {
@@ -242,13 +246,14 @@ struct DownOp {
 // fine residual is never stored.  Chunks start on even planes and run one plane
 // past their end so the F+ plane of their last coarse plane is seen.  The weights
 // of a point come with its fetch, ahead of use.
-template <typename T>
+template <typename T, typename TB = T>
 struct DownRestrictZOp {
   static constexpr int kAhead = 2;  // prefetch distance of the march (stencil_tile.hpp)
   const T *c;
   long long pitch;
   int nx, ny;
-  const T *dinv, *b;
+  const T *dinv;
+  const TB *b;
   T w;
   ZMap zm;  // rows of c, dinv, wm and wp
   const T *wm, *wp;
@@ -262,7 +267,7 @@ struct DownRestrictZOp {
   __device__ long long at(int i, int j, int k) const { return ((long long)k * ny + j) * nx + i; }
   __device__ Raw fetch(int i, int j, int k, int f) const {
     const long long t = at(i, j, k), u = zm.row(t, k, f);
-    return Raw{b[t], dinv[u], wm[u], wp[u], u};
+    return Raw{(T)b[t], dinv[u], wm[u], wp[u], u};
   }
   __device__ T finish(const Raw &q) const { return w * q.b * q.dinv; }
   __device__ void out(const Coef7<T> &a, const Raw &own, int i, int j, int k, const Nbr<T> &nb) {
@@ -282,27 +287,31 @@ struct DownRestrictZOp {
 };
 
 // Prolongation of the coarse correction plus one Jacobi sweep:
-// y = x + P e, then y + w D^-1 (b - A y), with x = w D^-1 b recomputed.
-template <typename T, int D>
+// y = x + P e, then y + w D^-1 (b - A y), with x = w D^-1 b recomputed.  b is
+// stored as TB and y written as TO (both wider than T on the finest level of a
+// reduced-precision cycle: the rounding of b to T and of y from T happen here).
+template <typename T, int D, typename TB = T, typename TO = T>
 struct UpOp {
   static constexpr int kAhead = 1;  // prefetch distance of the march (stencil_tile.hpp)
   const T *c;
   long long pitch;
   int nx, ny;
-  const T *dinv, *b, *wm, *wp, *yc;
+  const T *dinv;
+  const TB *b;
+  const T *wm, *wp, *yc;
   int cnx, cny, ncd;
-  T *y;
+  TO *y;
   T w;
   ZMap zm;  // rows of c, dinv, wm and wp
   using Raw = UpLoads<T>;
   __device__ long long at(int i, int j, int k) const { return ((long long)k * ny + j) * nx + i; }
   __device__ Raw fetch(int i, int j, int k, int f) const {
     const RowOf fixed{zm.row(at(i, j, k), k, f)};
-    return up_loads<T, D, false>(b, dinv, wm, wp, yc, fixed, i, j, k, nx, ny, cnx, cny, ncd);
+    return up_loads<T, D, false, RowOf, TB>(b, dinv, wm, wp, yc, fixed, i, j, k, nx, ny, cnx, cny, ncd);
   }
   __device__ T finish(const Raw &r) const { return up_value<T>(r, w); }
   __device__ void out(const Coef7<T> &a, const Raw &own, int i, int j, int k, const Nbr<T> &nb) {
-    y[at(i, j, k)] = nb.c + w * (own.b - a.apply(nb)) * own.dinv;
+    y[at(i, j, k)] = (TO)(nb.c + w * (own.b - a.apply(nb)) * own.dinv);
   }
 };
 

@@ -394,9 +394,10 @@ __global__ void k_prolong_add(int fnx, int fny, int fnz, int d, const T *__restr
 
 // Down-sweep and restriction of a small level in one launch, one thread per
 // coarse point (fine box nx x ny x nz, coarse cnx x cny x cnz).
-template <typename T>
+// TB: storage type of b (wider than T on the finest level of a reduced-precision cycle).
+template <typename T, typename TB = T>
 __global__ void k_down_restrict(const T *__restrict__ c, long long pitch, int nx, int ny, int nz,
-                                const T *__restrict__ dinv, const T *__restrict__ b, const T *__restrict__ wm,
+                                const T *__restrict__ dinv, const TB *__restrict__ b, const T *__restrict__ wm,
                                 const T *__restrict__ wp, int d, T w, int cnx, int cny, T *__restrict__ bc,
                                 const int *skip) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -404,27 +405,30 @@ __global__ void k_down_restrict(const T *__restrict__ c, long long pitch, int nx
   int k = blockIdx.z;
   if (i >= cnx || j >= cny) return;
   const int sk = skip_flag(skip);
-  const T v = mgd::down_restrict_point<T>(c, pitch, FullMap(), nx, ny, nz, dinv, b, w, wm, wp, d, i, j, k);
+  const T v = mgd::down_restrict_point<T, FullMap, TB>(c, pitch, FullMap(), nx, ny, nz, dinv, b, w, wm, wp, d, i, j, k);
   pin(v);
   if (sk) return;
   bc[((long long)k * cny + j) * cnx + i] = v;
 }
 
-template <typename T, int D>
+// TB, TO: storage types of b and y (wider than T on the finest level of a
+// reduced-precision cycle, which then reads and writes the FP64 vectors directly).
+template <typename T, int D, typename TB = T, typename TO = T>
 __global__ void k_up_jacobi(const T *__restrict__ c, long long pitch, int nx, int ny, int nz,
-                            const T *__restrict__ dinv, const T *__restrict__ b, const T *__restrict__ wm,
+                            const T *__restrict__ dinv, const TB *__restrict__ b, const T *__restrict__ wm,
                             const T *__restrict__ wp, const T *__restrict__ yc, int cnx, int cny, int ncd,
-                            T *__restrict__ y, T w, const int *skip) {
+                            TO *__restrict__ y, T w, const int *skip) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y;
   int k = blockIdx.z;
   if (i >= nx || j >= ny) return;
   long long t = ((long long)k * ny + j) * nx + i;
   const int sk = skip_flag(skip);
-  const T v = mgd::up_point<T, D>(c, pitch, FullMap(), nx, ny, nz, dinv, b, wm, wp, yc, cnx, cny, ncd, w, i, j, k, t);
+  const T v =
+      mgd::up_point<T, D, FullMap, TB>(c, pitch, FullMap(), nx, ny, nz, dinv, b, wm, wp, yc, cnx, cny, ncd, w, i, j, k, t);
   pin(v);
   if (sk) return;
-  y[t] = v;
+  y[t] = (TO)v;
 }
 
 // x = Ainv b on the coarsest level (FP64 inverse), one block.
@@ -432,23 +436,23 @@ __global__ void k_up_jacobi(const T *__restrict__ c, long long pitch, int nx, in
 // The same arithmetic as jacobi_resid / k_restrict / k_up_jacobi, on the
 // z-marching tiles: every row read once, z-neighbours carried in registers.
 
-template <typename T, int TY>
+template <typename T, int TY, typename TB = T>
 __global__ void __launch_bounds__(kTileX *TY)
     k_down_tiled(const T *__restrict__ c, long long pitch, ZMap zm, int nx, int ny, int nz, const T *__restrict__ dinv,
-                 const T *__restrict__ b, T *__restrict__ r, T w, int kz, const int *skip) {
+                 const TB *__restrict__ b, T *__restrict__ r, T w, int kz, const int *skip) {
   if (skip && *skip) return;
-  mgd::DownOp<T> op{c, pitch, nx, ny, dinv, b, r, w, zm};
+  mgd::DownOp<T, TB> op{c, pitch, nx, ny, dinv, b, r, w, zm};
   const int k0 = blockIdx.z * kz;
   march_tile<T, kTileX, TY>(op, nx, ny, nz, k0, min(k0 + kz, nz), blockIdx.x, blockIdx.y);
 }
 
-template <typename T, int TY>
+template <typename T, int TY, typename TB = T>
 __global__ void __launch_bounds__(kTileX *TY)
     k_down_restrict_z(const T *__restrict__ c, long long pitch, ZMap zm, int nx, int ny, int nz,
-                      const T *__restrict__ dinv, const T *__restrict__ b, const T *__restrict__ wm,
+                      const T *__restrict__ dinv, const TB *__restrict__ b, const T *__restrict__ wm,
                       const T *__restrict__ wp, T *__restrict__ bc, int ncz, T w, int kz, const int *skip) {
   if (skip && *skip) return;
-  mgd::DownRestrictZOp<T> op;
+  mgd::DownRestrictZOp<T, TB> op;
   op.c = c; op.pitch = pitch; op.nx = nx; op.ny = ny; op.dinv = dinv; op.b = b; op.w = w;
   op.zm = zm;
   op.wm = wm; op.wp = wp; op.bc = bc; op.nz = nz; op.ncz = ncz;
@@ -457,14 +461,14 @@ __global__ void __launch_bounds__(kTileX *TY)
   march_tile<T, kTileX, TY>(op, nx, ny, nz, k0, k1 < nz ? k1 + 1 : k1, blockIdx.x, blockIdx.y);
 }
 
-template <typename T, int D, int TY>
+template <typename T, int D, int TY, typename TB = T, typename TO = T>
 __global__ void __launch_bounds__(kTileX *TY)
     k_up_tiled(const T *__restrict__ c, long long pitch, ZMap zm, int nx, int ny, int nz, const T *__restrict__ dinv,
-               const T *__restrict__ b, const T *__restrict__ wm, const T *__restrict__ wp,
-               const T *__restrict__ yc, int cnx, int cny, int ncd, T *__restrict__ y, T w, int kz,
+               const TB *__restrict__ b, const T *__restrict__ wm, const T *__restrict__ wp,
+               const T *__restrict__ yc, int cnx, int cny, int ncd, TO *__restrict__ y, T w, int kz,
                const int *skip) {
   if (skip && *skip) return;
-  mgd::UpOp<T, D> op{c, pitch, nx, ny, dinv, b, wm, wp, yc, cnx, cny, ncd, y, w, zm};
+  mgd::UpOp<T, D, TB, TO> op{c, pitch, nx, ny, dinv, b, wm, wp, yc, cnx, cny, ncd, y, w, zm};
   const int k0 = blockIdx.z * kz;
   march_tile<T, kTileX, TY>(op, nx, ny, nz, k0, min(k0 + kz, nz), blockIdx.x, blockIdx.y);
 }
@@ -772,6 +776,106 @@ HotView<T> hot_view(const DLevel &F) {
                     (double)F.d.N};
 }
 
+// The fused down-sweep of level l: one weighted-Jacobi sweep from zero, the
+// residual and its restriction into the coarse b.  b is read as TB (FP64 on the
+// finest level of an FP32 cycle, rounded to T as it is read).
+template <typename T, typename TB>
+int fused_down(PfmgImpl *P, int l, const TB *b, T w, const int *skip, hipStream_t st) {
+  DLevel &F = P->L[(size_t)l];
+  DLevel &C = P->L[(size_t)l + 1];
+  const double sz = (double)sizeof(T), szb = (double)sizeof(TB);
+  // The z-marching kernels read the level's hot arrays (when it is extruded); the
+  // one-point kernels of the latency-bound small levels the full ones.
+  const HotView<T> H = hot_view<T>(F);
+  const T *hc = H.c, *hd = H.d, *hwm = H.wm, *hwp = H.wp;
+  const double rows = H.rows;  // rows of the coefficient, 1/diag and weight planes read
+  // bytes: 7 coefficients and 1/diag as stored, b at every point, and r written -- or,
+  // when z is restricted on the fly, one weight plane and the coarse b instead of r
+  const char *cls = (l == 0) ? "mg_down0" : "mg_down";
+  if (tile_worthwhile(F.d) && F.cdir == 2) {
+    TileLaunch tl = tile_launch<8>(F.d);
+    FPC_PROF_BEGIN(cls);
+    k_down_restrict_z<T, 8, TB><<<tl.grid, tl.block, 0, st>>>(hc, H.pitch, H.zm, F.d.nx, F.d.ny, F.d.nz, hd, b, hwm,
+                                                               hwp, (T *)C.b, C.d.nz, w, tl.kz, skip);
+    FPC_PROF_END(cls, rows * 9.0 * sz + (double)F.d.N * szb + (double)C.d.N * sz);
+    FPC_LAUNCH_CHECK();
+    return 0;
+  }
+  dim3 gc, bc;
+  launch_shape(C.d, gc, bc);
+  if (!tile_worthwhile(F.d)) {
+    // small level: residual and restriction in one launch (the fine residual is not stored)
+    FPC_PROF_BEGIN(cls);
+    k_down_restrict<T, TB><<<gc, bc, 0, st>>>((const T *)F.coef, F.pitch, F.d.nx, F.d.ny, F.d.nz,
+                                              (const T *)F.dinv, b, (const T *)F.wm, (const T *)F.wp, F.cdir, w,
+                                              C.d.nx, C.d.ny, (T *)C.b, skip);
+    FPC_PROF_END(cls, (double)F.d.N * (9.0 * sz + szb) + (double)C.d.N * sz);
+    FPC_LAUNCH_CHECK();
+    return 0;
+  }
+  FPC_PROF_BEGIN(cls);
+  {
+    TileLaunch tl = tile_launch<8>(F.d);
+    k_down_tiled<T, 8, TB><<<tl.grid, tl.block, 0, st>>>(hc, H.pitch, H.zm, F.d.nx, F.d.ny, F.d.nz, hd, b, (T *)F.r,
+                                                         w, tl.kz, skip);
+  }
+  FPC_PROF_END(cls, rows * 8.0 * sz + (double)F.d.N * (szb + sz));
+  FPC_LAUNCH_CHECK();
+  {
+    // bytes: r at every fine point, the weights at the F points, b of each coarse point
+    FPC_PROF_BEGIN("mg_restrict");
+    k_restrict<T><<<gc, bc, 0, st>>>(F.d.nx, F.d.ny, F.d.nz, F.cdir, (const T *)F.r, (const T *)F.wm,
+                                    (const T *)F.wp, C.d.nx, C.d.ny, C.d.nz, (T *)C.b, skip);
+    FPC_PROF_END("mg_restrict", ((double)F.d.N * 2.0 + (double)C.d.N) * sz);
+  }
+  FPC_LAUNCH_CHECK();
+  return 0;
+}
+
+// The fused up-sweep of level l: prolongation of the coarse correction yc and one
+// weighted-Jacobi sweep into y.  b is read as TB and y written as TO (FP64 on the
+// finest level of an FP32 cycle).
+template <typename T, typename TB, typename TO>
+int fused_up(PfmgImpl *P, int l, const TB *b, const T *yc, TO *y, T w, const int *skip, hipStream_t st) {
+  DLevel &F = P->L[(size_t)l];
+  DLevel &C = P->L[(size_t)l + 1];
+  // bytes: 7 coefficients, 1/diag and a weight pair as stored (the pair is read at
+  // the F points, half of them), b at every point, y written; the coarse correction once
+  const char *cls = (l == 0) ? "mg_up0" : "mg_up";
+  const HotView<T> H = hot_view<T>(F);
+  const double sz = (double)sizeof(T);
+  const double bytes =
+      H.rows * 9.0 * sz + (double)F.d.N * ((double)sizeof(TB) + (double)sizeof(TO)) + (double)C.d.N * sz;
+  FPC_PROF_BEGIN(cls);
+  const int ncd = (F.cdir == 0) ? C.d.nx : (F.cdir == 1 ? C.d.ny : C.d.nz);
+  const T *hc = H.c, *hd = H.d;
+  const T *wm = H.wm, *wp = H.wp;
+  if (tile_worthwhile(F.d)) {
+    TileLaunch tl = tile_launch<8>(F.d);
+#define FPC_UP_TILED(D)                                                                                               \
+  k_up_tiled<T, D, 8, TB, TO><<<tl.grid, tl.block, 0, st>>>(hc, H.pitch, H.zm, F.d.nx, F.d.ny, F.d.nz, hd, b, wm, wp, \
+                                                            yc, C.d.nx, C.d.ny, ncd, y, w, tl.kz, skip)
+    if (F.cdir == 0) FPC_UP_TILED(0);
+    else if (F.cdir == 1) FPC_UP_TILED(1);
+    else FPC_UP_TILED(2);
+#undef FPC_UP_TILED
+  } else {
+    dim3 g, bl;
+    launch_shape(F.d, g, bl);
+#define FPC_UP_SIMPLE(D)                                                                                         \
+  k_up_jacobi<T, D, TB, TO><<<g, bl, 0, st>>>((const T *)F.coef, F.pitch, F.d.nx, F.d.ny, F.d.nz,                  \
+                                              (const T *)F.dinv, b, (const T *)F.wm, (const T *)F.wp, yc, C.d.nx, \
+                                              C.d.ny, ncd, y, w, skip)
+    if (F.cdir == 0) FPC_UP_SIMPLE(0);
+    else if (F.cdir == 1) FPC_UP_SIMPLE(1);
+    else FPC_UP_SIMPLE(2);
+#undef FPC_UP_SIMPLE
+  }
+  FPC_PROF_END(cls, bytes);
+  FPC_LAUNCH_CHECK();
+  return 0;
+}
+
 // One V-cycle with zero initial guess: out = M^-1 in.
 template <typename T>
 int vcycle(PfmgImpl *P, const double *in, double *out, const int *skip) {
@@ -779,17 +883,21 @@ int vcycle(PfmgImpl *P, const double *in, double *out, const int *skip) {
   int nl = (int)P->L.size();
   bool fp64 = sizeof(T) == 8;
   DLevel &L0 = P->L[0];
+  bool rb = P->relax >= 2;
+  // one Jacobi sweep before and after: the fused down/up kernels
+  bool fused = !rb && P->npre == 1 && P->npost == 1;
+  // An FP32 cycle whose finest level runs the fused kernels reads `in` and writes
+  // `out` (FP64) there directly, rounding as it reads and writes; otherwise the
+  // FP64 vectors are converted to and from level-0 FP32 copies.
+  const bool wide0 = !fp64 && fused && nl > 1 && tile_tuning().mg_wide_finest;
   const T *b0;
-  if (fp64) {
-    b0 = (const T *)in;
+  if (fp64 || wide0) {
+    b0 = (const T *)in;  // FP64 cycle: in itself; wide0: not read through this pointer
   } else {
     k_convert<double, T><<<nb(L0.d.N), 256, 0, st>>>(in, (T *)L0.b, L0.d.N, skip);
     FPC_LAUNCH_CHECK();
     b0 = (const T *)L0.b;
   }
-  bool rb = P->relax >= 2;
-  // one Jacobi sweep before and after: the fused down/up kernels
-  bool fused = !rb && P->npre == 1 && P->npost == 1;
   // level-0 solution buffer: the output itself in FP64
   T *y0 = fp64 ? (T *)out : (T *)L0.y;
   auto bof = [&](int l) -> const T * { return l == 0 ? b0 : (const T *)P->L[(size_t)l].b; };
@@ -805,52 +913,10 @@ int vcycle(PfmgImpl *P, const double *in, double *out, const int *skip) {
     const T *c = (const T *)F.coef;
     const T *dv = (const T *)F.dinv;
     T *x = xof(l);
-    const double sz = (double)sizeof(T);
     if (fused) {
-      // The z-marching kernels read the level's hot arrays (when it is extruded); the
-      // one-point kernels of the latency-bound small levels the full ones.
-      const HotView<T> H = hot_view<T>(F);
-      const T *hc = H.c, *hd = H.d, *hwm = H.wm, *hwp = H.wp;
-      const double rows = H.rows;  // rows of the coefficient, 1/diag and weight planes read
-      // bytes: 7 coefficients and 1/diag as stored, b at every point, and r written -- or,
-      // when z is restricted on the fly, one weight plane and the coarse b instead of r
-      const char *cls = (l == 0) ? "mg_down0" : "mg_down";
-      if (tile_worthwhile(F.d) && F.cdir == 2) {
-        TileLaunch tl = tile_launch<8>(F.d);
-        FPC_PROF_BEGIN(cls);
-        k_down_restrict_z<T, 8><<<tl.grid, tl.block, 0, st>>>(hc, H.pitch, H.zm, F.d.nx, F.d.ny, F.d.nz, hd, bof(l),
-                                                               hwm, hwp, (T *)C.b, C.d.nz, w_of(l), tl.kz, skip);
-        FPC_PROF_END(cls, (rows * 9.0 + (double)F.d.N + (double)C.d.N) * sz);
-        FPC_LAUNCH_CHECK();
-        continue;
-      }
-      dim3 gc, bc;
-      launch_shape(C.d, gc, bc);
-      if (!tile_worthwhile(F.d)) {
-        // small level: residual and restriction in one launch (the fine residual is not stored)
-        FPC_PROF_BEGIN(cls);
-        k_down_restrict<T><<<gc, bc, 0, st>>>(c, F.pitch, F.d.nx, F.d.ny, F.d.nz, dv, bof(l), (const T *)F.wm,
-                                              (const T *)F.wp, F.cdir, w_of(l), C.d.nx, C.d.ny, (T *)C.b, skip);
-        FPC_PROF_END(cls, ((double)F.d.N * 10.0 + (double)C.d.N) * sz);
-        FPC_LAUNCH_CHECK();
-        continue;
-      }
-      FPC_PROF_BEGIN(cls);
-      {
-        TileLaunch tl = tile_launch<8>(F.d);
-        k_down_tiled<T, 8><<<tl.grid, tl.block, 0, st>>>(hc, H.pitch, H.zm, F.d.nx, F.d.ny, F.d.nz, hd, bof(l),
-                                                          (T *)F.r, w_of(l), tl.kz, skip);
-      }
-      FPC_PROF_END(cls, (rows * 8.0 + (double)F.d.N * 2.0) * sz);
-      FPC_LAUNCH_CHECK();
-      {
-        // bytes: r at every fine point, the weights at the F points, b of each coarse point
-        FPC_PROF_BEGIN("mg_restrict");
-        k_restrict<T><<<gc, bc, 0, st>>>(F.d.nx, F.d.ny, F.d.nz, F.cdir, (const T *)F.r, (const T *)F.wm,
-                                        (const T *)F.wp, C.d.nx, C.d.ny, C.d.nz, (T *)C.b, skip);
-        FPC_PROF_END("mg_restrict", ((double)F.d.N * 2.0 + (double)C.d.N) * sz);
-      }
-      FPC_LAUNCH_CHECK();
+      const int e = (l == 0 && wide0) ? fused_down<T, double>(P, 0, in, w_of(0), skip, st)
+                                       : fused_down<T, T>(P, l, bof(l), w_of(l), skip, st);
+      if (e) return e;
       continue;
     }
     if (rb) {
@@ -881,44 +947,19 @@ int vcycle(PfmgImpl *P, const double *in, double *out, const int *skip) {
   }
   for (int l = nl - 2; l >= 0; --l) {
     DLevel &F = P->L[(size_t)l];
-    DLevel &C = P->L[(size_t)l + 1];
     dim3 g, bl;
     launch_shape(F.d, g, bl);
     const T *c = (const T *)F.coef;
     const T *dv = (const T *)F.dinv;
     T *x = xof(l);
     if (fused) {
-      // bytes: 7 coefficients, 1/diag and a weight pair as stored (the pair is read at
-      // the F points, half of them), b at every point, y written; the coarse correction once
-      const char *cls = (l == 0) ? "mg_up0" : "mg_up";
-      const HotView<T> H = hot_view<T>(F);
-      const double bytes = (H.rows * 9.0 + (double)F.d.N * 2.0 + (double)C.d.N) * (double)sizeof(T);
-      FPC_PROF_BEGIN(cls);
-      const int ncd = (F.cdir == 0) ? C.d.nx : (F.cdir == 1 ? C.d.ny : C.d.nz);
-      const T *hc = H.c, *hd = H.d;
-      const T *wm = H.wm, *wp = H.wp, *yc = (const T *)yof(l + 1);
-      if (tile_worthwhile(F.d)) {
-        TileLaunch tl = tile_launch<8>(F.d);
-#define FPC_UP_TILED(D)                                                                                          \
-  k_up_tiled<T, D, 8><<<tl.grid, tl.block, 0, st>>>(hc, H.pitch, H.zm, F.d.nx, F.d.ny, F.d.nz, hd, bof(l), wm, wp, \
-                                                    yc, C.d.nx, C.d.ny, ncd, yof(l), w_of(l), tl.kz, skip)
-        if (F.cdir == 0) FPC_UP_TILED(0);
-        else if (F.cdir == 1) FPC_UP_TILED(1);
-        else FPC_UP_TILED(2);
-#undef FPC_UP_TILED
-      } else {
-#define FPC_UP_SIMPLE(D)                                                                                        \
-  k_up_jacobi<T, D><<<g, bl, 0, st>>>(c, F.pitch, F.d.nx, F.d.ny, F.d.nz, dv, bof(l), (const T *)F.wm,            \
-                                      (const T *)F.wp, yc, C.d.nx, C.d.ny, ncd, yof(l), w_of(l), skip)
-        if (F.cdir == 0) FPC_UP_SIMPLE(0);
-        else if (F.cdir == 1) FPC_UP_SIMPLE(1);
-        else FPC_UP_SIMPLE(2);
-#undef FPC_UP_SIMPLE
-      }
-      FPC_PROF_END(cls, bytes);
-      FPC_LAUNCH_CHECK();
+      const int e = (l == 0 && wide0)
+                        ? fused_up<T, double, double>(P, 0, in, (const T *)yof(1), out, w_of(0), skip, st)
+                        : fused_up<T, T, T>(P, l, bof(l), (const T *)yof(l + 1), yof(l), w_of(l), skip, st);
+      if (e) return e;
       continue;
     }
+    DLevel &C = P->L[(size_t)l + 1];
     k_prolong_add<T><<<g, bl, 0, st>>>(F.d.nx, F.d.ny, F.d.nz, F.cdir, (const T *)F.wm, (const T *)F.wp,
                                        (const T *)yof(l + 1), C.d.nx, C.d.ny, C.d.nz, x, skip);
     if (rb) {
@@ -942,7 +983,7 @@ int vcycle(PfmgImpl *P, const double *in, double *out, const int *skip) {
     }
     FPC_LAUNCH_CHECK();
   }
-  if (!fp64) {
+  if (!fp64 && !wide0) {
     k_convert<T, double><<<nb(L0.d.N), 256, 0, st>>>((const T *)L0.y, out, L0.d.N, skip);
     FPC_LAUNCH_CHECK();
   }

@@ -175,6 +175,7 @@ struct TileTuning {
   long long min_points = -1;
   int kz = -1;
   int z_extrude = 1;  // three-plane copies of z-extruded operators
+  int mg_wide_finest = 1;  // FP32 V-cycles read and write the FP64 vectors on the finest level
 };
 inline TileTuning &tile_tuning() {
   static TileTuning t = [] {
@@ -185,6 +186,8 @@ inline TileTuning &tile_tuning() {
     x.kz = e ? std::max(0, std::atoi(e)) : 0;
     e = std::getenv("FPC_Z_EXTRUDE");
     x.z_extrude = e ? (std::atoi(e) != 0) : 1;
+    e = std::getenv("FPC_MG_WIDE_FINEST");
+    x.mg_wide_finest = e ? (std::atoi(e) != 0) : 1;
     return x;
   }();
   return t;

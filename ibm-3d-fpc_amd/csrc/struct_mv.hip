@@ -6,11 +6,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
-#include <mutex>
-
-#include <atomic>
 #include <mutex>
 
 #include "extrude.hpp"
@@ -118,30 +116,37 @@ struct ProfRec {
   const void *owner;
   bool dropped;
 };
+// The records are process-wide and unsynchronised: timing is for one calling
+// thread at a time (fpc_ensemble_step refuses a multi-threaded pool while it is
+// on).  With timing off nothing here touches shared state.
 std::string g_prof_filter;
-bool g_prof = false;
+std::atomic<bool> g_prof{false};
 std::vector<ProfRec> g_prof_recs;
 std::vector<std::pair<hipEvent_t, hipEvent_t>> g_prof_pool;
 size_t g_prof_used = 0;
 hipEvent_t g_prof_open = nullptr;
-int g_prof_tag = -1;
-const void *g_prof_owner = nullptr;
+thread_local int t_prof_tag = -1;
+thread_local const void *t_prof_owner = nullptr;
 }  // namespace
 
-size_t prof_mark() { return g_prof_recs.size(); }
+bool prof_enabled() { return g_prof.load(std::memory_order_relaxed); }
+
+size_t prof_mark() { return prof_enabled() ? g_prof_recs.size() : 0; }
 
 void prof_tag(int tag, const void *owner) {
-  g_prof_tag = tag;
-  g_prof_owner = owner;
+  if (!prof_enabled()) return;
+  t_prof_tag = tag;
+  t_prof_owner = owner;
 }
 
 void prof_drop_skipped(size_t from, int ran, const void *owner) {
+  if (!prof_enabled()) return;
   for (size_t q = from; q < g_prof_recs.size(); ++q)
     if (g_prof_recs[q].owner == owner && g_prof_recs[q].tag >= ran) g_prof_recs[q].dropped = true;
 }
 
 bool prof_on(const char *cls) {
-  if (!g_prof) return false;
+  if (!prof_enabled()) return false;
   if (g_prof_filter == "*") return true;
   std::string f = "," + g_prof_filter + ",", c = "," + std::string(cls) + ",";
   return f.find(c) != std::string::npos;
@@ -162,7 +167,7 @@ void prof_end(const char *cls, double bytes) {
   if (!g_prof_open || g_prof_used >= g_prof_pool.size()) return;
   auto &pr = g_prof_pool[g_prof_used++];
   (void)hipEventRecord(pr.second, stream());
-  g_prof_recs.push_back({cls, bytes, pr.first, pr.second, g_prof_tag, g_prof_owner, false});
+  g_prof_recs.push_back({cls, bytes, pr.first, pr.second, t_prof_tag, t_prof_owner, false});
   g_prof_open = nullptr;
 }
 
@@ -856,6 +861,9 @@ int fpc_SetTuning(const char *key, long long value) {
   } else if (k == "z_extrude") {
     FPC_CHECK_ARG(value == 0 || value == 1, "z_extrude must be 0 or 1");
     t.z_extrude = (int)value;
+  } else if (k == "mg_wide_finest") {
+    FPC_CHECK_ARG(value == 0 || value == 1, "mg_wide_finest must be 0 or 1");
+    t.mg_wide_finest = (int)value;
   } else {
     FPC_CHECK_ARG(false, "unknown tuning key");
   }

@@ -217,7 +217,10 @@ int fpc_DeviceSynchronize(void);
  * workgroup of those kernels, 0 = automatic, at most 62 used (FPC_STENCIL_KZ); "z_extrude":
  * 1 (default) = operators and MG levels whose z-planes 1..nz-2 are equal keep a
  * three-plane copy that the solve loops read, 0 = off (FPC_Z_EXTRUDE); applies
- * to operators assembled and solvers set up afterwards, with identical results. */
+ * to operators assembled and solvers set up afterwards, with identical results;
+ * "mg_wide_finest": 1 (default) = an FP32 V-cycle's finest-level kernels read its
+ * FP64 input and write its FP64 output directly, 0 = through FP32 copies made by
+ * conversion kernels (FPC_MG_WIDE_FINEST), with identical results. */
 int fpc_SetTuning(const char *key, long long value);
 
 /* ---- kernel timing (no reference counterpart) ----

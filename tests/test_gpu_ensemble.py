@@ -79,3 +79,20 @@ def test_batched_ensemble_tiled_instances(torch_cuda):
             assert ens.members[i].scalar(k) == alone.scalar(k), (i, k)
         alone.close()
     ens.close()
+
+
+def test_threaded_ensemble_refuses_kernel_timing(torch_cuda):
+    """Kernel timing keeps process-wide records for one calling thread: a concurrent
+    ensemble with several workers refuses to step while it is on, and steps again
+    once it is off."""
+    pars = F.ensemble_workload(n=3, seed=4, nx=(24, 32), nyz=(8, 10), inlet_ramp_steps=2)
+    ens = F.Ensemble(pars, nthreads=2)
+    F.profile_enable("*")
+    try:
+        with pytest.raises(F.FpcError):
+            ens.step(1)
+    finally:
+        F.profile_enable("")
+    ens.step(1)
+    assert all(m.scalar("step") == 1 for m in ens.members)
+    ens.close()

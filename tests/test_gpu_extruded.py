@@ -35,6 +35,7 @@ def restore_tuning(torch_cuda):
     F.set_tuning("z_extrude", 1)
     F.set_tuning("tile_min_points", 1 << 19)
     F.set_tuning("stencil_kz", 0)
+    F.set_tuning("mg_wide_finest", 1)
 
 
 def _extruded(nx, ny, nz, scale, seed, flagged=0.0):
@@ -158,6 +159,31 @@ def test_extruded_fp32_vcycle_equals_full_storage(dims, scale, cdir):
     assert bits_equal(ext["y"], full["y"])
     assert ext["itp"] == full["itp"] and bits_equal(ext["xp"], full["xp"])
     assert rel_l2(ext["y"], F.pfmg_apply(F.StructBox(nx, ny, nz, vals), b)) < 1e-4
+
+
+@pytest.mark.parametrize("tiles", [True, False], ids=["tiled", "one-point"])
+@pytest.mark.parametrize("dims,scale,cdir", [CASES[0], CASES[2], CASES[3]], ids=["z", "y", "x"])
+def test_fp32_vcycle_reads_and_writes_fp64_on_the_finest_level(dims, scale, cdir, tiles):
+    """The FP32 V-cycle's finest-level kernels read the FP64 rhs and write the FP64
+    result directly (rounding as they read and write): the same bits as converting
+    to and from FP32 copies with separate kernels, on the z-restricting, tiled and
+    one-point finest-level kernels, for one V-cycle and a flexible GMRES solve."""
+    nx, ny, nz = dims
+    F.set_tuning("tile_min_points", 0 if tiles else 1 << 40)
+    vals = _extruded(nx, ny, nz, scale, seed=nz + 11, flagged=0.003)
+    b = np.random.default_rng(nz + 1).standard_normal(nx * ny * nz)
+    out = []
+    for wide in (1, 0):
+        F.set_tuning("mg_wide_finest", wide)
+        box = F.StructBox(nx, ny, nz, vals)
+        y = F.pfmg_apply(box, b, mg_bits=32)
+        sp = F.Solver(box, tol=1e-11, maxit=300, kdim=30, precond="pfmg", flexible=True, mg_bits=32)
+        x, its, _, rc = sp.solve(b)
+        assert rc == 0
+        out.append((y, x, its))
+    assert bits_equal(out[0][0], out[1][0]) and bits_equal(out[0][1], out[1][1]) and out[0][2] == out[1][2]
+    assert np.array_equal(out[0][0], out[0][0].astype(np.float32).astype(np.float64))  # FP32 values
+    assert rel_l2(out[0][0], F.pfmg_apply(F.StructBox(nx, ny, nz, vals), b)) < 1e-4
 
 
 def test_simulation_steps_equal_full_storage():
