@@ -53,8 +53,35 @@ ROOFLINE_CLASSES = ("spmv", "spmv_jacobi", "mg_down0", "mg_up0", "multidot", "or
 
 
 def _classes(rep):
-    return {k: {"launches": n, "ms": ms, "gbs": (b / (ms * 1e-3) / 1e9) if ms > 0 else None}
+    return {k: {"launches": n, "ms": ms, "bytes": b, "gbs": (b / (ms * 1e-3) / 1e9) if ms > 0 else None}
             for k, (n, ms, b) in rep.items()}
+
+
+def step_bytes(classes):
+    """Algorithmic bytes of one step summed over the timed kernel classes (the
+    event-timed warm step; Arnoldi tails, true residuals and reductions are not
+    timed classes), their kernel time, and the rate."""
+    if not classes:
+        return None
+    b = sum(v["bytes"] for v in classes.values())
+    ms = sum(v["ms"] for v in classes.values())
+    return {"bytes": b, "kernel_ms": ms, "achieved_gbs": b / (ms * 1e-3) / 1e9,
+            "frac": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+
+def vcycle_summary(classes):
+    """The pressure V-cycle from the event-timed warm step: kernel time and algorithmic
+    bytes (every level's down, restriction and up launches, the coarsest GEMV
+    excluded) per V-cycle, and the HBM rate they imply."""
+    n = classes.get("mg_down0", {}).get("launches")
+    if not n:
+        return None
+    mg = [v for k, v in classes.items() if k.startswith("mg")]
+    us = 1e3 * sum(v["ms"] for v in mg) / n
+    b = sum(v["bytes"] for v in mg) / n
+    return {"vcycles_per_step": n, "us_per_vcycle": us, "bytes_per_vcycle": b, "achieved_gbs": b / us / 1e3,
+            "frac": b / us / 1e3 / HBM_PEAK_GBS,
+            "level0_us": 1e3 * (classes["mg_down0"]["ms"] + classes["mg_up0"]["ms"]) / n}
 
 
 def run_sim(F, torch, kw, warmup, steps, roofline_timing=True):
@@ -196,6 +223,35 @@ def run_ensemble(F, torch, nthreads, n=64, steps=50, modes=("looped", "concurren
     return out
 
 
+def run_vcycle_rate(F, nit=12):
+    """Contraction of the pressure V-cycle on the benchmark's own pressure operator
+    (512x96x96, embedded with identity rows on solid cells): the stationary iteration
+    x += M^-1 (b - A x) from x = 0 on a seeded b (zero on solid cells), residual
+    norms per sweep, and the rate over the last sweeps; beside it, the mean residual
+    reduction per iteration that GMRES(30) + one V-cycle reaches on the same system."""
+    nx, ny, nz = WORKLOAD["nx"], WORKLOAD["ny"], WORKLOAD["nz"]
+    S = F.Setup(F.default_params(**WORKLOAD))
+    vals = S.get("Ap_vals")
+    fl = S.get("fluid_p").astype(bool)
+    box = F.StructBox(nx, ny, nz, vals)
+    b = np.random.default_rng(0).standard_normal(nx * ny * nz)
+    b[~fl] = 0.0
+    nb = float(np.linalg.norm(b))
+    x = np.zeros_like(b)
+    res = [1.0]
+    for _ in range(nit):
+        r = b - box.matvec(x)
+        x = x + F.pfmg_apply(box, r)
+        res.append(float(np.linalg.norm(b - box.matvec(x))) / nb)
+    tail = 6
+    rate = (res[-1] / res[-1 - tail]) ** (1.0 / tail)
+    sol = F.Solver(box, tol=1e-8, maxit=250, kdim=30, precond="pfmg")
+    _, its, rel, rc = sol.solve(b)
+    return {"metric": "pressure V-cycle contraction, 512x96x96 channel operator", "residuals": res,
+            "stationary_rate": rate, "gmres_its": its, "gmres_relres": rel,
+            "gmres_rate_per_iteration": (rel ** (1.0 / its)) if its else None}
+
+
 MIXED = dict(WORKLOAD, nx=1024, ny=160, nz=160)
 
 
@@ -306,6 +362,8 @@ def main():
                     help="256x64x64 drag-gradient run + 16 finite-difference directions (no headline line)")
     ap.add_argument("--mixed", action="store_true",
                     help="1024x160x160, 100 steps: FP64 against FP32-MG, per-field errors (no headline line)")
+    ap.add_argument("--vcycle-rate", action="store_true",
+                    help="contraction of the pressure V-cycle on the 512x96x96 operator (no headline line)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 2 GiB device-to-device copy 5 times (scale for HBM byte counters)")
     args = ap.parse_args()
@@ -335,6 +393,9 @@ def main():
     if args.mixed:
         print(json.dumps(run_mixed(F, torch)))
         return
+    if args.vcycle_rate:
+        print(json.dumps(run_vcycle_rate(F)))
+        return
     sim64, t64, its64, dom, rec, classes = run_sim(F, torch, WORKLOAD, args.warmup, args.steps,
                                                    not args.no_roofline)
     res = {
@@ -356,6 +417,8 @@ def main():
         "timers_s": {k: sim64.timer(k) for k in ("t_build", "t_mom", "t_p", "t_proj", "t_diag")},
         "cd": sim64.scalar("cd"), "cl": sim64.scalar("cl"),
         "roofline": roofline(dom, rec) if dom else None,
+        "vcycle": vcycle_summary(classes),
+        "timed_classes_step": step_bytes(classes),
         "cpu_baseline": None,
         "kernel_classes": classes,
     }
