@@ -84,6 +84,17 @@ def vcycle_summary(classes):
             "level0_us": 1e3 * (classes["mg_down0"]["ms"] + classes["mg_up0"]["ms"]) / n}
 
 
+def _its_totals(sim):
+    return [sim.scalar("its_total_" + c) for c in "uvwp"]
+
+
+def _its_per_step(sim, tot0, steps):
+    """Mean iterations per step over the last `steps` steps (one row: the K steps run
+    as one library call, so no host work of this script sits between them)."""
+    tot = _its_totals(sim)
+    return [[(b - a) / steps for a, b in zip(tot0, tot)]]
+
+
 def run_sim(F, torch, kw, warmup, steps, roofline_timing=True):
     """W untimed steps (the last one with every kernel class timed), then K timed steps
     with the dominant class timed by HIP events."""
@@ -91,13 +102,12 @@ def run_sim(F, torch, kw, warmup, steps, roofline_timing=True):
     if not roofline_timing:
         sim.step(warmup)
         torch.cuda.synchronize()
-        its = []
+        tot0 = _its_totals(sim)
         t0 = time.perf_counter()
-        for _ in range(steps):
-            sim.step(1)
-            its.append([int(sim.scalar(k)) for k in ("its_u", "its_v", "its_w", "its_p")])
+        sim.step(steps)
         torch.cuda.synchronize()
-        return sim, time.perf_counter() - t0, its, None, None, {}
+        dt = time.perf_counter() - t0
+        return sim, dt, _its_per_step(sim, tot0, steps), None, None, {}
     if warmup > 1:
         sim.step(warmup - 1)
     F.profile_enable("*")
@@ -108,13 +118,12 @@ def run_sim(F, torch, kw, warmup, steps, roofline_timing=True):
     dom = max(elig, key=lambda k: classes[k]["ms"]) if elig else ",".join(ROOFLINE_CLASSES)
     F.profile_enable(dom)
     torch.cuda.synchronize()
-    its = []
+    tot0 = _its_totals(sim)
     t0 = time.perf_counter()
-    for _ in range(steps):
-        sim.step(1)
-        its.append([int(sim.scalar(k)) for k in ("its_u", "its_v", "its_w", "its_p")])
+    sim.step(steps)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    its = _its_per_step(sim, tot0, steps)
     rep = F.profile_report()
     F.profile_enable("")
     if dom not in rep:
@@ -159,6 +168,19 @@ def copy_gbs(torch, nbytes=1 << 30, reps=10):
     e1.record()
     e1.synchronize()
     return 2.0 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+
+
+def gpu_step_iterations(F, warmup, nsteps):
+    """Iteration counts (u, v, w, p) of steps warmup+1 .. warmup+nsteps of the
+    workload on the GPU, one step per call (outside any timed region)."""
+    sim = F.Sim(F.default_params(**WORKLOAD))
+    sim.step(warmup)
+    out = []
+    for _ in range(nsteps):
+        sim.step(1)
+        out.append([int(sim.scalar("its_" + c)) for c in "uvwp"])
+    sim.close()
+    return out
 
 
 def cpu_baseline(warmup=2, nsteps=3):
@@ -441,7 +463,7 @@ def main():
         res["roofline"]["frac_of_copy"] = res["roofline"]["achieved"] / cg
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(args.warmup)
-        res["cpu_baseline"]["gpu_iterations"] = [[int(x) for x in r] for r in its64[:3]]
+        res["cpu_baseline"]["gpu_iterations"] = gpu_step_iterations(F, args.warmup, 3)
     print(json.dumps(res))
 
 

@@ -469,7 +469,10 @@ int batched_step(fpc_ensemble_s *E) {
   if ((rc = nonfinite_enqueue(E->nf_p))) return rc;
   if ((rc = stage_post(E->stage))) return rc;
   FPC_HIP(hipStreamSynchronize(stream()));
-  for (int i = 0; i < n; ++i) std::memcpy(E->sims[(size_t)i]->hdiag, E->stage->hdiag + 8 * i, 8 * sizeof(double));
+  for (int i = 0; i < n; ++i) {
+    std::memcpy(E->sims[(size_t)i]->hdiag, E->stage->hdiag + 8 * i, 8 * sizeof(double));
+    for (int c = 0; c < 4; ++c) E->sims[(size_t)i]->its_total[c] += E->sims[(size_t)i]->its[c];
+  }
   for (int i = 0; i < n; ++i)
     if (nonfinite_flags(E->nf_p)[i]) {
       set_error(FPC_ERROR_GENERIC, "non-finite pressure solve output at step %d of instance %d (reference exit 912)",

@@ -193,6 +193,7 @@ int fpc::sim_run_phase(fpc_sim M, int ph) {
       FPC_HIP(hipMemcpyAsync(M->hdiag, M->diag, 8 * sizeof(double), hipMemcpyDeviceToHost, st));
       FPC_HIP(hipStreamSynchronize(st));
       M->t_diag += now_s() - t0;
+      for (int c = 0; c < 4; ++c) M->its_total[c] += M->its[c];
       return 0;
     }
     default:
@@ -496,10 +497,12 @@ double fpc_sim_scalar(fpc_sim M, const char *name) {
     return n == zm[c] ? (double)m : (double)f;
   }
   const char *its[4] = {"its_u", "its_v", "its_w", "its_p"};
+  const char *tot[4] = {"its_total_u", "its_total_v", "its_total_w", "its_total_p"};
   const char *rel[4] = {"relres_u", "relres_v", "relres_w", "relres_p"};
   const char *err[4] = {"err_u", "err_v", "err_w", "err_p"};
   for (int c = 0; c < 4; ++c) {
     if (n == its[c]) return M->its[c];
+    if (n == tot[c]) return (double)M->its_total[c];
     if (n == rel[c]) return M->rel[c];
     if (n == err[c]) return M->err[c];
   }

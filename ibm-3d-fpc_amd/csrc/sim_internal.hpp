@@ -45,6 +45,7 @@ struct fpc_sim_s {
   int step = 0;
   double ramp = 0.0;
   int its[4] = {0, 0, 0, 0};
+  long long its_total[4] = {0, 0, 0, 0};  // summed over the steps run (u, v, w, p)
   double rel[4] = {0, 0, 0, 0};
   int err[4] = {0, 0, 0, 0};
   double t_build = 0, t_mom = 0, t_p = 0, t_proj = 0, t_diag = 0;

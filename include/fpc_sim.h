@@ -93,7 +93,8 @@ void *fpc_sim_stream(fpc_sim sim);
  * owned by the simulation; valid until fpc_sim_destroy). */
 int fpc_sim_flow_ops(fpc_sim sim, fpc_flow_ops *ops);
 /* step ramp div_l2 umax vmax wmax its_u its_v its_w its_p relres_u relres_v relres_w
- * relres_p fx fy cd cl err_u err_v err_w err_p; the setup counts Np Nu Nv Nw Nqu Nqv Nqw
+ * relres_p fx fy cd cl err_u err_v err_w err_p; its_total_c (c = u, v, w, p): iterations
+ * summed over the steps run; the setup counts Np Nu Nv Nw Nqu Nqv Nqw
  * n_pruned; zext_m_c and zext_flagged_c (c = u, v, w, p):
  * wall planes (0 = full planes) and flagged points of the hot copy the operator's solve
  * loops read (planned at its first solve); NaN if unknown. */

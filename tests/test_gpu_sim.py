@@ -229,3 +229,16 @@ def test_nonfinite_state_fails_the_momentum_solve(torch_cuda):
     sim.set("u", u)
     with pytest.raises(F.FpcError, match="code 1:"):
         sim.step(1)
+
+
+def test_iteration_totals_sum_the_steps(torch_cuda):
+    kw = dict(TINY, init_profile=1, noise_amp=1e-3, seed=4)
+    a = F.Sim(F.default_params(**kw))
+    per = []
+    for _ in range(3):
+        a.step(1)
+        per.append([a.scalar("its_" + c) for c in "uvwp"])
+    b = F.Sim(F.default_params(**kw))
+    b.step(3)
+    for q, c in enumerate("uvwp"):
+        assert a.scalar("its_total_" + c) == sum(r[q] for r in per) == b.scalar("its_total_" + c)
