@@ -57,7 +57,11 @@ struct BSys {
   State *S;
   const double *b;
   double *x;
+  int flex;  // Z holds one preconditioned vector per Arnoldi step (flexible GMRES), else one
 };
+
+// The preconditioned vector of Arnoldi step `step` (flexible: its own slot).
+__device__ inline double *z_slot(const BSys &s, int step) { return s.flex ? s.Z + (long long)step * s.vp : s.Z; }
 
 struct BLev {
   const double *c, *dinv, *wm, *wp;  // z-marching kernels: rows by zm
@@ -166,7 +170,7 @@ __global__ void __launch_bounds__(256) k_b_matvec(const BSys *sys, const int *it
   const int cur = S->st.i;
   int i, j, k;
   ijk_of(t, s.nx, s.ny, i, j, k);
-  const double *z = use_z ? s.Z : s.V + (long long)cur * s.vp;
+  const double *z = use_z ? z_slot(s, cur) : s.V + (long long)cur * s.vp;
   const double sg = S->sigma[cur];
   const double v = kry::arnoldi_row(s.fc, s.fpitch, FullMap(), s.nx, s.ny, s.nz, z, s.dinv_full, i, j, k, t);
   s.V[(long long)(cur + 1) * s.vp + t] = sg * v;
@@ -204,7 +208,7 @@ __global__ void k_b_backsolve(const BSys *sys, Sub sub) {
   if (q < sub.n) kry::backsolve_body(sys[sub.idx[q]].S);
 }
 
-// mode 0: x += [dinv] sum_j y_j V_j;  mode 1: W = sum_j y_j V_j
+// mode 0: x += [dinv] sum_j y_j V_j;  mode 1: W = sum_j y_j V_j;  mode 2: x += sum_j y_j Z_j
 __global__ void __launch_bounds__(256) k_b_combine(const BSys *sys, Sub sub, int mode) {
   __shared__ double y[KM + 1];
   const BSys &s = sys[sub.idx[blockIdx.y]];
@@ -214,6 +218,9 @@ __global__ void __launch_bounds__(256) k_b_combine(const BSys *sys, Sub sub, int
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < s.N; t += (long long)gridDim.x * blockDim.x) {
     if (mode == 0) {
       const double v = kry::combine_point(s.V, s.vp, t, y, i, s.dinv_full);
+      s.x[t] = s.x[t] + v;
+    } else if (mode == 2) {
+      const double v = kry::combine_point<double>(s.Z, s.vp, t, y, i, nullptr);
       s.x[t] = s.x[t] + v;
     } else {
       s.W[t] = kry::combine_point<double>(s.V, s.vp, t, y, i, nullptr);
@@ -296,7 +303,7 @@ __global__ void __launch_bounds__(256) k_b_gemv(const BLev *lev, const BSys *sys
   const int r = (blockIdx.x - first[q]) * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (r >= L.ncoarse) return;
   const double v = mgd::gemv_row<double>(L.inv, L.ncoarse, lev_b(L, s, in_mode, step), r, lane);
-  double *y = L.lvl == 0 ? s.Z : L.y;
+  double *y = L.lvl == 0 ? z_slot(s, step) : L.y;
   if (lane == 0) y[r] = v;
 }
 
@@ -321,7 +328,7 @@ __global__ void __launch_bounds__(256) k_b_up(const BLev *lev, const BSys *sys, 
   else
     v = mgd::up_point<double, 2>(L.fc, L.fpitch, FullMap(), L.nx, L.ny, L.nz, L.fdinv, b, L.fwm, L.fwp, L.yc, L.cnx,
                                  L.cny, L.ncd, L.w, i, j, k, t);
-  (L.lvl == 0 ? s.Z : L.y)[t] = v;
+  (L.lvl == 0 ? z_slot(s, step) : L.y)[t] = v;
 }
 
 // ------------------------------------------------------------ z-marching forms
@@ -332,7 +339,7 @@ __global__ void __launch_bounds__(kTileX *kTY) k_bt_matvec(const BSys *sys, cons
   State *S = s.S;
   if (S->st.done) return;
   const int cur = S->st.i;
-  kry::ArnoldiOp<kDinv> op{s.c, s.pitch, s.nx, s.ny, use_z ? s.Z : s.V + (long long)cur * s.vp, s.dinv,
+  kry::ArnoldiOp<kDinv> op{s.c, s.pitch, s.nx, s.ny, use_z ? z_slot(s, cur) : s.V + (long long)cur * s.vp, s.dinv,
                            s.V + (long long)(cur + 1) * s.vp, S->sigma[cur], s.zm};
   march_tile<double, kTileX, kTY>(op, s.nx, s.ny, s.nz, e.k0, e.k1, e.bxy & 0xffff, e.bxy >> 16);
 }
@@ -365,7 +372,7 @@ __global__ void __launch_bounds__(kTileX *kTY) k_bt_up(const BLev *lev, const BS
   const BSys &s = sys[L.sys];
   if (lev_skip(L, s, flags)) return;
   const double *b = lev_b(L, s, in_mode, step);
-  double *y = L.lvl == 0 ? s.Z : L.y;
+  double *y = L.lvl == 0 ? z_slot(s, step) : L.y;
   const int bx = e.bxy & 0xffff, by = e.bxy >> 16;
   if (L.cdir == 0) {
     mgd::UpOp<double, 0> op{L.c, L.pitch, L.nx, L.ny, L.dinv, b, L.wm, L.wp, L.yc, L.cnx, L.cny, L.ncd, y, L.w, L.zm};
@@ -463,6 +470,7 @@ class BatchGroup {
  public:
   int n = 0;
   int pc = 0;
+  int flex = 0;  // flexible GMRES (PFMG-style preconditioner): stored preconditioned vectors
   std::vector<GmresBatchView> view;
   BSys *dsys = nullptr;
   Table pts;          // one-point matvec over the small systems
@@ -711,9 +719,12 @@ int batch_group_create(const std::vector<fpc_StructSolver> &solvers, const std::
     GmresBatchView &v = G->view[(size_t)q];
     rc = gmres_batch_view(solvers[(size_t)q], &v);
     if (rc) break;
-    if (q == 0) G->pc = v.pc;
-    if (v.pc != G->pc) {
-      set_error(FPC_ERROR_ARG, "batched solve: all systems of a group need the same preconditioner");
+    if (q == 0) {
+      G->pc = v.pc;
+      G->flex = v.flexible;
+    }
+    if (v.pc != G->pc || v.flexible != G->flex) {
+      set_error(FPC_ERROR_ARG, "batched solve: all systems of a group need the same preconditioner and GMRES form");
       rc = FPC_ERROR_ARG;
       break;
     }
@@ -735,6 +746,7 @@ int batch_group_create(const std::vector<fpc_StructSolver> &solvers, const std::
     s.dinv_full = v.dinv;
     s.V = v.V;
     s.Z = v.Z;
+    s.flex = v.flexible;
     s.W = v.W;
     s.part = v.part;
     s.S = v.S;
@@ -938,7 +950,11 @@ int BatchGroup::solve(int *its_out, double *rel_out, int *err_out) {
     k_b_backsolve<<<(sc.n + 255) / 256, 256, 0, stream()>>>(dsys, sc);
     FPC_LAUNCH_CHECK();
     const unsigned gb = grid_blocks(maxN);
-    if (pc == 3) {
+    if (pc == 3 && flex) {
+      // flexible: the preconditioned vectors are stored; no V-cycle of the combination
+      k_b_combine<<<dim3(gb, sc.n), 256, 0, stream()>>>(dsys, sc, 2);
+      FPC_LAUNCH_CHECK();
+    } else if (pc == 3) {
       k_b_combine<<<dim3(gb, sc.n), 256, 0, stream()>>>(dsys, sc, 1);
       FPC_LAUNCH_CHECK();
       k_b_flags<<<1, 1024, 0, stream()>>>(sc, n, flags);

@@ -36,6 +36,7 @@ struct MgLevelView {
 // The workspace and settings of a set-up GMRES solver.
 struct GmresBatchView {
   int pc = 0;  // 0 none, 1 Jacobi fused into the matvec, 3 PFMG-style V-cycle
+  int flexible = 0;  // Z holds kd preconditioned vectors (x += sum y_j Z_j), else one
   double *V = nullptr, *Z = nullptr, *W = nullptr, *part = nullptr;
   kry::State *S = nullptr;
   long long N = 0, vp = 0;
@@ -49,8 +50,9 @@ struct GmresBatchView {
 };
 
 // Views of set-up solvers (krylov.hip, pfmg.hip).  Errors for settings the
-// batched kernels do not cover (flexible GMRES, callback or multi-sweep
-// preconditioners, FP32 or red/black V-cycles, more than one cycle).
+// batched kernels do not cover (flexible GMRES other than with the PFMG-style
+// preconditioner, callback or multi-sweep preconditioners, FP32 or red/black
+// V-cycles, more than one cycle).
 int gmres_batch_view(fpc_StructSolver s, GmresBatchView *v);
 int pfmg_batch_view(void *pfmg, std::vector<MgLevelView> &levels, const double **inv, int *ncoarse);
 

@@ -319,8 +319,8 @@ int gmres_batch_view(fpc_StructSolver s, GmresBatchView *v) {
     set_error(FPC_ERROR_ARG, "batched solve: a solver is not set up");
     return FPC_ERROR_ARG;
   }
-  if (G->flexible) {
-    set_error(FPC_ERROR_ARG, "batched solve: flexible GMRES is not supported");
+  if (G->flexible && G->pc_kind != PC_PFMG) {
+    set_error(FPC_ERROR_ARG, "batched solve: flexible GMRES is supported with the PFMG-style preconditioner only");
     return FPC_ERROR_ARG;
   }
   if (G->pc_kind == PC_NONE) v->pc = 0;
@@ -330,6 +330,7 @@ int gmres_batch_view(fpc_StructSolver s, GmresBatchView *v) {
     set_error(FPC_ERROR_ARG, "batched solve: only Jacobi (one sweep) and PFMG preconditioners are supported");
     return FPC_ERROR_ARG;
   }
+  v->flexible = G->flexible ? 1 : 0;
   v->V = G->V;
   v->Z = G->Z;
   v->W = G->W;

@@ -11,9 +11,11 @@
 //  * diagnostics (divL2, max|u|, forces) are reduced on the device;
 //  * the non-finite checks run on the device (exit codes 911 / 912 become
 //    error returns);
-//  * with mg_precision = 32 the pressure solve is flexible GMRES with the FP32
-//    PFMG, and the momentum Jacobi preconditioners keep 1/diag in FP32 (no
-//    reference counterpart).
+//  * the pressure solve keeps its preconditioned vectors (flexible GMRES form),
+//    which saves the V-cycle of HYPRE's final update x += M^-1 (V y);
+//  * with mg_precision = 32 the pressure solve runs the FP32 PFMG, and the
+//    momentum Jacobi preconditioners keep 1/diag in FP32 (no reference
+//    counterpart).
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -77,8 +79,12 @@ static int setup_solver_vel(fpc_StructSolver *s, fpc_StructSolver *pc, const fpc
 
 static int setup_solver_p(fpc_StructSolver *s, fpc_StructSolver *pc, const fpc_params &par, fpc_StructMatrix A,
                           fpc_StructVector b, fpc_StructVector x) {
-  // main.cu:207-221
-  int rc = (par.mg_precision == 32) ? fpc_StructFlexGMRESCreate(s) : fpc_StructGMRESCreate(s);
+  // main.cu:207-221.  The preconditioned vectors of the Arnoldi steps are kept (flexible
+  // form): the same iterates as HYPRE's right-preconditioned GMRES, whose final update
+  // x += M^-1 (V y) costs one more V-cycle per solve; here x += Z y.  FP32 V-cycles
+  // need the flexible form in any case.
+  const bool flex = par.mg_precision == 32 || fpc::tile_tuning().p_flexible;
+  int rc = flex ? fpc_StructFlexGMRESCreate(s) : fpc_StructGMRESCreate(s);
   rc |= fpc_StructGMRESSetTol(*s, par.p_tol);
   rc |= fpc_StructGMRESSetMaxIter(*s, par.p_maxit);
   rc |= fpc_StructGMRESSetKDim(*s, par.p_kdim);

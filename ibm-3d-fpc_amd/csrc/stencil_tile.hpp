@@ -176,6 +176,7 @@ struct TileTuning {
   int kz = -1;
   int z_extrude = 1;  // three-plane copies of z-extruded operators
   int mg_wide_finest = 1;  // FP32 V-cycles read and write the FP64 vectors on the finest level
+  int p_flexible = 1;      // the time step's pressure GMRES keeps its preconditioned vectors
 };
 inline TileTuning &tile_tuning() {
   static TileTuning t = [] {
@@ -188,6 +189,8 @@ inline TileTuning &tile_tuning() {
     x.z_extrude = e ? (std::atoi(e) != 0) : 1;
     e = std::getenv("FPC_MG_WIDE_FINEST");
     x.mg_wide_finest = e ? (std::atoi(e) != 0) : 1;
+    e = std::getenv("FPC_P_FLEXIBLE");
+    x.p_flexible = e ? (std::atoi(e) != 0) : 1;
     return x;
   }();
   return t;

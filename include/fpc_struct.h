@@ -220,7 +220,12 @@ int fpc_DeviceSynchronize(void);
  * to operators assembled and solvers set up afterwards, with identical results;
  * "mg_wide_finest": 1 (default) = an FP32 V-cycle's finest-level kernels read its
  * FP64 input and write its FP64 output directly, 0 = through FP32 copies made by
- * conversion kernels (FPC_MG_WIDE_FINEST), with identical results. */
+ * conversion kernels (FPC_MG_WIDE_FINEST), with identical results; "p_flexible":
+ * 1 (default) = the time step's pressure GMRES keeps the preconditioned vector of
+ * every Arnoldi step and forms x += sum y_j z_j, 0 = HYPRE's right-preconditioned
+ * update x += M^-1 (sum y_j v_j) with one more V-cycle (FPC_P_FLEXIBLE); the same
+ * iterates, the final update differing in rounding; applies to simulations created
+ * afterwards. */
 int fpc_SetTuning(const char *key, long long value);
 
 /* ---- kernel timing (no reference counterpart) ----

@@ -242,3 +242,26 @@ def test_iteration_totals_sum_the_steps(torch_cuda):
     b.step(3)
     for q, c in enumerate("uvwp"):
         assert a.scalar("its_total_" + c) == sum(r[q] for r in per) == b.scalar("its_total_" + c)
+
+
+def test_pressure_flexible_form_equals_right_preconditioned(torch_cuda):
+    """The time step's pressure GMRES keeps its preconditioned vectors (x += Z y); HYPRE's
+    right-preconditioned form (x += M^-1 V y, one more V-cycle) gives the same iterates:
+    equal iteration counts, fields equal to rounding."""
+    kw = dict(CFG1, init_profile=1, noise_amp=1e-3 * 0.45, seed=3)
+    out = []
+    try:
+        for flex in (1, 0):
+            F.set_tuning("p_flexible", flex)
+            s = F.Sim(F.default_params(**kw))
+            its = []
+            for _ in range(4):
+                s.step(1)
+                its.append([s.scalar("its_" + c) for c in "uvwp"])
+            out.append((its, {f: s.get(f) for f in FIELDS}))
+            s.close()
+    finally:
+        F.set_tuning("p_flexible", 1)
+    assert out[0][0] == out[1][0]
+    for f in FIELDS:
+        assert rel_l2(out[0][1][f], out[1][1][f]) < 1e-12, f
