@@ -539,9 +539,24 @@ struct PfmgImpl {
   int ncoarse = 0;
   double *w64 = nullptr;  // FP64 work (stand-alone solves)
   double *e64 = nullptr;
+  // Captured V-cycles, one per (input, output, skip flag) seen ("mg_graph" tuning key).
+  struct Graph {
+    const double *in;
+    double *out;
+    const int *skip;
+    int bits;
+    hipGraphExec_t exec;
+  };
+  std::vector<Graph> graphs;
 };
 
+void free_graphs(PfmgImpl *P) {
+  for (auto &g : P->graphs) (void)hipGraphExecDestroy(g.exec);
+  P->graphs.clear();
+}
+
 void free_levels(PfmgImpl *P) {
+  free_graphs(P);  // their launches hold the levels' buffers
   for (auto &l : P->L) {
     if (l.own_coef && l.coef) (void)hipFree(l.coef);
     if (l.own_dinv && l.dinv) (void)hipFree(l.dinv);
@@ -990,6 +1005,40 @@ int vcycle(PfmgImpl *P, const double *in, double *out, const int *skip) {
   return 0;
 }
 
+// One V-cycle as a captured graph: the level launches of a V-cycle are fixed once the
+// hierarchy is built, so the first cycle with a given input, output and skip flag is
+// captured and later ones replay it (one launch call instead of one per kernel; the
+// same kernels and arguments, so the same bits).  Not while kernel timing is on (its
+// events would be captured), and at most 96 graphs per hierarchy.
+template <typename T>
+int vcycle_maybe_graph(PfmgImpl *P, const double *in, double *out, const int *skip) {
+  if (!tile_tuning().mg_graph || prof_enabled()) return vcycle<T>(P, in, out, skip);
+  const int bits = (int)sizeof(T) * 8;
+  hipStream_t st = stream();
+  for (const auto &g : P->graphs)
+    if (g.in == in && g.out == out && g.skip == skip && g.bits == bits) {
+      FPC_HIP(hipGraphLaunch(g.exec, st));
+      return 0;
+    }
+  if (P->graphs.size() >= 96) return vcycle<T>(P, in, out, skip);
+  FPC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+  const int rc = vcycle<T>(P, in, out, skip);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(st, &graph);
+  if (rc || e != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    if (rc) return rc;
+    FPC_HIP(e);
+  }
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  FPC_HIP(ei);
+  P->graphs.push_back({in, out, skip, bits, exec});
+  FPC_HIP(hipGraphLaunch(exec, st));
+  return 0;
+}
+
 PfmgImpl *pimpl(fpc_StructSolver s) {
   if (!s || s->kind != SK_PFMG) return nullptr;
   return (PfmgImpl *)s->impl;
@@ -1053,7 +1102,7 @@ int pfmg_apply(void *pf, const double *b, double *x, const int *skip) {
     set_error(FPC_ERROR_GENERIC, "PFMG used before setup");
     return FPC_ERROR_GENERIC;
   }
-  int rc = (P->built_bits == 32) ? vcycle<float>(P, b, x, skip) : vcycle<double>(P, b, x, skip);
+  int rc = (P->built_bits == 32) ? vcycle_maybe_graph<float>(P, b, x, skip) : vcycle_maybe_graph<double>(P, b, x, skip);
   if (rc) return rc;
   // further cycles as a stationary iteration (MaxIter > 1 as a preconditioner)
   for (int it = 1; it < P->maxit; ++it) {

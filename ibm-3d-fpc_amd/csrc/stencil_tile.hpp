@@ -177,6 +177,7 @@ struct TileTuning {
   int z_extrude = 1;  // three-plane copies of z-extruded operators
   int mg_wide_finest = 1;  // FP32 V-cycles read and write the FP64 vectors on the finest level
   int p_flexible = 1;      // the time step's pressure GMRES keeps its preconditioned vectors
+  int mg_graph = 0;        // V-cycles replayed as captured graphs
 };
 inline TileTuning &tile_tuning() {
   static TileTuning t = [] {
@@ -191,6 +192,8 @@ inline TileTuning &tile_tuning() {
     x.mg_wide_finest = e ? (std::atoi(e) != 0) : 1;
     e = std::getenv("FPC_P_FLEXIBLE");
     x.p_flexible = e ? (std::atoi(e) != 0) : 1;
+    e = std::getenv("FPC_MG_GRAPH");
+    x.mg_graph = e ? (std::atoi(e) != 0) : 0;
     return x;
   }();
   return t;

@@ -867,6 +867,9 @@ int fpc_SetTuning(const char *key, long long value) {
   } else if (k == "p_flexible") {
     FPC_CHECK_ARG(value == 0 || value == 1, "p_flexible must be 0 or 1");
     t.p_flexible = (int)value;
+  } else if (k == "mg_graph") {
+    FPC_CHECK_ARG(value == 0 || value == 1, "mg_graph must be 0 or 1");
+    t.mg_graph = (int)value;
   } else {
     FPC_CHECK_ARG(false, "unknown tuning key");
   }

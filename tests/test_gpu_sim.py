@@ -265,3 +265,28 @@ def test_pressure_flexible_form_equals_right_preconditioned(torch_cuda):
     assert out[0][0] == out[1][0]
     for f in FIELDS:
         assert rel_l2(out[0][1][f], out[1][1][f]) < 1e-12, f
+
+
+def test_vcycle_graphs_replay_the_same_bits(torch_cuda, tiles_everywhere):
+    """With "mg_graph" on, each pressure V-cycle is captured once per (input, output)
+    and replayed: the same kernels and arguments, so every step's fields equal the
+    eagerly launched run bit for bit (several replays per graph, with the time step's
+    other work between them)."""
+    kw = dict(CFG1, init_profile=1, noise_amp=1e-3 * 0.45, seed=6)
+    out = []
+    try:
+        for g in (0, 1):
+            F.set_tuning("mg_graph", g)
+            s = F.Sim(F.default_params(**kw))
+            per = []
+            for _ in range(4):
+                s.step(1)
+                per.append(({f: s.get(f) for f in FIELDS}, [s.scalar("its_" + c) for c in "uvwp"]))
+            out.append(per)
+            s.close()
+    finally:
+        F.set_tuning("mg_graph", 0)
+    for (fa, ia), (fb, ib) in zip(out[0], out[1]):
+        assert ia == ib
+        for f in FIELDS:
+            assert bits_equal(fa[f], fb[f]), f
