@@ -217,7 +217,8 @@ struct RowOf {
 // (wider than T on the finest level of a reduced-precision cycle).
 template <typename T, typename TB = T>
 struct DownOp {
-  static constexpr int kAhead = 2;  // prefetch distance of the march (stencil_tile.hpp)
+  static constexpr int kAhead = 2;           // prefetch distance of march_tile (stencil_tile.hpp)
+  static constexpr int kAheadPipelined = 1;  // and of march_tile_pipelined
   const T *c;
   long long pitch;
   int nx, ny;
@@ -248,7 +249,8 @@ struct DownOp {
 // of a point come with its fetch, ahead of use.
 template <typename T, typename TB = T>
 struct DownRestrictZOp {
-  static constexpr int kAhead = 2;  // prefetch distance of the march (stencil_tile.hpp)
+  static constexpr int kAhead = 2;           // prefetch distance of march_tile (stencil_tile.hpp)
+  static constexpr int kAheadPipelined = 1;  // and of march_tile_pipelined
   const T *c;
   long long pitch;
   int nx, ny;

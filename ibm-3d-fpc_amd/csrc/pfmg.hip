@@ -443,7 +443,7 @@ __global__ void __launch_bounds__(kTileX *TY)
   if (skip && *skip) return;
   mgd::DownOp<T, TB> op{c, pitch, nx, ny, dinv, b, r, w, zm};
   const int k0 = blockIdx.z * kz;
-  march_tile<T, kTileX, TY>(op, nx, ny, nz, k0, min(k0 + kz, nz), blockIdx.x, blockIdx.y);
+  march_tile_pipelined<T, kTileX, TY>(op, nx, ny, nz, k0, min(k0 + kz, nz), blockIdx.x, blockIdx.y);
 }
 
 template <typename T, int TY, typename TB = T>
@@ -458,7 +458,7 @@ __global__ void __launch_bounds__(kTileX *TY)
   op.wm = wm; op.wp = wp; op.bc = bc; op.nz = nz; op.ncz = ncz;
   const int k0 = blockIdx.z * kz, k1 = min(k0 + kz, nz);
   op.k0 = k0;
-  march_tile<T, kTileX, TY>(op, nx, ny, nz, k0, k1 < nz ? k1 + 1 : k1, blockIdx.x, blockIdx.y);
+  march_tile_pipelined<T, kTileX, TY>(op, nx, ny, nz, k0, k1 < nz ? k1 + 1 : k1, blockIdx.x, blockIdx.y);
 }
 
 template <typename T, int D, int TY, typename TB = T, typename TO = T>
@@ -791,6 +791,10 @@ HotView<T> hot_view(const DLevel &F) {
                     (double)F.d.N};
 }
 
+// Tile rows of the down sweeps: 64x4 tiles (256 threads), so that with the registers of
+// the pipelined march (march_tile_pipelined) three workgroups share a CU.
+constexpr int kDownTY = 4;
+
 // The fused down-sweep of level l: one weighted-Jacobi sweep from zero, the
 // residual and its restriction into the coarse b.  b is read as TB (FP64 on the
 // finest level of an FP32 cycle, rounded to T as it is read).
@@ -808,9 +812,9 @@ int fused_down(PfmgImpl *P, int l, const TB *b, T w, const int *skip, hipStream_
   // when z is restricted on the fly, one weight plane and the coarse b instead of r
   const char *cls = (l == 0) ? "mg_down0" : "mg_down";
   if (tile_worthwhile(F.d) && F.cdir == 2) {
-    TileLaunch tl = tile_launch<8>(F.d);
+    TileLaunch tl = tile_launch<kDownTY>(F.d);
     FPC_PROF_BEGIN(cls);
-    k_down_restrict_z<T, 8, TB><<<tl.grid, tl.block, 0, st>>>(hc, H.pitch, H.zm, F.d.nx, F.d.ny, F.d.nz, hd, b, hwm,
+    k_down_restrict_z<T, kDownTY, TB><<<tl.grid, tl.block, 0, st>>>(hc, H.pitch, H.zm, F.d.nx, F.d.ny, F.d.nz, hd, b, hwm,
                                                                hwp, (T *)C.b, C.d.nz, w, tl.kz, skip);
     FPC_PROF_END(cls, rows * 9.0 * sz + (double)F.d.N * szb + (double)C.d.N * sz);
     FPC_LAUNCH_CHECK();
@@ -830,8 +834,8 @@ int fused_down(PfmgImpl *P, int l, const TB *b, T w, const int *skip, hipStream_
   }
   FPC_PROF_BEGIN(cls);
   {
-    TileLaunch tl = tile_launch<8>(F.d);
-    k_down_tiled<T, 8, TB><<<tl.grid, tl.block, 0, st>>>(hc, H.pitch, H.zm, F.d.nx, F.d.ny, F.d.nz, hd, b, (T *)F.r,
+    TileLaunch tl = tile_launch<kDownTY>(F.d);
+    k_down_tiled<T, kDownTY, TB><<<tl.grid, tl.block, 0, st>>>(hc, H.pitch, H.zm, F.d.nx, F.d.ny, F.d.nz, hd, b, (T *)F.r,
                                                          w, tl.kz, skip);
   }
   FPC_PROF_END(cls, rows * 8.0 * sz + (double)F.d.N * (szb + sz));

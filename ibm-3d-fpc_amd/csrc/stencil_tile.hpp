@@ -163,6 +163,98 @@ __device__ inline void march_tile(Op &op, int nx, int ny, int nz, int k0, int ke
   }
 }
 
+// The same walk, software-pipelined (used by the MG down sweeps, whose raws are small):
+// the own column is fetched D+1 planes ahead and the halo ring D planes ahead
+// (D = Op::kAheadPipelined) into a ring of P = D+2 raws per stream, and the plane loop
+// is unrolled by P, so every raw slot is a fixed register set.  march_tile rotates its
+// raws from register to register at the end of each plane, and a move of a register
+// whose load is in flight waits for it: there every plane waits for all its loads
+// (s_waitcnt vmcnt(0) before the barrier).  Here each use waits only for its own load
+// (vmcnt(n), n = loads issued after it), so the loads of the next planes stay in flight
+// across the barrier.  Every fetch is unconditional: a lane outside the box or the
+// ring, or a plane past the walk, loads a valid point already read (the nearest box
+// point, the last plane the walk needs), which keeps the count of loads in flight the
+// same on every lane and path.  The values computed are those of march_tile, bit for
+// bit.  The unrolled slots cost registers (126 VGPRs for the z-restricting down sweep
+// at 64x4 tiles and D = 1), so the up sweeps, whose raws are larger, keep march_tile.
+template <typename T, int TX, int TY, class Op>
+__device__ inline void march_tile_pipelined(Op &op, int nx, int ny, int nz, int k0, int kend, int bx, int by) {
+  using Raw = typename Op::Raw;
+  constexpr int D = Op::kAheadPipelined;
+  static_assert(D == 1 || D == 2, "prefetch distance 1 or 2");
+  constexpr int P = D + 2;
+  __shared__ T plane[2][TY + 2][TX + 2];
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i = bx * TX + tx, j = by * TY + ty;
+  const bool in = (i < nx) && (j < ny);
+  const int ic = i < nx ? i : nx - 1, jc = j < ny ? j : ny - 1;
+  int hx = 0, hy = 0;
+  const bool ring = tile_ring<TX, TY>(ty * TX + tx, hx, hy);
+  const int hi = bx * TX + hx, hj = by * TY + hy;
+  const bool ring_in = ring && hi >= 0 && hi < nx && hj >= 0 && hj < ny;
+  const int hic = hi < 0 ? 0 : (hi < nx ? hi : nx - 1), hjc = hj < 0 ? 0 : (hj < ny ? hj : ny - 1);
+  // The own and ring columns' flag words of a hot copy (bit k & 63 of word k >> 6; 0
+  // without flags).  A chunk's fetches cover planes k0-1 .. kend, at most 65 planes
+  // (chunks are at most kMaxChunk planes), so two words: both loaded here, and a
+  // fetch takes its point's flag from them by a select -- no load per plane.
+  const int w0 = (k0 > 0 ? k0 - 1 : 0) >> 6, w1 = (w0 + 1) * 64 < nz ? w0 + 1 : w0;
+  const long long co = (long long)jc * nx + ic, cr = (long long)hjc * nx + hic;
+  const unsigned long long o0 = op.zm.word(co, w0), o1 = op.zm.word(co, w1);
+  const unsigned long long r0 = op.zm.word(cr, w0), r1 = op.zm.word(cr, w1);
+  pin_all(o0, o1, r0, r1);  // their wait here, once, not at every use in the loop
+  auto fown = [&](int kk) -> int { return (int)((((kk >> 6) == w0 ? o0 : o1) >> (kk & 63)) & 1ull); };
+  auto fring = [&](int kk) -> int { return (int)((((kk >> 6) == w0 ? r0 : r1) >> (kk & 63)) & 1ull); };
+  // planes the walk needs: own column k0-1 .. min(kend, nz-1), ring k0 .. kend-1
+  const int ko_last = kend < nz ? kend : nz - 1, kr_last = kend - 1;
+  auto fo = [&](int kk) -> Raw {
+    kk = kk < ko_last ? kk : ko_last;
+    return op.fetch(ic, jc, kk, fown(kk));
+  };
+  auto fr = [&](int kk) -> Raw {
+    kk = kk < kr_last ? kk : kr_last;
+    return op.fetch(hic, hjc, kk, fring(kk));
+  };
+  const Raw below = fo(k0 > 0 ? k0 - 1 : 0);
+  Raw own[P], rq[P];
+#pragma unroll
+  for (int q = 0; q <= D; ++q) own[q] = fo(k0 + q);
+#pragma unroll
+  for (int q = 0; q < D; ++q) rq[q] = fr(k0 + q);
+  T zm = (in && k0 > 0) ? op.finish(below) : T(0);
+  T zc = in ? op.finish(own[0]) : T(0);
+  Coef7<T> a;
+  long long arow = -1;
+  // plane k in slot s = (k - k0) mod P
+  auto plane_step = [&](const int s, const int k) {
+    if (in && own[s].u != arow) {
+      a.load(op.c, op.pitch, own[s].u);
+      arow = own[s].u;
+    }
+    own[(s + D + 1) % P] = fo(k + D + 1);
+    rq[(s + D) % P] = fr(k + D);
+    const T zp = (in && k + 1 < nz) ? op.finish(own[(s + 1) % P]) : T(0);
+    T(*Pl)[TX + 2] = plane[(k - k0) & 1];
+    Pl[ty + 1][tx + 1] = zc;
+    if (ring) Pl[hy + 1][hx + 1] = ring_in ? op.finish(rq[s]) : T(0);
+    __syncthreads();
+    if (in) {
+      Nbr<T> nb{zc, Pl[ty + 1][tx], Pl[ty + 1][tx + 2], Pl[ty][tx + 1], Pl[ty + 2][tx + 1], zm, zp,
+                nbr_in_box(i, j, k, nx, ny, nz)};
+      op.out(a, own[s], i, j, k, nb);
+    }
+    zm = zc;
+    zc = zp;
+  };
+  int kb = k0;
+  for (; kb + P <= kend; kb += P) {
+#pragma unroll
+    for (int s = 0; s < P; ++s) plane_step(s, kb + s);
+  }
+#pragma unroll
+  for (int s = 0; s < P - 1; ++s)
+    if (kb + s < kend) plane_step(s, kb + s);
+}
+
 // Launch geometry of a z-marching kernel: tile rows TY and z-chunk length.
 struct TileLaunch {
   dim3 grid, block;
