@@ -493,8 +493,17 @@ class BatchGroup {
   std::vector<BSys> hsys;
   std::vector<int> nlev, base;  // levels per system, first level index
   TileConfig tiles{4, 16384};
+  // V-cycles of the Arnoldi steps captured as graphs, by step ("mg_graph" tuning key);
+  // freed whenever the launch tables they hold are rebuilt
+  std::vector<std::pair<int, hipGraphExec_t>> vgraphs;
+
+  void free_graphs() {
+    for (auto &g : vgraphs) (void)hipGraphExecDestroy(g.second);
+    vgraphs.clear();
+  }
 
   ~BatchGroup() {
+    free_graphs();
     if (dsys) (void)hipFree(dsys);
     if (dlev) (void)hipFree(dlev);
     free_table(pts);
@@ -570,10 +579,39 @@ class BatchGroup {
     return 0;
   }
 
+  // The V-cycle of Arnoldi step `step` as a captured graph: captured at the first step
+  // with this index and replayed afterwards (the same kernels and arguments; systems
+  // already converged skip on their solve state, read when the kernels run).
+  int vcycle_graph(int step) {
+    hipStream_t st = stream();
+    for (const auto &g : vgraphs)
+      if (g.first == step) {
+        FPC_HIP(hipGraphLaunch(g.second, st));
+        return 0;
+      }
+    FPC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    const int rc = vcycle(0, step, nullptr);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &graph);
+    if (rc || e != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      if (rc) return rc;
+      FPC_HIP(e);
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    FPC_HIP(ei);
+    vgraphs.emplace_back(step, exec);
+    FPC_HIP(hipGraphLaunch(exec, st));
+    return 0;
+  }
+
   // One Arnoldi step of every system still in its cycle; step = host-side basis index.
   int run_step(int step) {
     if (pc == 3) {
-      int rc = vcycle(0, step, nullptr);
+      const bool graph = tile_tuning().mg_graph && !prof_enabled();
+      int rc = graph ? vcycle_graph(step) : vcycle(0, step, nullptr);
       if (rc) return rc;
     }
     if (pts.nblocks) {
@@ -602,6 +640,7 @@ class BatchGroup {
 
   // (Re)builds the launch tables of the matvec and V-cycle levels for tile configuration cfg.
   int build_tables(const TileConfig &cfg) {
+    free_graphs();
     free_table(pts);
     free_tiles(ptsT);
     for (auto &t : down) free_table(t);

@@ -96,3 +96,23 @@ def test_threaded_ensemble_refuses_kernel_timing(torch_cuda):
     ens.step(1)
     assert all(m.scalar("step") == 1 for m in ens.members)
     ens.close()
+
+
+def test_batched_ensemble_with_captured_vcycles(torch_cuda):
+    """With "mg_graph" on, the batched V-cycle of each Arnoldi step is captured once and
+    replayed: every instance still equals its standalone run bit for bit."""
+    pars = _params()
+    F.set_tuning("mg_graph", 1)
+    try:
+        ens = F.Ensemble(pars, nthreads=0)
+        ens.step(3)
+        got = [{f: m.get(f) for f in ("u", "v", "w", "p")} for m in ens.members]
+        ens.close()
+    finally:
+        F.set_tuning("mg_graph", 0)
+    for i, p in enumerate(pars):
+        alone = F.Sim(p)
+        alone.step(3)
+        for f in ("u", "v", "w", "p"):
+            assert bits_equal(got[i][f], alone.get(f)), (i, f)
+        alone.close()
