@@ -10,8 +10,10 @@ same steps and reported beside it.
 
 roofline: the kernel class with the largest device time (found on the last
 warmup step with every class timed) is timed over the whole timed region with
-HIP events on the library's stream; achieved = its algorithmic bytes per
-launch / its mean launch duration, against the 8 TB/s HBM3E peak.  Launches
+HIP events on the library's stream, on every 8th launch (a step has ~37 launches
+of the Gram-Schmidt classes, so the sample cycles through the basis lengths, at
+an eighth of the event cost); achieved = its algorithmic bytes per launch / its
+mean launch duration, against the 8 TB/s HBM3E peak.  Launches
 enqueued ahead of a converged GMRES cycle return at once and are not counted.
 traffic = HBM bytes per launch read from profiles/hbm_traffic.txt (FETCH_SIZE +
 WRITE_SIZE per dispatch, calibrated on a copy kernel) when that file names the
@@ -46,6 +48,7 @@ WORKLOAD = dict(nx=512, ny=96, nz=96, Lx=2.5, Ly=0.41, Lz=0.41, xcirc=0.5, ycirc
 
 
 HBM_PEAK_GBS = 8000.0
+ROOFLINE_SAMPLE = 8  # the roofline class's launches timed: every 8th
 # kernel classes whose every launch covers one fixed box (a roofline figure per
 # launch is meaningful); the other MG classes mix the coarse levels
 ROOFLINE_CLASSES = ("spmv", "spmv_jacobi", "mg_down0", "mg_up0", "multidot", "orth_update", "explicit",
@@ -116,7 +119,10 @@ def run_sim(F, torch, kw, warmup, steps, roofline_timing=True):
     classes = _classes(F.profile_report()) if warmup >= 1 else {}
     elig = [k for k in classes if k in ROOFLINE_CLASSES]
     dom = max(elig, key=lambda k: classes[k]["ms"]) if elig else ",".join(ROOFLINE_CLASSES)
+    # the dominant class timed on every ROOFLINE_SAMPLE-th launch across the timed region:
+    # an unbiased mean launch duration at an eighth of the event cost
     F.profile_enable(dom)
+    F.profile_sample(ROOFLINE_SAMPLE)
     torch.cuda.synchronize()
     tot0 = _its_totals(sim)
     t0 = time.perf_counter()
@@ -125,6 +131,7 @@ def run_sim(F, torch, kw, warmup, steps, roofline_timing=True):
     dt = time.perf_counter() - t0
     its = _its_per_step(sim, tot0, steps)
     rep = F.profile_report()
+    F.profile_sample(1)
     F.profile_enable("")
     if dom not in rep:
         classes = _classes(rep)

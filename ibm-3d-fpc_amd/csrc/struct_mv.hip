@@ -125,6 +125,8 @@ std::vector<ProfRec> g_prof_recs;
 std::vector<std::pair<hipEvent_t, hipEvent_t>> g_prof_pool;
 size_t g_prof_used = 0;
 hipEvent_t g_prof_open = nullptr;
+int g_prof_every = 1;            // time every n-th launch of the enabled classes
+unsigned long long g_prof_seen = 0;
 thread_local int t_prof_tag = -1;
 thread_local const void *t_prof_owner = nullptr;
 }  // namespace
@@ -145,11 +147,17 @@ void prof_drop_skipped(size_t from, int ran, const void *owner) {
     if (g_prof_recs[q].owner == owner && g_prof_recs[q].tag >= ran) g_prof_recs[q].dropped = true;
 }
 
-bool prof_on(const char *cls) {
-  if (!prof_enabled()) return false;
+static bool prof_class_on(const char *cls) {
   if (g_prof_filter == "*") return true;
   std::string f = "," + g_prof_filter + ",", c = "," + std::string(cls) + ",";
   return f.find(c) != std::string::npos;
+}
+
+bool prof_on(const char *cls) {
+  if (!prof_enabled()) return false;
+  if (!prof_class_on(cls)) return false;
+  if (g_prof_every <= 1) return true;
+  return (g_prof_seen++ % (unsigned long long)g_prof_every) == 0;
 }
 
 void prof_begin(const char *cls) {
@@ -873,6 +881,13 @@ int fpc_SetTuning(const char *key, long long value) {
   } else {
     FPC_CHECK_ARG(false, "unknown tuning key");
   }
+  return 0;
+}
+
+int fpc_ProfileSample(int every) {
+  FPC_CHECK_ARG(every >= 1, "every must be >= 1");
+  g_prof_every = every;
+  g_prof_seen = 0;
   return 0;
 }
 

@@ -141,7 +141,8 @@ SIGNATURES = {
     "fpc_Free": (_i, [_vp]), "fpc_MemcpyToHost": (_i, [_vp, _vp, _sz]), "fpc_MemcpyToDevice": (_i, [_vp, _vp, _sz]),
     "fpc_DeviceSynchronize": (_i, []),
     "fpc_SetTuning": (_i, [_cp, ctypes.c_longlong]),
-    "fpc_ProfileEnable": (_i, [_cp]), "fpc_ProfileReset": (_i, []), "fpc_ProfileReport": (_i, [_cp, _i]),
+    "fpc_ProfileEnable": (_i, [_cp]), "fpc_ProfileSample": (_i, [_i]), "fpc_ProfileReset": (_i, []),
+    "fpc_ProfileReport": (_i, [_cp, _i]),
     # fpc_dense.h
     "fpc_apply_bc_cleanup": (_i, [_vp] * 7 + [_i] * 3),
     "fpc_conv_u": (_i, [_vp] * 5 + [_i] * 3 + [_d] * 3), "fpc_conv_v": (_i, [_vp] * 5 + [_i] * 3 + [_d] * 3),
@@ -509,6 +510,11 @@ class Ensemble:
             self.close()
         except Exception:
             pass
+
+
+def profile_sample(every=1):
+    """Time only every `every`-th launch of the enabled classes."""
+    check(lib().fpc_ProfileSample(int(every)), "ProfileSample")
 
 
 def profile_enable(names="*"):

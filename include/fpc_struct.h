@@ -235,6 +235,9 @@ int fpc_SetTuning(const char *key, long long value);
  * = off).  Report lines: "class launches total_ms algorithmic_bytes\n"; returns
  * the length needed including the terminating zero. */
 int fpc_ProfileEnable(const char *names);
+/* Time only every n-th launch of the enabled classes (n >= 1; 1 = every launch, the
+ * default): the event cost then falls on 1/n of the launches. */
+int fpc_ProfileSample(int every);
 int fpc_ProfileReset(void);
 int fpc_ProfileReport(char *buf, int buflen);
 

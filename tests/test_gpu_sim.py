@@ -290,3 +290,23 @@ def test_vcycle_graphs_replay_the_same_bits(torch_cuda, tiles_everywhere):
         assert ia == ib
         for f in FIELDS:
             assert bits_equal(fa[f], fb[f]), f
+
+
+def test_kernel_timing_sampled_launches(torch_cuda):
+    """fpc_ProfileSample(n): only every n-th launch of the enabled classes is timed."""
+    kw = dict(TINY, init_profile=1, noise_amp=1e-3, seed=2)
+    counts = []
+    for n in (1, 4):
+        s = F.Sim(F.default_params(**kw))
+        s.step(1)
+        F.profile_enable("multidot")
+        F.profile_sample(n)
+        try:
+            s.step(2)
+            rep = F.profile_report()
+        finally:
+            F.profile_sample(1)
+            F.profile_enable("")
+        counts.append(rep["multidot"][0])
+        s.close()
+    assert counts[0] >= 8 and abs(counts[1] - counts[0] / 4) <= 1 + counts[0] / 8, counts
