@@ -124,12 +124,14 @@ __device__ inline void ijk_of(long long t, int nx, int ny, int &i, int &j, int &
 }
 
 // ------------------------------------------------------------ Krylov kernels
-__global__ void __launch_bounds__(256) k_b_residual(const BSys *sys, Sub sub, int out_mode) {
+// zero: every system starts from x = 0, so r = b (copied, the same partials bit for bit)
+__global__ void __launch_bounds__(256) k_b_residual(const BSys *sys, Sub sub, int out_mode, int zero) {
   const BSys &s = sys[sub.idx[blockIdx.y]];
   if ((int)blockIdx.x >= s.nred) return;
   if (blockIdx.z == 0) {
     double *r = (out_mode == 0) ? s.V : (out_mode == 1 ? s.W : nullptr);
-    kry::residual_body(s.fc, s.fpitch, FullMap(), s.nx, s.ny, s.nz, s.b, s.x, r, s.part, blockIdx.x, s.nred);
+    if (zero && r) kry::copy_dot_body(s.b, r, s.N, s.part, blockIdx.x, s.nred);
+    else kry::residual_body(s.fc, s.fpitch, FullMap(), s.nx, s.ny, s.nz, s.b, s.x, r, s.part, blockIdx.x, s.nred);
   } else {
     kry::dot_body(s.b, s.b, s.N, s.part + NB, blockIdx.x, s.nred);
   }
@@ -531,11 +533,11 @@ class BatchGroup {
 
   // ||b - A x||^2 (r into V0 when out_mode 0, W when 1) and optionally ||b||^2 of the
   // systems ids; values in hscal[q * 2 + v]
-  int true_residual(const std::vector<int> &ids, int out_mode, bool with_b) {
+  int true_residual(const std::vector<int> &ids, int out_mode, bool with_b, bool zero = false) {
     if (ids.empty()) return 0;
     const Sub s = sub_of(ids);
     const int nv = with_b ? 2 : 1;
-    k_b_residual<<<dim3(maxnred, s.n, nv), 256, 0, stream()>>>(dsys, s, out_mode);
+    k_b_residual<<<dim3(maxnred, s.n, nv), 256, 0, stream()>>>(dsys, s, out_mode, zero ? 1 : 0);
     FPC_LAUNCH_CHECK();
     k_b_reduce<<<dim3(nv, s.n), 1024, 0, stream()>>>(dsys, s, 2, dscal);
     FPC_LAUNCH_CHECK();
@@ -896,7 +898,12 @@ int BatchGroup::solve(int *its_out, double *rel_out, int *err_out) {
   std::vector<int> iter((size_t)n, 0), active((size_t)n, 1), first((size_t)n, 1);
   std::vector<int> all((size_t)n);
   for (int q = 0; q < n; ++q) all[(size_t)q] = q;
-  int rc = true_residual(all, 0, true);
+  // every system hinted to start from x = 0: the initial residuals are copies of b
+  bool zero = true;
+  for (int q = 0; q < n; ++q) zero = zero && view[(size_t)q].x0_zero && *view[(size_t)q].x0_zero;
+  for (int q = 0; q < n; ++q)
+    if (view[(size_t)q].x0_zero) *view[(size_t)q].x0_zero = false;
+  int rc = true_residual(all, 0, true, zero);
   if (rc) return rc;
   for (int q = 0; q < n; ++q) {
     r_norm[(size_t)q] = std::sqrt(hscal[2 * q]);

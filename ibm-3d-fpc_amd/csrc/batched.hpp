@@ -47,6 +47,7 @@ struct GmresBatchView {
   void *pfmg = nullptr;
   int *pred = nullptr, *its = nullptr;
   double *relres = nullptr;
+  bool *x0_zero = nullptr;  // the solver's zero-initial-guess hint (gmres_next_guess_is_zero)
 };
 
 // Views of set-up solvers (krylov.hip, pfmg.hip).  Errors for settings the

@@ -442,6 +442,11 @@ int batched_step(fpc_ensemble_s *E) {
   const int n = (int)E->sims.size();
   int rc = stage_pre(E->stage, E->sims);
   if (rc) return rc;
+  // the x0 just written: zero where the instance asks for it (main.cu:272), so the
+  // solves copy b as their initial residual
+  for (fpc_sim M : E->sims)
+    if (M->par.use_zero_initial_guess)
+      for (fpc_StructSolver s : {M->su, M->sv, M->sw}) gmres_next_guess_is_zero(s);
   if ((rc = batch_group_solve(E->mom, E->its.data(), E->rel.data(), E->err.data()))) return rc;
   for (int i = 0; i < n; ++i)
     for (int c = 0; c < 3; ++c) {
@@ -460,6 +465,7 @@ int batched_step(fpc_ensemble_s *E) {
     }
   }
   if ((rc = stage_mid(E->stage))) return rc;
+  for (fpc_sim M : E->sims) gmres_next_guess_is_zero(M->sp);  // pressure x0 = 0 (main.cu:320)
   if ((rc = batch_group_solve(E->pres, E->its.data(), E->rel.data(), E->err.data()))) return rc;
   for (int i = 0; i < n; ++i) {
     E->sims[(size_t)i]->its[3] = E->its[(size_t)i];

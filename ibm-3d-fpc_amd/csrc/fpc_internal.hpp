@@ -338,9 +338,17 @@ int residual_norm_partials(const fpc_StructMatrix_s *A, const double *b, const d
 // sum of nparts partials into *out (device)
 int reduce_partials(const double *part, int nparts, double *out);
 int dot_partials(const double *a, const double *b, long long n, double *part);
+// r = b with the partial sums of b*b that residual_norm_partials gives for x = 0 (bit for bit)
+int copy_norm_partials(const double *b, double *r, long long n, double *part);
 int matvec(double alpha, const fpc_StructMatrix_s *A, const double *x, double beta, double *y);
 int ensure_dinv(fpc_StructMatrix_s *A);
 int ensure_fp32(fpc_StructMatrix_s *A);
+
+// The next solve of GMRES solver s starts from x = 0 (its caller has just written a zero
+// initial guess): its initial residual is then b itself, copied instead of computed with
+// a matvec (the same bits).  Cleared by that solve.
+void gmres_next_guess_is_zero(fpc_StructSolver s);
+void gmres_clear_guess_hint(fpc_StructSolver s);
 
 // n GMRES solves (fpc_StructGMRESSolve each) advanced in lockstep with one host sync
 // per round for all of them (krylov.hip); rc[q] is what solve q alone would return.

@@ -268,6 +268,7 @@ struct GmresImpl {
   int its = 0;
   double relres = 0.0;
   int pred = 4;  // expected Arnoldi steps per cycle (from the previous solve)
+  bool x0_zero = false;  // the next solve starts from x = 0 (gmres_next_guess_is_zero)
   fpc_StructMatrix_s *A = nullptr;
   fpc_StructVector_s tmp_in, tmp_out;  // callback preconditioner views
 };
@@ -313,6 +314,14 @@ int precond_apply(GmresImpl *G, const double *in, double *out, const int *skip) 
 
 }  // namespace
 
+void gmres_next_guess_is_zero(fpc_StructSolver s) {
+  if (GmresImpl *G = gimpl(s)) G->x0_zero = true;
+}
+
+void gmres_clear_guess_hint(fpc_StructSolver s) {
+  if (GmresImpl *G = gimpl(s)) G->x0_zero = false;
+}
+
 int gmres_batch_view(fpc_StructSolver s, GmresBatchView *v) {
   GmresImpl *G = gimpl(s);
   if (!G || !G->A || G->N == 0 || !G->S) {
@@ -347,6 +356,7 @@ int gmres_batch_view(fpc_StructSolver s, GmresBatchView *v) {
   v->A = G->A;
   v->pfmg = (v->pc == 3) ? G->pc->impl : nullptr;
   v->pred = &G->pred;
+  v->x0_zero = &G->x0_zero;
   v->its = &G->its;
   v->relres = &G->relres;
   return 0;
@@ -590,7 +600,18 @@ static int run_top(GmresRun &R) {
 
 static int run_begin(GmresRun &R) {
   R.stage = GmresRun::kInit;
-  return true_residual(R.G, R.b, R.x, R.G->V, true);  // initial residual into V[0]
+  GmresImpl *G = R.G;
+  if (G->x0_zero) {
+    // x = 0: the initial residual is b (into V[0]), and ||r||^2 = ||b||^2 from the same partials
+    G->x0_zero = false;
+    int rc = copy_norm_partials(R.b, G->V, G->N, G->part);
+    if (!rc) rc = reduce_partials(G->part, G->nred, G->dscal);
+    if (!rc) rc = reduce_partials(G->part, G->nred, G->dscal + 1);
+    if (rc) return rc;
+    FPC_HIP(hipMemcpyAsync(G->hscal, G->dscal, 2 * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    return 0;
+  }
+  return true_residual(G, R.b, R.x, G->V, true);  // initial residual into V[0]
 }
 
 // After a sync: read this run's host copy and enqueue its next stage.

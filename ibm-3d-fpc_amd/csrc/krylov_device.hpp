@@ -387,6 +387,22 @@ __device__ inline void dot_body(const double *__restrict__ a, const double *__re
   if (threadIdx.x == 0) part[bid] = tot;
 }
 
+// r = b with per-block partial sums of b^2: the residual of a zero initial guess.  Each
+// thread visits t, t + S, t + 2S, ... as residual_body does two points per trip, so the
+// partials equal residual_body's with x = 0 bit for bit (b - A 0 = b exactly).
+__device__ inline void copy_dot_body(const double *__restrict__ b, double *__restrict__ r, long long n,
+                                     double *__restrict__ part, int bid, int nblk) {
+  __shared__ double lds[8];
+  double acc = 0.0;
+  for (long long t = bid * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)nblk * blockDim.x) {
+    const double v = b[t];
+    r[t] = v;
+    acc += v * v;
+  }
+  double tot = block_sum<256>(acc, lds);
+  if (threadIdx.x == 0) part[bid] = tot;
+}
+
 // Sum of n partials into *out (one block of 1024).
 __device__ inline void reduce_body(const double *__restrict__ part, int n, double *__restrict__ out) {
   __shared__ double lds[16];

@@ -122,6 +122,8 @@ int fpc::sim_run_phase(fpc_sim M, int ph) {
       rc = stage_explicit_async(&M->ops, M->u, M->v, M->w, M->p, M->uin_step, M->pbu, M->pxu, M->pbv, M->pxv, M->pbw,
                                 M->pxw, par.use_zero_initial_guess != 0);
       if (rc) return rc;
+      if (par.use_zero_initial_guess)  // x0 = 0 just written: the solves copy b as r0
+        for (fpc_StructSolver s : {M->su, M->sv, M->sw}) fpc::gmres_next_guess_is_zero(s);
       FPC_HIP(hipStreamSynchronize(st));
       M->t_build += now_s() - t0;
       return 0;
@@ -165,7 +167,9 @@ int fpc::sim_run_phase(fpc_sim M, int ph) {
       return rc;
     case 4:  // main.cu:314-324
       rc = stage_divergence_async(&M->ops, M->ut, M->vt, M->wt, M->ramp, M->pbp, M->pxp);
-      return rc;
+      if (rc) return rc;
+      fpc::gmres_next_guess_is_zero(M->sp);  // x0 = 0 just written (main.cu:320)
+      return 0;
     case 5: {  // main.cu:326-336
       double t0 = now_s();
       rc = fpc_StructGMRESSolve(M->sp, M->Ap, M->bp, M->xp);
@@ -417,6 +421,8 @@ int fpc_sim_set(fpc_sim M, const char *name, const void *src, long long nbytes) 
   double *d = field_ptr(M, name, &n);
   FPC_CHECK_ARG(d, "unknown field");
   FPC_CHECK_ARG(nbytes == n * (long long)sizeof(double), "size mismatch");
+  // a field set by the caller may be a solver's initial guess: no zero-guess shortcut
+  for (fpc_StructSolver s : {M->su, M->sv, M->sw, M->sp}) fpc::gmres_clear_guess_hint(s);
   FPC_HIP(hipMemcpyAsync(d, src, (size_t)nbytes, hipMemcpyHostToDevice, fpc::stream()));
   FPC_HIP(hipStreamSynchronize(fpc::stream()));
   return 0;

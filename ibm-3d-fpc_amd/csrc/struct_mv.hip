@@ -273,6 +273,11 @@ __global__ void __launch_bounds__(256) k_dot(const double *__restrict__ a, const
   kry::dot_body(a, b, n, part, blockIdx.x, gridDim.x);
 }
 
+__global__ void __launch_bounds__(256) k_copy_dot(const double *__restrict__ b, double *__restrict__ r, long long n,
+                                                  double *__restrict__ part) {
+  kry::copy_dot_body(b, r, n, part, blockIdx.x, gridDim.x);
+}
+
 __global__ void __launch_bounds__(1024) k_reduce(const double *__restrict__ part, int n, double *__restrict__ out) {
   kry::reduce_body(part, n, out);
 }
@@ -327,6 +332,12 @@ int residual_norm_partials(const fpc_StructMatrix_s *A, const double *b, const d
     k_residual<FullMap><<<red_blocks(d.N), 256, 0, stream()>>>(A->coef, A->pitch, FullMap(), d.nx, d.ny, d.nz, b, x, r,
                                                                part);
   }
+  FPC_LAUNCH_CHECK();
+  return 0;
+}
+
+int copy_norm_partials(const double *b, double *r, long long n, double *part) {
+  k_copy_dot<<<red_blocks(n), 256, 0, stream()>>>(b, r, n, part);
   FPC_LAUNCH_CHECK();
   return 0;
 }

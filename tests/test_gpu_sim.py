@@ -310,3 +310,26 @@ def test_kernel_timing_sampled_launches(torch_cuda):
         counts.append(rep["multidot"][0])
         s.close()
     assert counts[0] >= 8 and abs(counts[1] - counts[0] / 4) <= 1 + counts[0] / 8, counts
+
+
+def test_zero_guess_residual_equals_the_matvec_residual(torch_cuda, tiles_everywhere):
+    """After the stages write x0 = 0 the solves take b itself as the initial residual
+    (no matvec); setting a field from the host clears that shortcut.  Both give the same
+    bits: b - A 0 = b, and the norm partials are summed in the same order."""
+    kw = dict(CFG1, init_profile=1, noise_amp=1e-3 * 0.45, seed=9)
+    runs = []
+    for clear in (False, True):
+        s = F.Sim(F.default_params(**kw))
+        for _ in range(3):
+            for ph in range(8):
+                if clear and ph in (2, 5):
+                    name = "x_p" if ph == 5 else "x_u"
+                    s.set(name, s.get(name))  # same values; clears the zero-guess hints
+                s.phase(ph)
+        runs.append(({f: s.get(f) for f in FIELDS}, [s.scalar("its_" + c) for c in "uvwp"],
+                     [s.scalar("relres_" + c) for c in "uvwp"]))
+        s.close()
+    (fa, ia, ra), (fb, ib, rb) = runs
+    assert ia == ib and ra == rb
+    for f in FIELDS:
+        assert bits_equal(fa[f], fb[f]), f
