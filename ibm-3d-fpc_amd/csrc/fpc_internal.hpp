@@ -113,10 +113,13 @@ struct Dims {
   long long N = 0;
 };
 
-inline void launch_shape(const Dims &d, dim3 &grid, dim3 &block) {
+// threads: workgroup size (a power of two, 64..256).  Smaller workgroups spread a
+// small box over more CUs.
+inline void launch_shape(const Dims &d, dim3 &grid, dim3 &block, int threads = 256) {
   int tx = 64;
   while (tx > 1 && tx / 2 >= d.nx) tx /= 2;
-  int ty = 256 / tx;
+  int ty = threads / tx;
+  if (ty < 1) ty = 1;
   while (ty > 1 && ty / 2 >= d.ny) ty /= 2;
   block = dim3(tx, ty, 1);
   grid = dim3((d.nx + tx - 1) / tx, (d.ny + ty - 1) / ty, d.nz);

@@ -29,30 +29,92 @@ __device__ inline T add_if(bool in, T s, T a, T v) {
   return in ? s + a * v : s;
 }
 
-// Residual of one weighted-Jacobi sweep from zero: b - A (w D^-1 b) at (i, j, k) = row t.
-// b may be stored wider than T (TB): each value is rounded to T when read.
+// v unchanged, passed through a volatile no-op: memory operations cannot move
+// across it, and what is computed from the result comes after it.  Applied to every
+// raw loaded value of a one-point body after all of its loads, it makes the wave issue
+// every load before the first wait (one memory round trip), instead of the scheduler
+// interleaving arithmetic, and the waits it needs, between the loads.
+template <typename T>
+__device__ inline void hold(T &v) {
+  asm volatile("" : "+v"(v));
+}
+
+// The loads of one Jacobi-from-zero residual (down_point): b (as stored, TB) and
+// 1/diag at the point and its six neighbours (slot order C, -x, +x, -y, +y, -z, +z)
+// and the seven coefficients of its row.  A neighbour outside the box is read at
+// the point itself.
+template <typename T, typename TB = T>
+struct DownRaw {
+  TB b[7];
+  T d[7], a[7];
+};
+
 template <typename T, class M, typename TB = T>
-__device__ inline T down_point(const T *__restrict__ c, long long pitch, M zm, int nx, int ny, int nz,
-                               const T *__restrict__ dinv, const TB *__restrict__ b, T w, int i, int j, int k,
-                               long long t) {
+__device__ inline DownRaw<T, TB> down_loads(const T *__restrict__ c, long long pitch, M zm, int nx, int ny, int nz,
+                                            const T *__restrict__ dinv, const TB *__restrict__ b, int i, int j,
+                                            int k, long long t) {
   const long long pl = (long long)nx * ny;
   const bool xm = i > 0, xp = i < nx - 1, ym = j > 0, yp = j < ny - 1, zl = k > 0, zh = k < nz - 1;
   const long long u = zm(t, k);
   const long long t1 = xm ? t - 1 : t, t2 = xp ? t + 1 : t, t3 = ym ? t - nx : t, t4 = yp ? t + nx : t;
   const long long t5 = zl ? t - pl : t, t6 = zh ? t + pl : t;
-  const T b0 = (T)b[t];
-  const T x0 = w * b0 * dinv[u];
-  const T x1 = w * (T)b[t1] * dinv[zm(t1, k)], x2 = w * (T)b[t2] * dinv[zm(t2, k)];
-  const T x3 = w * (T)b[t3] * dinv[zm(t3, k)], x4 = w * (T)b[t4] * dinv[zm(t4, k)];
-  const T x5 = w * (T)b[t5] * dinv[zm(t5, zl ? k - 1 : k)], x6 = w * (T)b[t6] * dinv[zm(t6, zh ? k + 1 : k)];
-  T s = c[u] * x0;
-  s = add_if(xm, s, c[pitch + u], x1);
-  s = add_if(xp, s, c[2 * pitch + u], x2);
-  s = add_if(ym, s, c[3 * pitch + u], x3);
-  s = add_if(yp, s, c[4 * pitch + u], x4);
-  s = add_if(zl, s, c[5 * pitch + u], x5);
-  s = add_if(zh, s, c[6 * pitch + u], x6);
+  DownRaw<T, TB> r;
+  r.b[0] = b[t];
+  r.d[0] = dinv[u];
+  r.b[1] = b[t1];
+  r.d[1] = dinv[zm(t1, k)];
+  r.b[2] = b[t2];
+  r.d[2] = dinv[zm(t2, k)];
+  r.b[3] = b[t3];
+  r.d[3] = dinv[zm(t3, k)];
+  r.b[4] = b[t4];
+  r.d[4] = dinv[zm(t4, k)];
+  r.b[5] = b[t5];
+  r.d[5] = dinv[zm(t5, zl ? k - 1 : k)];
+  r.b[6] = b[t6];
+  r.d[6] = dinv[zm(t6, zh ? k + 1 : k)];
+#pragma unroll
+  for (int s = 0; s < 7; ++s) r.a[s] = c[s * pitch + u];
+  return r;
+}
+
+template <typename T, typename TB>
+__device__ inline void hold_raw(DownRaw<T, TB> &r) {
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    hold(r.b[q]);
+    hold(r.d[q]);
+    hold(r.a[q]);
+  }
+}
+
+// b - A (w D^-1 b) at (i, j, k) from its loads.
+// b is rounded to T here (the value down_point has always used).
+template <typename T, typename TB>
+__device__ inline T down_eval(const DownRaw<T, TB> &r, T w, int i, int j, int k, int nx, int ny, int nz) {
+  const bool xm = i > 0, xp = i < nx - 1, ym = j > 0, yp = j < ny - 1, zl = k > 0, zh = k < nz - 1;
+  const T b0 = (T)r.b[0];
+  const T x0 = w * b0 * r.d[0];
+  const T x1 = w * (T)r.b[1] * r.d[1], x2 = w * (T)r.b[2] * r.d[2];
+  const T x3 = w * (T)r.b[3] * r.d[3], x4 = w * (T)r.b[4] * r.d[4];
+  const T x5 = w * (T)r.b[5] * r.d[5], x6 = w * (T)r.b[6] * r.d[6];
+  T s = r.a[0] * x0;
+  s = add_if(xm, s, r.a[1], x1);
+  s = add_if(xp, s, r.a[2], x2);
+  s = add_if(ym, s, r.a[3], x3);
+  s = add_if(yp, s, r.a[4], x4);
+  s = add_if(zl, s, r.a[5], x5);
+  s = add_if(zh, s, r.a[6], x6);
   return b0 - s;
+}
+
+// Residual of one weighted-Jacobi sweep from zero: b - A (w D^-1 b) at (i, j, k) = row t.
+template <typename T, class M, typename TB = T>
+__device__ inline T down_point(const T *__restrict__ c, long long pitch, M zm, int nx, int ny, int nz,
+                               const T *__restrict__ dinv, const TB *__restrict__ b, T w, int i, int j, int k,
+                               long long t) {
+  return down_eval<T, TB>(down_loads<T, M, TB>(c, pitch, zm, nx, ny, nz, dinv, b, i, j, k, t), w, i, j, k, nx, ny,
+                          nz);
 }
 
 // Restriction to coarse point (i, j, k) of a level coarsened in direction d:
@@ -89,11 +151,21 @@ __device__ inline T down_restrict_point(const T *__restrict__ c, long long pitch
   const bool hp = pd + 1 < fnd;
   const int pi = hp ? fi + di : fi, pj = hp ? fj + dj : fj, pk = hp ? fk + dk : fk;
   const long long fp = hp ? f0 + stride : f0;
-  const T r0 = down_point<T, M, TB>(c, pitch, zm, nx, ny, nz, dinv, b, w, fi, fj, fk, f0);
-  const T rm = down_point<T, M, TB>(c, pitch, zm, nx, ny, nz, dinv, b, w, fi - di, fj - dj, fk - dk, f0 - stride);
-  const T rp = down_point<T, M, TB>(c, pitch, zm, nx, ny, nz, dinv, b, w, pi, pj, pk, fp);
-  const T v = r0 + wp[zm(f0 - stride, fk - dk)] * rm;
-  return add_if(hp, v, wm[zm(fp, pk)], rp);
+  // every load of the three fine points and the two weights issued before the first wait
+  DownRaw<T, TB> q0 = down_loads<T, M, TB>(c, pitch, zm, nx, ny, nz, dinv, b, fi, fj, fk, f0);
+  DownRaw<T, TB> qm = down_loads<T, M, TB>(c, pitch, zm, nx, ny, nz, dinv, b, fi - di, fj - dj, fk - dk, f0 - stride);
+  DownRaw<T, TB> qp = down_loads<T, M, TB>(c, pitch, zm, nx, ny, nz, dinv, b, pi, pj, pk, fp);
+  T wpm = wp[zm(f0 - stride, fk - dk)], wmp = wm[zm(fp, pk)];
+  hold_raw(q0);
+  hold_raw(qm);
+  hold_raw(qp);
+  hold(wpm);
+  hold(wmp);
+  const T r0 = down_eval<T, TB>(q0, w, fi, fj, fk, nx, ny, nz);
+  const T rm = down_eval<T, TB>(qm, w, fi - di, fj - dj, fk - dk, nx, ny, nz);
+  const T rp = down_eval<T, TB>(qp, w, pi, pj, pk, nx, ny, nz);
+  const T v = r0 + wpm * rm;
+  return add_if(hp, v, wmp, rp);
 }
 
 // What the interpolated value x of fine point (i, j, k) needs, the level coarsened
@@ -143,6 +215,64 @@ __device__ inline UpLoads<T> up_loads(const TB *b, const T *dinv, const T *wm, c
   }
   return r;
 }
+// The one-point form of up_loads (kSpec) in two parts: every operand loaded
+// unconditionally at a valid address (up_raw), then the selects (up_select), so that
+// the raw values can be held (hold) between the two: the same values as up_loads<kSpec>.
+template <typename T, typename TB = T>
+struct UpRaw {
+  TB b;
+  T dinv, wm, wp, y0, y1;
+  bool odd, lo, hi;
+  long long u;
+};
+template <typename T, int D, class M, typename TB = T>
+__device__ inline UpRaw<T, TB> up_raw(const TB *b, const T *dinv, const T *wm, const T *wp, const T *yc, M zm, int i,
+                                      int j, int k, int fnx, int fny, int cnx, int cny, int ncd) {
+  const long long t = ((long long)k * fny + j) * fnx + i, u = zm(t, k);
+  const int pd = (D == 0) ? i : (D == 1 ? j : k);
+  const long long cstride = (D == 0) ? 1 : (D == 1 ? (long long)cnx : (long long)cnx * cny);
+  const int q = (pd & 1) ? ((pd - 1) >> 1) : (pd >> 1);
+  const int ci = (D == 0) ? q : i, cj = (D == 1) ? q : j, ck = (D == 2) ? q : k;
+  const long long cr = ((long long)ck * cny + cj) * cnx + ci;
+  UpRaw<T, TB> r;
+  r.odd = pd & 1;
+  r.lo = !r.odd && q >= 1;
+  r.hi = !r.odd && q < ncd;
+  r.u = u;
+  // valid coarse rows: cr exists for C points and for q < ncd; q = ncd implies q >= 1
+  const long long c1 = (r.odd || r.hi) ? cr : cr - cstride;
+  const long long c0 = r.lo ? cr - cstride : c1;
+  r.b = b[t];
+  r.dinv = dinv[u];
+  r.wm = wm[u];
+  r.wp = wp[u];
+  r.y0 = yc[c0];
+  r.y1 = yc[c1];
+  return r;
+}
+template <typename T, typename TB>
+__device__ inline void hold_raw(UpRaw<T, TB> &r) {
+  hold(r.b);
+  hold(r.dinv);
+  hold(r.wm);
+  hold(r.wp);
+  hold(r.y0);
+  hold(r.y1);
+}
+template <typename T, typename TB>
+__device__ inline UpLoads<T> up_select(const UpRaw<T, TB> &r) {
+  UpLoads<T> o;
+  o.u = r.u;
+  o.b = (T)r.b;
+  o.dinv = r.dinv;
+  o.f = !r.odd;
+  o.wm = r.lo ? r.wm : T(0);
+  o.e0 = r.lo ? r.y0 : T(0);
+  o.wp = r.hi ? r.wp : T(0);
+  o.e1 = (r.odd || r.hi) ? r.y1 : T(0);
+  return o;
+}
+
 // x = w D^-1 b plus the interpolated coarse correction.
 template <typename T>
 __device__ inline T up_value(const UpLoads<T> &r, T w) {
@@ -165,23 +295,40 @@ __device__ inline T up_point(const T *__restrict__ c, long long pitch, M zm, int
                              const T *__restrict__ dinv, const TB *__restrict__ b, const T *__restrict__ wm,
                              const T *__restrict__ wp, const T *__restrict__ yc, int cnx, int cny, int ncd, T w, int i,
                              int j, int k, long long t) {
-#define FPC_X(ii, jj, kk) \
-  up_value<T>(up_loads<T, D, true, M, TB>(b, dinv, wm, wp, yc, zm, ii, jj, kk, nx, ny, cnx, cny, ncd), w)
+#define FPC_L(ii, jj, kk) up_raw<T, D, M, TB>(b, dinv, wm, wp, yc, zm, ii, jj, kk, nx, ny, cnx, cny, ncd)
   const bool xm = i > 0, xp = i < nx - 1, ym = j > 0, yp = j < ny - 1, zl = k > 0, zh = k < nz - 1;
   const long long u = zm(t, k);
-  const T xc = FPC_X(i, j, k);
-  const T x1 = FPC_X(xm ? i - 1 : i, j, k), x2 = FPC_X(xp ? i + 1 : i, j, k);
-  const T x3 = FPC_X(i, ym ? j - 1 : j, k), x4 = FPC_X(i, yp ? j + 1 : j, k);
-  const T x5 = FPC_X(i, j, zl ? k - 1 : k), x6 = FPC_X(i, j, zh ? k + 1 : k);
-#undef FPC_X
-  T s = c[u] * xc;
-  s = add_if(xm, s, c[pitch + u], x1);
-  s = add_if(xp, s, c[2 * pitch + u], x2);
-  s = add_if(ym, s, c[3 * pitch + u], x3);
-  s = add_if(yp, s, c[4 * pitch + u], x4);
-  s = add_if(zl, s, c[5 * pitch + u], x5);
-  s = add_if(zh, s, c[6 * pitch + u], x6);
-  return xc + w * ((T)b[t] - s) * dinv[u];
+  // every load of the point, its six neighbours and its row issued before the first wait
+  UpRaw<T, TB> r0 = FPC_L(i, j, k);
+  UpRaw<T, TB> r1 = FPC_L(xm ? i - 1 : i, j, k), r2 = FPC_L(xp ? i + 1 : i, j, k);
+  UpRaw<T, TB> r3 = FPC_L(i, ym ? j - 1 : j, k), r4 = FPC_L(i, yp ? j + 1 : j, k);
+  UpRaw<T, TB> r5 = FPC_L(i, j, zl ? k - 1 : k), r6 = FPC_L(i, j, zh ? k + 1 : k);
+#undef FPC_L
+  Coef7<T> a;
+  a.load(c, pitch, u);
+  hold_raw(r0);
+  hold_raw(r1);
+  hold_raw(r2);
+  hold_raw(r3);
+  hold_raw(r4);
+  hold_raw(r5);
+  hold_raw(r6);
+#pragma unroll
+  for (int q = 0; q < 7; ++q) hold(a.a[q]);
+  const UpLoads<T> lc = up_select(r0), l1 = up_select(r1), l2 = up_select(r2), l3 = up_select(r3);
+  const UpLoads<T> l4 = up_select(r4), l5 = up_select(r5), l6 = up_select(r6);
+  const T xc = up_value<T>(lc, w);
+  const T x1 = up_value<T>(l1, w), x2 = up_value<T>(l2, w);
+  const T x3 = up_value<T>(l3, w), x4 = up_value<T>(l4, w);
+  const T x5 = up_value<T>(l5, w), x6 = up_value<T>(l6, w);
+  T s = a.a[0] * xc;
+  s = add_if(xm, s, a.a[1], x1);
+  s = add_if(xp, s, a.a[2], x2);
+  s = add_if(ym, s, a.a[3], x3);
+  s = add_if(yp, s, a.a[4], x4);
+  s = add_if(zl, s, a.a[5], x5);
+  s = add_if(zh, s, a.a[6], x6);
+  return xc + w * (lc.b - s) * lc.dinv;
 }
 
 // Row r of the coarsest solve x = inv b (FP64 sums), one wavefront; valid in lane 0.

@@ -396,7 +396,7 @@ __global__ void k_prolong_add(int fnx, int fny, int fnz, int d, const T *__restr
 // coarse point (fine box nx x ny x nz, coarse cnx x cny x cnz).
 // TB: storage type of b (wider than T on the finest level of a reduced-precision cycle).
 template <typename T, typename TB = T>
-__global__ void k_down_restrict(const T *__restrict__ c, long long pitch, int nx, int ny, int nz,
+__global__ void __launch_bounds__(256) k_down_restrict(const T *__restrict__ c, long long pitch, int nx, int ny, int nz,
                                 const T *__restrict__ dinv, const TB *__restrict__ b, const T *__restrict__ wm,
                                 const T *__restrict__ wp, int d, T w, int cnx, int cny, T *__restrict__ bc,
                                 const int *skip) {
@@ -414,7 +414,7 @@ __global__ void k_down_restrict(const T *__restrict__ c, long long pitch, int nx
 // TB, TO: storage types of b and y (wider than T on the finest level of a
 // reduced-precision cycle, which then reads and writes the FP64 vectors directly).
 template <typename T, int D, typename TB = T, typename TO = T>
-__global__ void k_up_jacobi(const T *__restrict__ c, long long pitch, int nx, int ny, int nz,
+__global__ void __launch_bounds__(256) k_up_jacobi(const T *__restrict__ c, long long pitch, int nx, int ny, int nz,
                             const T *__restrict__ dinv, const TB *__restrict__ b, const T *__restrict__ wm,
                             const T *__restrict__ wp, const T *__restrict__ yc, int cnx, int cny, int ncd,
                             TO *__restrict__ y, T w, const int *skip) {
@@ -821,7 +821,7 @@ int fused_down(PfmgImpl *P, int l, const TB *b, T w, const int *skip, hipStream_
     return 0;
   }
   dim3 gc, bc;
-  launch_shape(C.d, gc, bc);
+  launch_shape(C.d, gc, bc, tile_tuning().mg_block);
   if (!tile_worthwhile(F.d)) {
     // small level: residual and restriction in one launch (the fine residual is not stored)
     FPC_PROF_BEGIN(cls);
@@ -880,7 +880,7 @@ int fused_up(PfmgImpl *P, int l, const TB *b, const T *yc, TO *y, T w, const int
 #undef FPC_UP_TILED
   } else {
     dim3 g, bl;
-    launch_shape(F.d, g, bl);
+    launch_shape(F.d, g, bl, tile_tuning().mg_block);
 #define FPC_UP_SIMPLE(D)                                                                                         \
   k_up_jacobi<T, D, TB, TO><<<g, bl, 0, st>>>((const T *)F.coef, F.pitch, F.d.nx, F.d.ny, F.d.nz,                  \
                                               (const T *)F.dinv, b, (const T *)F.wm, (const T *)F.wp, yc, C.d.nx, \

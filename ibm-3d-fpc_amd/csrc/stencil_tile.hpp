@@ -270,6 +270,7 @@ struct TileTuning {
   int mg_wide_finest = 1;  // FP32 V-cycles read and write the FP64 vectors on the finest level
   int p_flexible = 1;      // the time step's pressure GMRES keeps its preconditioned vectors
   int mg_graph = 0;        // V-cycles replayed as captured graphs
+  int mg_block = 64;       // workgroup size of the one-point multigrid kernels
 };
 inline TileTuning &tile_tuning() {
   static TileTuning t = [] {
@@ -286,6 +287,9 @@ inline TileTuning &tile_tuning() {
     x.p_flexible = e ? (std::atoi(e) != 0) : 1;
     e = std::getenv("FPC_MG_GRAPH");
     x.mg_graph = e ? (std::atoi(e) != 0) : 0;
+    e = std::getenv("FPC_MG_BLOCK");
+    x.mg_block = e ? std::atoi(e) : 64;
+    if (x.mg_block != 64 && x.mg_block != 128 && x.mg_block != 256) x.mg_block = 64;
     return x;
   }();
   return t;

@@ -889,6 +889,9 @@ int fpc_SetTuning(const char *key, long long value) {
   } else if (k == "mg_graph") {
     FPC_CHECK_ARG(value == 0 || value == 1, "mg_graph must be 0 or 1");
     t.mg_graph = (int)value;
+  } else if (k == "mg_block") {
+    FPC_CHECK_ARG(value == 64 || value == 128 || value == 256, "mg_block must be 64, 128 or 256");
+    t.mg_block = (int)value;
   } else {
     FPC_CHECK_ARG(false, "unknown tuning key");
   }
