@@ -98,12 +98,11 @@ int launch_arnoldi_matvec(const fpc_StructMatrix_s *A, const double *V, long lon
   return 0;
 }
 
-// Partial dots <p_j, w> for j = 0..cur, G vectors per pass over w.
-template <int G>
+// Partial dots <p_j, w> for j = 0..cur, eight vectors per pass over w.
 __global__ void __launch_bounds__(256) k_multidot(const double *__restrict__ V, long long vp, long long n, State *S,
                                                   double *__restrict__ part) {
   if (S->st.done) return;
-  multidot_body<G>(V, vp, n, S, part, blockIdx.x, gridDim.x);
+  multidot_body(V, vp, n, S, part, blockIdx.x, gridDim.x);
 }
 
 // h_{j,cur} = sigma_j * sum_b part[j][b]; one block per j.
@@ -494,10 +493,7 @@ static int arnoldi_tail(GmresImpl *G, int step) {
   {
     // bytes per row: the step+1 basis vectors once, w once per group of eight
     FPC_PROF_BEGIN("multidot");
-    if (tile_tuning().md_group == 16)
-      k_multidot<16><<<G->nred, 256, 0, stream()>>>(G->V, G->vp, G->N, G->S, G->part);
-    else
-      k_multidot<8><<<G->nred, 256, 0, stream()>>>(G->V, G->vp, G->N, G->S, G->part);
+    k_multidot<<<G->nred, 256, 0, stream()>>>(G->V, G->vp, G->N, G->S, G->part);
     FPC_PROF_END("multidot", (double)G->N * 8.0 * (nv + std::ceil(nv / 8.0)));
   }
   FPC_LAUNCH_CHECK();

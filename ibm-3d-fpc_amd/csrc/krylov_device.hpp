@@ -77,27 +77,24 @@ __device__ inline double arnoldi_row(const double *__restrict__ c, long long pit
 }
 
 // Partial dots <p_j, w> for j = 0..cur, eight vectors per pass over w; block bid of nblk.
-// G basis vectors per pass over w (8 or 16): w is read ceil((cur + 1) / G) times; each
-// sum runs over the points in the same order whatever the grouping, so G changes no bit.
-template <int G = 8>
 __device__ inline void multidot_body(const double *__restrict__ V, long long vp, long long n, State *S,
                                      double *__restrict__ part, int bid, int nblk) {
-  __shared__ double lds[G * 4];
+  __shared__ double lds[8 * 4];
   int cur = S->st.i;
   const double *w = V + (long long)(cur + 1) * vp;
   int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (int j0 = 0; j0 <= cur; j0 += G) {
-    int nj = min(G, cur + 1 - j0);
-    double acc[G];
+  for (int j0 = 0; j0 <= cur; j0 += 8) {
+    int nj = min(8, cur + 1 - j0);
+    double acc[8];
 #pragma unroll
-    for (int q = 0; q < G; ++q) acc[q] = 0.0;
+    for (int q = 0; q < 8; ++q) acc[q] = 0.0;
     // two points per trip: twice the loads in flight per thread
     const long long stride = (long long)nblk * blockDim.x;
     long long t = bid * (long long)blockDim.x + threadIdx.x;
     for (; t + stride < n; t += 2 * stride) {
       const double wa = w[t], wb = w[t + stride];
 #pragma unroll
-      for (int q = 0; q < G; ++q)
+      for (int q = 0; q < 8; ++q)
         if (q < nj) {
           const double *vq = V + (long long)(j0 + q) * vp;
           acc[q] += vq[t] * wa;
@@ -107,11 +104,11 @@ __device__ inline void multidot_body(const double *__restrict__ V, long long vp,
     if (t < n) {
       const double wa = w[t];
 #pragma unroll
-      for (int q = 0; q < G; ++q)
+      for (int q = 0; q < 8; ++q)
         if (q < nj) acc[q] += V[(long long)(j0 + q) * vp + t] * wa;
     }
 #pragma unroll
-    for (int q = 0; q < G; ++q) {
+    for (int q = 0; q < 8; ++q) {
       double v = acc[q];
       for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
       acc[q] = v;
@@ -119,7 +116,7 @@ __device__ inline void multidot_body(const double *__restrict__ V, long long vp,
     __syncthreads();
     if (lane == 0)
 #pragma unroll
-      for (int q = 0; q < G; ++q) lds[q * 4 + wid] = acc[q];
+      for (int q = 0; q < 8; ++q) lds[q * 4 + wid] = acc[q];
     __syncthreads();
     if (threadIdx.x < nj) {
       int q = threadIdx.x;

@@ -225,47 +225,29 @@ struct UpRaw {
   bool odd, lo, hi;
   long long u;
 };
-// Coarse rows of the correction that fine point (i, j, k) reads, the level coarsened in
-// direction D: c1 (C points, and the right coarse neighbour of F points) and c0 (the left
-// one), both valid rows whether or not the point uses them; odd: a C point; lo / hi: an F
-// point with a left / right coarse neighbour.
-struct UpCoarse {
-  long long c0, c1;
-  bool odd, lo, hi;
-};
-template <int D>
-__device__ inline UpCoarse up_coarse(int i, int j, int k, int cnx, int cny, int ncd) {
+template <typename T, int D, class M, typename TB = T>
+__device__ inline UpRaw<T, TB> up_raw(const TB *b, const T *dinv, const T *wm, const T *wp, const T *yc, M zm, int i,
+                                      int j, int k, int fnx, int fny, int cnx, int cny, int ncd) {
+  const long long t = ((long long)k * fny + j) * fnx + i, u = zm(t, k);
   const int pd = (D == 0) ? i : (D == 1 ? j : k);
   const long long cstride = (D == 0) ? 1 : (D == 1 ? (long long)cnx : (long long)cnx * cny);
   const int q = (pd & 1) ? ((pd - 1) >> 1) : (pd >> 1);
   const int ci = (D == 0) ? q : i, cj = (D == 1) ? q : j, ck = (D == 2) ? q : k;
   const long long cr = ((long long)ck * cny + cj) * cnx + ci;
-  UpCoarse r;
+  UpRaw<T, TB> r;
   r.odd = pd & 1;
   r.lo = !r.odd && q >= 1;
   r.hi = !r.odd && q < ncd;
-  // cr exists for C points and for q < ncd; q = ncd implies q >= 1
-  r.c1 = (r.odd || r.hi) ? cr : cr - cstride;
-  r.c0 = r.lo ? cr - cstride : r.c1;
-  return r;
-}
-
-template <typename T, int D, class M, typename TB = T>
-__device__ inline UpRaw<T, TB> up_raw(const TB *b, const T *dinv, const T *wm, const T *wp, const T *yc, M zm, int i,
-                                      int j, int k, int fnx, int fny, int cnx, int cny, int ncd) {
-  const long long t = ((long long)k * fny + j) * fnx + i, u = zm(t, k);
-  const UpCoarse q = up_coarse<D>(i, j, k, cnx, cny, ncd);
-  UpRaw<T, TB> r;
-  r.odd = q.odd;
-  r.lo = q.lo;
-  r.hi = q.hi;
   r.u = u;
+  // valid coarse rows: cr exists for C points and for q < ncd; q = ncd implies q >= 1
+  const long long c1 = (r.odd || r.hi) ? cr : cr - cstride;
+  const long long c0 = r.lo ? cr - cstride : c1;
   r.b = b[t];
   r.dinv = dinv[u];
   r.wm = wm[u];
   r.wp = wp[u];
-  r.y0 = yc[q.c0];
-  r.y1 = yc[q.c1];
+  r.y0 = yc[c0];
+  r.y1 = yc[c1];
   return r;
 }
 template <typename T, typename TB>
@@ -479,53 +461,6 @@ struct UpOp {
   __device__ T finish(const Raw &r) const { return up_value<T>(r, w); }
   __device__ void out(const Coef7<T> &a, const Raw &own, int i, int j, int k, const Nbr<T> &nb) {
     y[at(i, j, k)] = (TO)(nb.c + w * (own.b - a.apply(nb)) * own.dinv);
-  }
-};
-
-// UpOp on march_tile_lean: a raw is b and the two coarse values (c0, c1 of up_coarse,
-// always read); 1/diag and the weights are row values.  The values are UpOp's (the
-// interpolation assembled as up_select does, the sweep as UpOp::out), bit for bit.
-template <typename T, int D, typename TB = T, typename TO = T>
-struct UpLeanOp {
-  static constexpr int kAhead = 2;  // prefetch distance of march_tile_lean
-  const T *c;
-  long long pitch;
-  int nx, ny;
-  const T *dinv;
-  const TB *b;
-  const T *wm, *wp, *yc;
-  int cnx, cny, ncd;
-  TO *y;
-  T w;
-  ZMap zm;  // rows of c, dinv, wm and wp
-  struct Raw {
-    TB b;
-    T y0, y1;
-  };
-  struct Cst {
-    T dinv, wm, wp;
-  };
-  __device__ long long at(int i, int j, int k) const { return ((long long)k * ny + j) * nx + i; }
-  __device__ Raw fetch(int i, int j, int k) const {
-    const UpCoarse q = up_coarse<D>(i, j, k, cnx, cny, ncd);
-    return Raw{b[at(i, j, k)], yc[q.c0], yc[q.c1]};
-  }
-  __device__ Cst consts(long long u) const { return Cst{dinv[u], wm[u], wp[u]}; }
-  __device__ T finish(const Raw &r, const Cst &q, int i, int j, int k) const {
-    const UpCoarse p = up_coarse<D>(i, j, k, cnx, cny, ncd);
-    UpLoads<T> o;
-    o.u = 0;
-    o.b = (T)r.b;
-    o.dinv = q.dinv;
-    o.f = !p.odd;
-    o.wm = p.lo ? q.wm : T(0);
-    o.e0 = p.lo ? r.y0 : T(0);
-    o.wp = p.hi ? q.wp : T(0);
-    o.e1 = (p.odd || p.hi) ? r.y1 : T(0);
-    return up_value<T>(o, w);
-  }
-  __device__ void out(const Coef7<T> &a, const Raw &own, const Cst &q, int i, int j, int k, const Nbr<T> &nb) {
-    y[at(i, j, k)] = (TO)(nb.c + w * ((T)own.b - a.apply(nb)) * q.dinv);
   }
 };
 

@@ -473,18 +473,6 @@ __global__ void __launch_bounds__(kTileX *TY)
   march_tile<T, kTileX, TY>(op, nx, ny, nz, k0, min(k0 + kz, nz), blockIdx.x, blockIdx.y);
 }
 
-// The same up-sweep on march_tile_lean (levels with a z-extruded hot copy).
-template <typename T, int D, int TY, typename TB = T, typename TO = T>
-__global__ void __launch_bounds__(kTileX *TY)
-    k_up_lean(const T *__restrict__ c, long long pitch, ZMap zm, int nx, int ny, int nz, const T *__restrict__ dinv,
-              const TB *__restrict__ b, const T *__restrict__ wm, const T *__restrict__ wp, const T *__restrict__ yc,
-              int cnx, int cny, int ncd, TO *__restrict__ y, T w, int kz, const int *skip) {
-  if (skip && *skip) return;
-  mgd::UpLeanOp<T, D, TB, TO> op{c, pitch, nx, ny, dinv, b, wm, wp, yc, cnx, cny, ncd, y, w, zm};
-  const int k0 = blockIdx.z * kz;
-  march_tile_lean<T, kTileX, TY>(op, nx, ny, nz, k0, min(k0 + kz, nz), blockIdx.x, blockIdx.y);
-}
-
 // Coarsest level: x = inv b (FP64 inverse, FP64 sums), one wavefront per row.
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -881,24 +869,7 @@ int fused_up(PfmgImpl *P, int l, const TB *b, const T *yc, TO *y, T w, const int
   const int ncd = (F.cdir == 0) ? C.d.nx : (F.cdir == 1 ? C.d.ny : C.d.nz);
   const T *hc = H.c, *hd = H.d;
   const T *wm = H.wm, *wp = H.wp;
-  if (tile_worthwhile(F.d) && H.zm.pl && tile_tuning().mg_lean) {
-    // z-extruded level: the lean march (row values held in registers, deeper prefetch)
-#define FPC_UP_LEAN(D, TY)                                                                                            \
-  k_up_lean<T, D, TY, TB, TO><<<tl.grid, tl.block, 0, st>>>(hc, H.pitch, H.zm, F.d.nx, F.d.ny, F.d.nz, hd, b, wm, wp, \
-                                                           yc, C.d.nx, C.d.ny, ncd, y, w, tl.kz, skip)
-    if (tile_tuning().up_ty == 4) {
-      TileLaunch tl = tile_launch<4>(F.d);
-      if (F.cdir == 0) FPC_UP_LEAN(0, 4);
-      else if (F.cdir == 1) FPC_UP_LEAN(1, 4);
-      else FPC_UP_LEAN(2, 4);
-    } else {
-      TileLaunch tl = tile_launch<8>(F.d);
-      if (F.cdir == 0) FPC_UP_LEAN(0, 8);
-      else if (F.cdir == 1) FPC_UP_LEAN(1, 8);
-      else FPC_UP_LEAN(2, 8);
-    }
-#undef FPC_UP_LEAN
-  } else if (tile_worthwhile(F.d)) {
+  if (tile_worthwhile(F.d)) {
     TileLaunch tl = tile_launch<8>(F.d);
 #define FPC_UP_TILED(D)                                                                                               \
   k_up_tiled<T, D, 8, TB, TO><<<tl.grid, tl.block, 0, st>>>(hc, H.pitch, H.zm, F.d.nx, F.d.ny, F.d.nz, hd, b, wm, wp, \

@@ -255,108 +255,6 @@ __device__ inline void march_tile_pipelined(Op &op, int nx, int ny, int nz, int 
     if (kb + s < kend) plane_step(s, kb + s);
 }
 
-// The pipelined walk for ops whose per-point data splits into what changes from plane
-// to plane (Op::Raw: vector values, fetched Op::kAhead + 1 planes ahead for the own
-// column and Op::kAhead for the halo ring) and what belongs to a row of the stored
-// planes (Op::Cst: 1/diag, interpolation weights, ... at row u, op.consts(u)).  On a
-// z-extruded hot copy every interior plane of a column reads one row, so the row
-// values stay in registers with the coefficients and are loaded again only where the
-// row changes (wall planes, flagged points); the raws in flight are then a third or
-// less of the full ones, and the prefetch distance can be longer at the same
-// occupancy.  A reload is issued before the plane's prefetches, so its wait covers
-// only loads the plane needs anyway and the prefetches stay in flight.  Op interface:
-// fetch(i, j, k) -> Raw; consts(u) -> Cst; finish(raw, cst, i, j, k) -> the value the
-// neighbours see; out(a, own, cst, i, j, k, nb).  As march_tile_pipelined, every fetch
-// is unconditional at a valid point, and the values are those of march_tile.
-template <typename T, int TX, int TY, class Op>
-__device__ inline void march_tile_lean(Op &op, int nx, int ny, int nz, int k0, int kend, int bx, int by) {
-  using Raw = typename Op::Raw;
-  using Cst = typename Op::Cst;
-  constexpr int D = Op::kAhead;
-  static_assert(D == 1 || D == 2, "prefetch distance 1 or 2");
-  constexpr int P = D + 2;
-  __shared__ T plane[2][TY + 2][TX + 2];
-  const int tx = threadIdx.x, ty = threadIdx.y;
-  const int i = bx * TX + tx, j = by * TY + ty;
-  const bool in = (i < nx) && (j < ny);
-  const int ic = i < nx ? i : nx - 1, jc = j < ny ? j : ny - 1;
-  int hx = 0, hy = 0;
-  const bool ring = tile_ring<TX, TY>(ty * TX + tx, hx, hy);
-  const int hi = bx * TX + hx, hj = by * TY + hy;
-  const bool ring_in = ring && hi >= 0 && hi < nx && hj >= 0 && hj < ny;
-  const int hic = hi < 0 ? 0 : (hi < nx ? hi : nx - 1), hjc = hj < 0 ? 0 : (hj < ny ? hj : ny - 1);
-  // flag words of the own and ring columns (march_tile_pipelined)
-  const int w0 = (k0 > 0 ? k0 - 1 : 0) >> 6, w1 = (w0 + 1) * 64 < nz ? w0 + 1 : w0;
-  const long long pl = (long long)nx * ny;
-  const long long co = (long long)jc * nx + ic, cr = (long long)hjc * nx + hic;
-  const unsigned long long o0 = op.zm.word(co, w0), o1 = op.zm.word(co, w1);
-  const unsigned long long r0 = op.zm.word(cr, w0), r1 = op.zm.word(cr, w1);
-  pin_all(o0, o1, r0, r1);
-  auto fown = [&](int kk) -> int { return (int)((((kk >> 6) == w0 ? o0 : o1) >> (kk & 63)) & 1ull); };
-  auto fring = [&](int kk) -> int { return (int)((((kk >> 6) == w0 ? r0 : r1) >> (kk & 63)) & 1ull); };
-  auto rown = [&](int kk) -> long long { return op.zm.row(co + (long long)kk * pl, kk, fown(kk)); };
-  auto rring = [&](int kk) -> long long { return op.zm.row(cr + (long long)kk * pl, kk, fring(kk)); };
-  const int ko_last = kend < nz ? kend : nz - 1, kr_last = kend - 1;
-  auto fo = [&](int kk) -> Raw { return op.fetch(ic, jc, kk < ko_last ? kk : ko_last); };
-  auto fr = [&](int kk) -> Raw { return op.fetch(hic, hjc, kk < kr_last ? kk : kr_last); };
-  const int kb0 = k0 > 0 ? k0 - 1 : 0;
-  // row values of the plane below, the first plane and the first ring plane
-  const Cst cb = op.consts(rown(kb0));
-  long long nu = rown(k0), ru = rring(k0);
-  Cst cn = op.consts(nu), rc = op.consts(ru);
-  const Raw below = fo(kb0);
-  Raw own[P], rq[P];
-#pragma unroll
-  for (int q = 0; q <= D; ++q) own[q] = fo(k0 + q);
-#pragma unroll
-  for (int q = 0; q < D; ++q) rq[q] = fr(k0 + q);
-  T zm = (in && k0 > 0) ? op.finish(below, cb, ic, jc, kb0) : T(0);
-  T zc = in ? op.finish(own[0], cn, ic, jc, k0) : T(0);
-  Cst cc = cn;  // row values of the plane being written
-  Coef7<T> a;
-  long long arow = -1;
-  auto plane_step = [&](const int s, const int k) {
-    // row changes of this plane (coefficients), the next own plane and this ring plane
-    if (in && nu != arow) {
-      a.load(op.c, op.pitch, nu);
-      arow = nu;
-    }
-    const int k1 = k + 1 < nz ? k + 1 : k;
-    const long long u1 = rown(k1), urk = rring(k);
-    if (u1 != nu) {
-      cn = op.consts(u1);
-      nu = u1;
-    }
-    if (urk != ru) {
-      rc = op.consts(urk);
-      ru = urk;
-    }
-    own[(s + D + 1) % P] = fo(k + D + 1);
-    rq[(s + D) % P] = fr(k + D);
-    const T zp = (in && k + 1 < nz) ? op.finish(own[(s + 1) % P], cn, ic, jc, k + 1) : T(0);
-    T(*Pl)[TX + 2] = plane[(k - k0) & 1];
-    Pl[ty + 1][tx + 1] = zc;
-    if (ring) Pl[hy + 1][hx + 1] = ring_in ? op.finish(rq[s], rc, hic, hjc, k) : T(0);
-    __syncthreads();
-    if (in) {
-      Nbr<T> nb{zc, Pl[ty + 1][tx], Pl[ty + 1][tx + 2], Pl[ty][tx + 1], Pl[ty + 2][tx + 1], zm, zp,
-                nbr_in_box(i, j, k, nx, ny, nz)};
-      op.out(a, own[s], cc, i, j, k, nb);
-    }
-    cc = cn;
-    zm = zc;
-    zc = zp;
-  };
-  int kb = k0;
-  for (; kb + P <= kend; kb += P) {
-#pragma unroll
-    for (int s = 0; s < P; ++s) plane_step(s, kb + s);
-  }
-#pragma unroll
-  for (int s = 0; s < P - 1; ++s)
-    if (kb + s < kend) plane_step(s, kb + s);
-}
-
 // Launch geometry of a z-marching kernel: tile rows TY and z-chunk length.
 struct TileLaunch {
   dim3 grid, block;
@@ -373,9 +271,6 @@ struct TileTuning {
   int p_flexible = 1;      // the time step's pressure GMRES keeps its preconditioned vectors
   int mg_graph = 0;        // V-cycles replayed as captured graphs
   int mg_block = 64;       // workgroup size of the one-point multigrid kernels
-  int mg_lean = 0;         // up-sweeps of z-extruded levels on march_tile_lean
-  int up_ty = 8;           // tile rows of those up-sweeps (4 or 8)
-  int md_group = 8;        // basis vectors per pass of the Gram-Schmidt dot products (8 or 16)
 };
 inline TileTuning &tile_tuning() {
   static TileTuning t = [] {
@@ -395,12 +290,6 @@ inline TileTuning &tile_tuning() {
     e = std::getenv("FPC_MG_BLOCK");
     x.mg_block = e ? std::atoi(e) : 64;
     if (x.mg_block != 64 && x.mg_block != 128 && x.mg_block != 256) x.mg_block = 64;
-    e = std::getenv("FPC_MG_LEAN");
-    x.mg_lean = e ? (std::atoi(e) != 0) : 0;
-    e = std::getenv("FPC_UP_TY");
-    x.up_ty = (e && std::atoi(e) == 4) ? 4 : 8;
-    e = std::getenv("FPC_MD_GROUP");
-    x.md_group = (e && std::atoi(e) == 16) ? 16 : 8;
     return x;
   }();
   return t;

@@ -892,15 +892,6 @@ int fpc_SetTuning(const char *key, long long value) {
   } else if (k == "mg_block") {
     FPC_CHECK_ARG(value == 64 || value == 128 || value == 256, "mg_block must be 64, 128 or 256");
     t.mg_block = (int)value;
-  } else if (k == "mg_lean") {
-    FPC_CHECK_ARG(value == 0 || value == 1, "mg_lean must be 0 or 1");
-    t.mg_lean = (int)value;
-  } else if (k == "up_ty") {
-    FPC_CHECK_ARG(value == 4 || value == 8, "up_ty must be 4 or 8");
-    t.up_ty = (int)value;
-  } else if (k == "md_group") {
-    FPC_CHECK_ARG(value == 8 || value == 16, "md_group must be 8 or 16");
-    t.md_group = (int)value;
   } else {
     FPC_CHECK_ARG(false, "unknown tuning key");
   }

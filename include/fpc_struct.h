@@ -226,12 +226,8 @@ int fpc_DeviceSynchronize(void);
  * update x += M^-1 (sum y_j v_j) with one more V-cycle (FPC_P_FLEXIBLE); the same
  * iterates, the final update differing in rounding; applies to simulations created
  * afterwards; "mg_graph": 1 = V-cycles replayed as captured HIP graphs (default 0,
- * FPC_MG_GRAPH); "mg_block": workgroup size of the one-point multigrid kernels, 64
- * (default), 128 or 256 (FPC_MG_BLOCK); "mg_lean": 1 = up-sweeps of z-extruded levels
- * on the lean march (row values held in registers, two planes of prefetch; default 0,
- * measured slower, FPC_MG_LEAN); "up_ty": its tile rows, 4 or 8 (FPC_UP_TY);
- * "md_group": basis vectors per pass of the Gram-Schmidt dot products, 8 (default)
- * or 16 (measured slower, FPC_MD_GROUP).  All of these give identical results. */
+ * FPC_MG_GRAPH), identical results; "mg_block": workgroup size of the one-point
+ * multigrid kernels, 64 (default), 128 or 256 (FPC_MG_BLOCK), identical results. */
 int fpc_SetTuning(const char *key, long long value);
 
 /* ---- kernel timing (no reference counterpart) ----

@@ -36,8 +36,6 @@ def restore_tuning(torch_cuda):
     F.set_tuning("tile_min_points", 1 << 19)
     F.set_tuning("stencil_kz", 0)
     F.set_tuning("mg_wide_finest", 1)
-    F.set_tuning("mg_lean", 0)
-    F.set_tuning("up_ty", 8)
 
 
 def _extruded(nx, ny, nz, scale, seed, flagged=0.0):
@@ -126,31 +124,6 @@ def test_extruded_matches_oracle_and_full_storage(dims, scale, cdir, tiles, flag
         assert bits_equal(ext[key], full[key]), key
     for key in ("itp", "itj", "itj32"):
         assert ext[key] == full[key], key
-
-
-@pytest.mark.parametrize("up_ty", [4, 8])
-@pytest.mark.parametrize("dims,scale,cdir", CASES, ids=IDS)
-def test_lean_up_sweep_equals_original_march(dims, scale, cdir, up_ty):
-    """Up-sweeps of z-extruded levels on the lean march (1/diag and weights held per
-    row, b and the coarse values fetched two planes ahead; tile rows 4 and 8) against
-    the original march: one V-cycle, FP64 and FP32, and a GMRES + PFMG solve, the same
-    bits, with flagged points and every first coarsening direction."""
-    nx, ny, nz = dims
-    F.set_tuning("tile_min_points", 0)
-    vals = _extruded(nx, ny, nz, scale, seed=nz + 17, flagged=0.003)
-    b = np.random.default_rng(nz + 7).standard_normal(nx * ny * nz)
-    out = []
-    for lean in (1, 0):
-        F.set_tuning("mg_lean", lean)
-        F.set_tuning("up_ty", up_ty)
-        box = F.StructBox(nx, ny, nz, vals)
-        assert box.extruded()[0] >= 1
-        x, its, _, rc = F.Solver(box, tol=1e-11, maxit=300, kdim=30, precond="pfmg").solve(b)
-        assert rc == 0
-        out.append((F.pfmg_apply(box, b), F.pfmg_apply(box, b, mg_bits=32), x, its))
-    for q in range(3):
-        assert bits_equal(out[0][q], out[1][q]), q
-    assert out[0][3] == out[1][3]
 
 
 @pytest.mark.parametrize("kz", [6, 10, 100])

@@ -113,24 +113,3 @@ def test_tiled_equal_one_point_bitwise(torch_cuda, dims, scale, cdir):
     assert bits_equal(t[0], o[0])
     assert t[2] == o[2] and bits_equal(t[1], o[1])
     assert t[4] == o[4] and bits_equal(t[3], o[3])
-
-
-def test_multidot_grouping_changes_no_bit():
-    """The Gram-Schmidt dot products read the basis 8 or 16 vectors per pass over w;
-    every sum runs over the points in the same order either way: a Jacobi-GMRES(30)
-    solve whose cycles grow past 16 vectors gives the same iterates and count."""
-    nx, ny, nz = 64, 24, 24
-    vals = _aniso(nx, ny, nz, (1.0, 1.0, 4.0), 23)
-    b = np.random.default_rng(24).standard_normal(nx * ny * nz)
-    out = []
-    try:
-        for g in (8, 16):
-            F.set_tuning("md_group", g)
-            x, its, _, rc = F.Solver(F.StructBox(nx, ny, nz, vals), tol=1e-11, maxit=400, kdim=30,
-                                     precond="jacobi").solve(b)
-            assert rc == 0
-            out.append((x, its))
-    finally:
-        F.set_tuning("md_group", 8)
-    assert out[0][1] > 16, out[0][1]
-    assert out[0][1] == out[1][1] and bits_equal(out[0][0], out[1][0])
