@@ -57,6 +57,17 @@ def test_bad_arguments_return_arg_error():
     assert L.fpc_ClearError(F.ERROR_ARG) == 0
 
 
+def test_tuning_keys_reject_bad_values():
+    L = F.lib()
+    for key, bad, good in (("mg_block", 100, 64), ("mg_block", 512, 256), ("p_flexible", 2, 1), ("mg_graph", -1, 0),
+                           ("stencil_kz", -1, 0)):
+        assert L.fpc_SetTuning(key.encode(), bad) == F.ERROR_ARG, (key, bad)
+        assert L.fpc_ClearError(F.ERROR_ARG) == 0
+        assert L.fpc_SetTuning(key.encode(), good) == 0, (key, good)
+    assert L.fpc_SetTuning(b"no_such_key", 1) == F.ERROR_ARG
+    assert L.fpc_ClearError(F.ERROR_ARG) == 0
+
+
 def test_setup_runs_on_the_host_without_a_gpu():
     S = F.Setup(F.default_params())
     assert S.count("Np") > 0

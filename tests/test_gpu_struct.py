@@ -207,3 +207,25 @@ def test_maxit_returns_conv_flag(torch_cuda):
     x, its, rel, rc = sol.solve(b)
     assert rc == F.ERROR_CONV and its == 2 and rel > 1e-14
 
+
+
+def test_one_point_workgroup_size_changes_no_bit(torch_cuda):
+    """The one-point multigrid kernels of the small levels on workgroups of 64, 128 or
+    256 threads ("mg_block"): one V-cycle and a GMRES + PFMG solve of the pressure
+    operator give the same bits."""
+    kw = dict(SMALL)
+    S, dims, vals = _box(kw, "p")
+    b = np.random.default_rng(71).standard_normal(int(np.prod(dims)))
+    out = []
+    try:
+        for blk in (64, 128, 256):
+            F.set_tuning("mg_block", blk)
+            box = F.StructBox(*dims, vals)
+            y = F.pfmg_apply(box, b)
+            x, its, _, rc = F.Solver(box, tol=1e-10, maxit=200, kdim=30, precond="pfmg").solve(b)
+            assert rc == 0
+            out.append((y, x, its))
+    finally:
+        F.set_tuning("mg_block", 64)
+    for y, x, its in out[1:]:
+        assert bits_equal(y, out[0][0]) and bits_equal(x, out[0][1]) and its == out[0][2]
