@@ -88,24 +88,28 @@ __device__ inline void multidot_body(const double *__restrict__ V, long long vp,
     double acc[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) acc[q] = 0.0;
-    // two points per trip: twice the loads in flight per thread
+    // The walk runs from the vectors' ends to their starts, two points per trip (twice the
+    // loads in flight per thread): the orthogonalisation pass that follows walks from the
+    // starts, where these reads leave the vectors in the last-level cache.
     const long long stride = (long long)nblk * blockDim.x;
-    long long t = bid * (long long)blockDim.x + threadIdx.x;
-    for (; t + stride < n; t += 2 * stride) {
-      const double wa = w[t], wb = w[t + stride];
+    const long long t0 = bid * (long long)blockDim.x + threadIdx.x;
+    long long m = (t0 < n) ? (n - 1 - t0) / stride : -1;  // index of this thread's last point
+    for (; m >= 1; m -= 2) {
+      const long long ta = t0 + m * stride, tb = ta - stride;
+      const double wa = w[ta], wb = w[tb];
 #pragma unroll
       for (int q = 0; q < 8; ++q)
         if (q < nj) {
           const double *vq = V + (long long)(j0 + q) * vp;
-          acc[q] += vq[t] * wa;
-          acc[q] += vq[t + stride] * wb;
+          acc[q] += vq[ta] * wa;
+          acc[q] += vq[tb] * wb;
         }
     }
-    if (t < n) {
-      const double wa = w[t];
+    if (m == 0) {
+      const double wa = w[t0];
 #pragma unroll
       for (int q = 0; q < 8; ++q)
-        if (q < nj) acc[q] += V[(long long)(j0 + q) * vp + t] * wa;
+        if (q < nj) acc[q] += V[(long long)(j0 + q) * vp + t0] * wa;
     }
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
