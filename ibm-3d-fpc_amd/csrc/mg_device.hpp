@@ -273,19 +273,20 @@ __device__ inline UpLoads<T> up_select(const UpRaw<T, TB> &r) {
   return o;
 }
 
-// x = w D^-1 b plus the interpolated coarse correction.
+// x = w D^-1 b plus the interpolated coarse correction: (0 + wm e0) + wp e1 at F
+// points, e1 at C points.  The multiply-adds are written out as fused operations, so
+// every kernel that evaluates this rounds it the same way, whatever contractions the
+// code around it would let the compiler choose.
 template <typename T>
 __device__ inline T up_value(const UpLoads<T> &r, T w) {
-  T x = w * r.b * r.dinv;
-  if (r.f) {
-    T s = (T)0;
-    s += r.wm * r.e0;
-    s += r.wp * r.e1;
-    x += s;
-  } else {
-    x += r.e1;
-  }
-  return x;
+  const T s = fma(r.wp, r.e1, fma(r.wm, r.e0, T(0)));
+  return fma(w * r.b, r.dinv, r.f ? s : r.e1);
+}
+
+// The Jacobi sweep's output at a point: x + w (b - (A x)) / diag, as one fused step.
+template <typename T>
+__device__ inline T up_out(T x, T b, T ax, T dinv, T w) {
+  return fma(w * (b - ax), dinv, x);
 }
 
 // Prolongation + one weighted-Jacobi sweep at (i, j, k) = row t:
@@ -328,7 +329,7 @@ __device__ inline T up_point(const T *__restrict__ c, long long pitch, M zm, int
   s = add_if(yp, s, a.a[4], x4);
   s = add_if(zl, s, a.a[5], x5);
   s = add_if(zh, s, a.a[6], x6);
-  return xc + w * (lc.b - s) * lc.dinv;
+  return up_out<T>(xc, lc.b, s, lc.dinv, w);
 }
 
 // Row r of the coarsest solve x = inv b (FP64 sums), one wavefront; valid in lane 0.
@@ -460,7 +461,7 @@ struct UpOp {
   }
   __device__ T finish(const Raw &r) const { return up_value<T>(r, w); }
   __device__ void out(const Coef7<T> &a, const Raw &own, int i, int j, int k, const Nbr<T> &nb) {
-    y[at(i, j, k)] = (TO)(nb.c + w * (own.b - a.apply(nb)) * own.dinv);
+    y[at(i, j, k)] = (TO)up_out<T>(nb.c, own.b, a.apply(nb), own.dinv, w);
   }
 };
 
