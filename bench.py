@@ -10,9 +10,9 @@ same steps and reported beside it.
 
 roofline: the kernel class with the largest device time (found on the last
 warmup step with every class timed) is timed over the whole timed region with
-HIP events on the library's stream, on every 8th launch (a step has ~37 launches
+HIP events on the library's stream, on every 32nd launch (a step has ~37 launches
 of the Gram-Schmidt classes, so the sample cycles through the basis lengths, at
-an eighth of the event cost); achieved = its algorithmic bytes per launch / its
+a small fraction of the event cost); achieved = its algorithmic bytes per launch / its
 mean launch duration, against the 8 TB/s HBM3E peak.  Launches
 enqueued ahead of a converged GMRES cycle return at once and are not counted.
 traffic = HBM bytes per launch read from profiles/hbm_traffic.txt (FETCH_SIZE +
@@ -48,7 +48,7 @@ WORKLOAD = dict(nx=512, ny=96, nz=96, Lx=2.5, Ly=0.41, Lz=0.41, xcirc=0.5, ycirc
 
 
 HBM_PEAK_GBS = 8000.0
-ROOFLINE_SAMPLE = 8  # the roofline class's launches timed: every 8th
+ROOFLINE_SAMPLE = 32  # the roofline class's launches timed: every 32nd
 # kernel classes whose every launch covers one fixed box (a roofline figure per
 # launch is meaningful); the other MG classes mix the coarse levels
 ROOFLINE_CLASSES = ("spmv", "spmv_jacobi", "mg_down0", "mg_up0", "multidot", "orth_update", "explicit",
@@ -98,7 +98,7 @@ def _its_per_step(sim, tot0, steps):
     return [[(b - a) / steps for a, b in zip(tot0, tot)]]
 
 
-def run_sim(F, torch, kw, warmup, steps, roofline_timing=True):
+def run_sim(F, torch, kw, warmup, steps, roofline_timing=True, sample=ROOFLINE_SAMPLE):
     """W untimed steps (the last one with every kernel class timed), then K timed steps
     with the dominant class timed by HIP events."""
     sim = F.Sim(F.default_params(**kw))
@@ -120,9 +120,11 @@ def run_sim(F, torch, kw, warmup, steps, roofline_timing=True):
     elig = [k for k in classes if k in ROOFLINE_CLASSES]
     dom = max(elig, key=lambda k: classes[k]["ms"]) if elig else ",".join(ROOFLINE_CLASSES)
     # the dominant class timed on every ROOFLINE_SAMPLE-th launch across the timed region:
-    # an unbiased mean launch duration at an eighth of the event cost
+    # an unbiased mean launch duration at a fraction of the event cost (profile_enable
+    # resets the timing, which frees the warmup step's events: left alive, they slow
+    # every later step)
     F.profile_enable(dom)
-    F.profile_sample(ROOFLINE_SAMPLE)
+    F.profile_sample(sample)
     torch.cuda.synchronize()
     tot0 = _its_totals(sim)
     t0 = time.perf_counter()
@@ -381,6 +383,8 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-roofline", action="store_true", help="no kernel timing at all (A/B timing runs)")
+    ap.add_argument("--roofline-sample", type=int, default=ROOFLINE_SAMPLE,
+                    help="time every n-th launch of the roofline class in the timed region")
     ap.add_argument("--ensemble", action="store_true",
                     help="64 uneven channels, 50 steps: cell-updates/s looped vs concurrent (no headline line)")
     ap.add_argument("--ensemble-threads", type=int, default=8)
@@ -426,7 +430,7 @@ def main():
         print(json.dumps(run_vcycle_rate(F)))
         return
     sim64, t64, its64, dom, rec, classes = run_sim(F, torch, WORKLOAD, args.warmup, args.steps,
-                                                   not args.no_roofline)
+                                                   not args.no_roofline, args.roofline_sample)
     res = {
         "metric": "time-steps/s @512x96x96: FP64 vs FP32-MG at 1e-8 resid; stencil HBM GB/s",
         "value": args.steps / t64,
@@ -453,7 +457,7 @@ def main():
     }
     if not args.no_fp32:
         sim32, t32, its32, dom32, rec32, cls32 = run_sim(F, torch, dict(WORKLOAD, mg_precision=32), args.warmup,
-                                                         args.steps, not args.no_roofline)
+                                                         args.steps, not args.no_roofline, args.roofline_sample)
         errs = {}
         for f in ("u", "v", "w", "p"):
             a, b = sim64.get(f), sim32.get(f)

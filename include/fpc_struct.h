@@ -240,6 +240,8 @@ int fpc_ProfileEnable(const char *names);
 /* Time only every n-th launch of the enabled classes (n >= 1; 1 = every launch, the
  * default): the event cost then falls on 1/n of the launches. */
 int fpc_ProfileSample(int every);
+/* Drops the records and frees their HIP events (after the stream is synchronised):
+ * recorded events left alive slow the runtime's later launches and synchronisations. */
 int fpc_ProfileReset(void);
 int fpc_ProfileReport(char *buf, int buflen);
 
