@@ -295,13 +295,16 @@ def test_vcycle_graphs_replay_the_same_bits(torch_cuda, tiles_everywhere):
 
 
 def test_kernel_timing_sampled_launches(torch_cuda):
-    """fpc_ProfileSample(n): only every n-th launch of the enabled classes is timed."""
+    """fpc_ProfileSample(n): only every n-th launch of the enabled classes is timed.  Each
+    profile_enable resets the timing (records and their events freed), so the second
+    round starts from an empty report and times its own launches only."""
     kw = dict(TINY, init_profile=1, noise_amp=1e-3, seed=2)
     counts = []
     for n in (1, 4):
         s = F.Sim(F.default_params(**kw))
         s.step(1)
         F.profile_enable("multidot")
+        assert F.profile_report() == {}
         F.profile_sample(n)
         try:
             s.step(2)
